@@ -1,0 +1,233 @@
+/* minecu_hip.h - C ABI of libminecu_hip.so, the gfx950 (MI355X) batched
+ * voxel-environment step.
+ *
+ * The reference puts a pybind/libtorch module (src/bindings.cpp:985-1269) over
+ * `extern "C" void launch_*(raw pointers, ints, floats)` launchers
+ * (src/kernels.cu:2076-3245).  This header is the equivalent boundary: plain
+ * pointers, sizes and an explicit stream; no torch types.  Every function
+ * returns 0 on success or a negative MC_E* code; mc_last_error() gives the
+ * message of the calling thread's last failure.  All launches are
+ * asynchronous on `stream` (a hipStream_t; NULL = the null stream), and
+ * nothing here synchronises the device.
+ *
+ * Layouts (identical to the reference's tensors, minecu/__init__.py:214-256):
+ *   voxels      int8   [B, Y, X, Z]   block ids, AIR = -1
+ *   positions   f32    [B, 3]         x, y, z (feet)
+ *   velocities  f32    [B, 3]         only [:,1] is integrated
+ *   yaws, pitches f32  [B]
+ *   on_ground   bool   [B]  (1 byte)
+ *   step_count  int32  [B]
+ *   do_reset    bool   [B]
+ *   cameras     f32    [B, 5]         x, y, z, yaw, pitch
+ *   obs         uint8 or f32 [B, H, W, 3]
+ *   rewards     f32    [B]
+ *   buttons     int8   [B, 8]   look f32 [B, 2]   (multi-hot + radians)
+ *   actions     int64  [B]      discrete ids 0..10 (docs/environment.md:3-17)
+ */
+#ifndef MINECU_HIP_H
+#define MINECU_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MC_OK = 0,
+  MC_E_INVALID = -1,   /* bad argument (shape, null pointer, range) */
+  MC_E_HIP = -2,       /* a HIP runtime call failed */
+  MC_E_UNSUPPORTED = -3
+};
+
+/* Observation encodings (the reference's `obs_dtype`, minecu/__init__.py:167). */
+enum {
+  MC_OBS_U8 = 0,   /* shaded + fogged, truncated to uint8  (render_kernel_uint8, kernels.cu:300-470) */
+  MC_OBS_F32 = 1   /* shaded + fogged float32 in [0,1]     (render_kernel, kernels.cu:87-293) */
+};
+
+/* Action encodings accepted by the step. */
+enum {
+  MC_ACT_BUTTONS = 0,  /* int8 [B,8] multi-hot + f32 [B,2] look (minecu/__init__.py:304-331) */
+  MC_ACT_DISCRETE = 1  /* int64 [B] ids in [0,11); others act as no-op                        */
+};
+
+/* Static configuration of one env group: the MineEnv constructor arguments
+ * (minecu/__init__.py:141-165) plus derived spawn/tree/house coordinates
+ * (minecu/__init__.py:196-206). */
+typedef struct mc_config {
+  int batch_size;
+  int world_x, world_y, world_z;
+  int width, height;
+  int max_steps;
+  float view_distance;
+  float fov_degrees;
+  float dt, gravity, walk_speed, sprint_mult, jump_vel;
+  int target_block;        /* < 0: every block rewards */
+  float reward_value;
+  float spawn_x, spawn_y, spawn_z;
+  int ground_height;
+  unsigned int seed;       /* accepted and unused, as in the reference (kernels.cu:1800) */
+  int tree_x, tree_z, house_x, house_z;
+  int episode_length;      /* < 0: no episode limit (the reference's None) */
+  float look_delta;        /* radians per discrete look action */
+  int obs_mode;            /* MC_OBS_* */
+} mc_config;
+
+/* Device pointers of one env group's state (caller-owned, mutated in place). */
+typedef struct mc_state {
+  int8_t* voxels;
+  float* positions;
+  float* velocities;
+  float* yaws;
+  float* pitches;
+  uint8_t* on_ground;
+  int32_t* step_count;
+  uint8_t* do_reset;
+  float* cameras;
+  void* obs;
+  float* rewards;
+} mc_state;
+
+/* Minimum workspace (device bytes) for the stateless mc_step / mc_env_reset_state. */
+#define MC_WORKSPACE_BYTES 256
+
+const char* mc_last_error(void);
+int mc_version(void);
+
+/* ---- world generation -------------------------------------------------- */
+
+/* Replaces launch_generate_world (src/kernels.cu:2281-2291; binding
+ * bindings.cpp:357-371): flat layers by y, BEDROCK/DIRT/GRASS/AIR. */
+int mc_generate_world(int8_t* voxels, int batch_size, int world_x, int world_y, int world_z,
+                      int ground_height, unsigned int seed, void* stream);
+
+/* Replaces launch_place_tree (src/kernels.cu:2293-2302; bindings.cpp:373-388). */
+int mc_place_tree(int8_t* voxels, int batch_size, int world_x, int world_y, int world_z,
+                  int tree_x, int tree_z, int ground_height, void* stream);
+
+/* Replaces launch_place_house (src/kernels.cu:2304-2313; bindings.cpp:390-406). */
+int mc_place_house(int8_t* voxels, int batch_size, int world_x, int world_y, int world_z,
+                   int house_x, int house_z, int ground_height, void* stream);
+
+/* One pass writing generate_world + place_tree + place_house for the envs whose
+ * mask byte is set (mask NULL = all).  Replaces the three-launch regeneration
+ * MineEnv performs (minecu/__init__.py:258-268, 296-300, 439-442). */
+int mc_regen_worlds(const mc_config* cfg, int8_t* voxels, const uint8_t* mask, void* stream);
+
+/* ---- agent ------------------------------------------------------------- */
+
+/* Replaces launch_reset (src/kernels.cu:2268-2279; bindings.cpp:333-355). */
+int mc_reset(float* positions, float* velocities, float* yaws, float* pitches,
+             uint8_t* on_ground, const uint8_t* do_reset, int batch_size, float spawn_x,
+             float spawn_y, float spawn_z, void* stream);
+
+/* ---- render ------------------------------------------------------------ */
+
+/* Replaces launch_render_uint8 (src/kernels.cu:2094-2108; bindings.cpp:244-271)
+ * and launch_render (kernels.cu:2078-2092; bindings.cpp:215-242), selected by
+ * obs_mode.  cameras f32 [B,5]; world extents are X, Y, Z per axis. */
+int mc_render(const int8_t* voxels, const float* cameras, void* obs, int obs_mode,
+              int batch_size, int world_x, int world_y, int world_z, int width, int height,
+              int max_steps, float view_distance, float fov_degrees, void* stream);
+
+/* ---- step -------------------------------------------------------------- */
+
+/* One full environment step for every env of the group, fused into one
+ * gfx950 kernel after a one-workgroup episode pre-pass.  Replaces
+ * launch_step_uint8 (src/kernels.cu:2594-2652) / launch_step
+ * (kernels.cu:2497-2591) together with the Python bookkeeping around them
+ * (minecu/__init__.py:362-444: episode check, step_count update, regeneration
+ * of every world when any env is done).  `actions` is int8 [B,8] buttons with
+ * `look` f32 [B,2] (MC_ACT_BUTTONS) or int64 [B] ids (MC_ACT_DISCRETE, look
+ * ignored).  dones_out (bool [B], may be NULL) receives the done mask, as does
+ * st->do_reset.  workspace: >= MC_WORKSPACE_BYTES of device memory, private to
+ * this call until it completes on `stream`. */
+int mc_step(const mc_config* cfg, const mc_state* st, const void* actions, int action_kind,
+            const float* look, uint8_t* dones_out, void* workspace, void* stream);
+
+/* MineEnv.reset(mask) (minecu/__init__.py:270-302) in two launches: masked
+ * respawn, step_count = 0, regeneration of every world when mask is NULL or
+ * any mask byte is set, camera update and render. */
+int mc_env_reset_state(const mc_config* cfg, const mc_state* st, const uint8_t* mask,
+                       void* workspace, void* stream);
+
+/* Camera update + render of the current state (MineEnv._render,
+ * minecu/__init__.py:446-466). */
+int mc_render_state(const mc_config* cfg, const mc_state* st, void* stream);
+
+/* ---- reference-signature launchers ------------------------------------
+ * Same argument lists as the reference's extern "C" launchers (cubic
+ * world_size = voxels.size(1), bindings.cpp:231,546), plus a stream, so the
+ * reference's pybind layer could bind them unchanged.  The decode buffers
+ * (forward_in .. speed_mult) may all be NULL, in which case they are not
+ * written; when non-NULL they receive what decode_actions_kernel writes. */
+
+/* Replaces launch_step_uint8 (src/kernels.cu:2594-2652; binding
+ * bindings.cpp:492-575): decode, physics, raycast break/place, reset masked by
+ * do_reset, camera update, shaded uint8 render -- one fused kernel. */
+int mc_launch_step_uint8(int8_t* voxels, float* positions, float* velocities, float* yaws,
+                         float* pitches, uint8_t* on_ground, const int8_t* buttons,
+                         const float* look, const uint8_t* do_reset, float* forward_in,
+                         float* strafe_in, float* delta_yaw_in, float* delta_pitch_in,
+                         uint8_t* jump_in, uint8_t* do_break, uint8_t* do_place,
+                         float* speed_mult, float* cameras, uint8_t* obs_buffer, float* rewards,
+                         int batch_size, int world_size, int width, int height, int max_steps,
+                         float view_distance, float fov_degrees, float dt, float gravity,
+                         float walk_speed, float sprint_mult, float jump_vel, int8_t target_block,
+                         float reward_value, float spawn_x, float spawn_y, float spawn_z,
+                         void* stream);
+
+/* Replaces launch_step (src/kernels.cu:2497-2591; bindings.cpp:408-490), the
+ * float32-observation step.  Unlike the reference it does not synchronise and
+ * keeps no process-global graph (kernels.cu:2338-2382), so fresh action
+ * pointers are always honoured. */
+int mc_launch_step(int8_t* voxels, float* positions, float* velocities, float* yaws,
+                   float* pitches, uint8_t* on_ground, const int8_t* buttons, const float* look,
+                   const uint8_t* do_reset, float* forward_in, float* strafe_in,
+                   float* delta_yaw_in, float* delta_pitch_in, uint8_t* jump_in,
+                   uint8_t* do_break, uint8_t* do_place, float* speed_mult, float* cameras,
+                   float* obs_buffer, float* rewards, int batch_size, int world_size, int width,
+                   int height, int max_steps, float view_distance, float fov_degrees, float dt,
+                   float gravity, float walk_speed, float sprint_mult, float jump_vel,
+                   int8_t target_block, float reward_value, float spawn_x, float spawn_y,
+                   float spawn_z, void* stream);
+
+/* Replaces launch_render_uint8 (src/kernels.cu:2094-2108). */
+int mc_launch_render_uint8(const int8_t* voxels, const float* cameras, uint8_t* output,
+                           int batch_size, int world_size, int width, int height, int max_steps,
+                           float view_distance, float fov_degrees, void* stream);
+
+/* Replaces launch_render (src/kernels.cu:2078-2092). */
+int mc_launch_render(const int8_t* voxels, const float* cameras, float* output, int batch_size,
+                     int world_size, int width, int height, int max_steps, float view_distance,
+                     float fov_degrees, void* stream);
+
+/* ---- env handle (no reference counterpart: per-group state instead of the
+ * reference's process-global StepGraphState, kernels.cu:2338-2382) ------- */
+
+typedef struct mc_env mc_env;
+
+/* Validates cfg/st once, allocates the group's workspace (a 4-word episode
+ * flag ring: each step launch reads the "some env is done now" word left by
+ * the previous launch, so mc_env_step is a single kernel launch). */
+int mc_env_create(const mc_config* cfg, const mc_state* st, mc_env** out);
+/* Recomputes the episode flag from step_count (after the caller rewrites
+ * state, e.g. a checkpoint restore, or at start-up). */
+int mc_env_sync(mc_env* env, void* stream);
+int mc_env_destroy(mc_env* env);
+int mc_env_step(mc_env* env, const void* actions, int action_kind, const float* look,
+                uint8_t* dones_out, void* stream);
+int mc_env_reset(mc_env* env, const uint8_t* mask, void* stream);
+int mc_env_render(mc_env* env, void* stream);
+
+/* ---- test hooks -------------------------------------------------------- */
+
+/* Device evaluation of the shared sin/cos (mc_math.h) on n inputs. */
+int mc_debug_sincosf(const float* x, float* s, float* c, int n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MINECU_HIP_H */

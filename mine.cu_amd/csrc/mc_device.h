@@ -1,0 +1,832 @@
+// mc_device.h - gfx950 device code of the batched voxel-environment step.
+//
+// One workgroup owns one environment (its 64-lane waves share the env's world
+// staged in LDS).  Wave 0 runs the per-env logic -- episode check, action
+// decode, physics, the 32-sample break/place raycast (one sample per lane,
+// resolved with a wave ballot), masked respawn -- and publishes the camera
+// frame; then every lane traces rays through the LDS copy of the world and
+// writes the observation with wide coalesced stores.  World regeneration, when
+// any env of the group is done, is fused after the render.
+//
+// Arithmetic follows the reference operation by operation (src/kernels.cu,
+// cited per function) with FP contraction off and IEEE division/sqrt, so the
+// results are bit-identical to the scalar host restatement.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mc_math.h"
+
+namespace mc {
+
+constexpr int kWave = 64;
+constexpr int kMaxLdsWorld = 65536;   // bytes of world staged in LDS per workgroup
+
+// Block ids (src/kernels.cu:11-24).
+constexpr int8_t kAir = -1, kGrass = 0, kDirt = 1, kOakLog = 3, kLeaves = 4, kBrick = 8,
+                 kCobble = 9, kPlanks = 10, kBedrock = 12;
+
+// Shaded palette, 13 blocks x RGB then the sky (src/kernels.cu:27-44).  Copied
+// into LDS at kernel start so the per-pixel lookup is a ds_read.
+__device__ __forceinline__ float palette_entry(int i) {
+  switch (i) {
+    case 0: return 0.30f; case 1: return 0.65f; case 2: return 0.20f;    // grass
+    case 3: return 0.55f; case 4: return 0.35f; case 5: return 0.20f;    // dirt
+    case 6: return 0.50f; case 7: return 0.50f; case 8: return 0.50f;    // stone
+    case 9: return 0.55f; case 10: return 0.40f; case 11: return 0.25f;  // oak log
+    case 12: return 0.20f; case 13: return 0.55f; case 14: return 0.15f; // leaves
+    case 15: return 0.90f; case 16: return 0.85f; case 17: return 0.60f; // sand
+    case 18: return 0.20f; case 19: return 0.40f; case 20: return 0.80f; // water
+    case 21: return 0.85f; case 22: return 0.90f; case 23: return 0.95f; // glass
+    case 24: return 0.70f; case 25: return 0.35f; case 26: return 0.30f; // brick
+    case 27: return 0.40f; case 28: return 0.40f; case 29: return 0.40f; // cobblestone
+    case 30: return 0.75f; case 31: return 0.60f; case 32: return 0.40f; // planks
+    case 33: return 0.95f; case 34: return 0.95f; case 35: return 0.98f; // snow
+    case 36: return 0.15f; case 37: return 0.15f; case 38: return 0.15f; // bedrock
+    case 39: return 0.53f; case 40: return 0.81f; default: return 0.92f; // sky
+  }
+}
+constexpr int kPaletteLen = 42;
+constexpr int kSkyBase = 39;
+
+// Everything one launch needs; passed by value as the kernel argument.
+struct Params {
+  int B, X, Y, Z, W, H;
+  int max_steps;
+  float view_distance, half_fov, aspect;
+  float dt, gravity, walk_speed, sprint_mult, jump_vel;
+  int target_block;
+  float reward_value;
+  float spawn_x, spawn_y, spawn_z;
+  int ground_height, tree_x, tree_z, house_x, house_z;
+  int episode_length;          // < 0: none
+  float look_delta;
+  int action_kind;             // 0 buttons+look, 1 discrete int64
+  const void* actions;
+  const float* look;
+  int8_t* voxels;
+  float* positions;
+  float* velocities;
+  float* yaws;
+  float* pitches;
+  uint8_t* on_ground;
+  int32_t* step_count;
+  uint8_t* do_reset;
+  uint8_t* dones;
+  float* cameras;
+  void* obs;
+  float* rewards;
+  const uint8_t* mask;         // reset: NULL = all
+  // Episode flag ring (4 device words, NULL: no episode logic).  A step
+  // launch reads ring[slot] ("some env is done this step", which decides the
+  // regeneration of every world), sets ring[(slot+1)%3] when some env will be
+  // done on the next step, and clears ring[(slot+2)%3] for the launch after.
+  // ring[3] is the reset path's "regenerate" flag.
+  uint32_t* ring;
+  int slot;
+  // done_mode 0: done = step_count >= episode_length, with the MineEnv
+  // bookkeeping (step_count, do_reset, dones, regeneration).  done_mode 1:
+  // done = do_reset[b] as given, nothing else touched (launch_step_uint8).
+  int done_mode;
+  // Optional decoded-control outputs ([B] each; NULL = not written), the
+  // reference's decode buffers (src/kernels.cu:1459-1504).
+  float* forward_in;
+  float* strafe_in;
+  float* delta_yaw_in;
+  float* delta_pitch_in;
+  uint8_t* jump_in;
+  uint8_t* do_break;
+  uint8_t* do_place;
+  float* speed_mult_in;
+};
+
+// Camera frame of one env, published by wave 0 through LDS.
+struct Frame {
+  float ex, ey, ez;            // eye
+  float fx, fy, fz;            // forward
+  float rx, rz;                // right (right_y == 0)
+  float ux, uy, uz;            // up
+  int fill_block;              // >= 0: eye inside this solid block, every ray hits it at t=0
+};
+
+// ---------------------------------------------------------------------------
+// World regeneration, closed form of generate_world_kernel + place_tree_kernel
+// + place_house_kernel applied in that order (src/kernels.cu:1793-1946,
+// minecu/__init__.py:258-268).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int8_t layer_block(int y, int gh) {
+  return (y == 0) ? kBedrock : (y < gh - 1) ? kDirt : (y == gh - 1) ? kGrass : kAir;
+}
+
+// Value the house writes at offset (dy, dx, dz) of its 5x5x4 box (dy, dx, dz),
+// or kNoWrite where it writes nothing (interior, door).
+constexpr int8_t kNoWrite = -2;
+__device__ __forceinline__ int8_t house_block(int dy, int dx, int dz) {
+  if (dy == 0) return kCobble;                         // floor
+  if (dy == 4) return kBrick;                          // roof
+  const bool xe = (dx == 0 || dx == 4), ze = (dz == 0 || dz == 3);
+  if (xe && ze) return kOakLog;                        // corner pillars (written last)
+  if (dz == 0) return (dx == 2 && dy <= 2) ? kNoWrite : kPlanks;   // front wall, door
+  if (dz == 3 || xe) return kPlanks;                   // back / side walls
+  return kNoWrite;                                     // interior
+}
+
+__device__ __forceinline__ int8_t fresh_voxel(const Params& p, int y, int x, int z) {
+  int8_t v = layer_block(y, p.ground_height);
+  if (x == p.tree_x && z == p.tree_z) {
+    const int dy = y - p.ground_height;
+    if (dy >= 0 && dy < 3) v = kOakLog;
+    else if (dy == 3) v = kLeaves;
+  }
+  const int hx = x - p.house_x, hy = y - p.ground_height, hz = z - p.house_z;
+  if (hx >= 0 && hx < 5 && hz >= 0 && hz < 4 && hy >= 0 && hy <= 4) {
+    const int8_t h = house_block(hy, hx, hz);
+    if (h != kNoWrite) v = h;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t pack4(int8_t a, int8_t b, int8_t c, int8_t d) {
+  return (uint32_t)(uint8_t)a | ((uint32_t)(uint8_t)b << 8) | ((uint32_t)(uint8_t)c << 16) |
+         ((uint32_t)(uint8_t)d << 24);
+}
+
+// 16 consecutive voxels starting at linear index i0 (i0 % 16 == 0), packed
+// little-endian into a uint4.  Every index below is a compile-time constant
+// after unrolling, so the chunk stays in registers.
+__device__ __forceinline__ uint4 fresh_chunk(const Params& p, int i0) {
+  const int XZ = p.X * p.Z;
+  uint32_t w[4] = {0u, 0u, 0u, 0u};
+  if ((p.Z & 15) == 0) {
+    // The chunk lies in one (y, x) row: uniform layer, then tree/house fix-ups.
+    const int y = i0 / XZ, rem = i0 - y * XZ, x = rem / p.Z, z0 = rem - x * p.Z;
+    const int8_t base = layer_block(y, p.ground_height);
+    const int dy = y - p.ground_height;
+    const bool tree_row = (x == p.tree_x) && dy >= 0 && dy <= 3;
+    const int8_t tree_v = (dy < 3) ? kOakLog : kLeaves;
+    const int hx = x - p.house_x;
+    const bool house_row = hx >= 0 && hx < 5 && dy >= 0 && dy <= 4;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int z = z0 + k;
+      int8_t v = base;
+      if (tree_row && z == p.tree_z) v = tree_v;
+      const int hz = z - p.house_z;
+      if (house_row && hz >= 0 && hz < 4) {
+        const int8_t h = house_block(dy, hx, hz);
+        if (h != kNoWrite) v = h;
+      }
+      w[k >> 2] |= (uint32_t)(uint8_t)v << (8 * (k & 3));
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int i = i0 + k;
+      const int y = i / XZ, rem = i - y * XZ, x = rem / p.Z, z = rem - x * p.Z;
+      w[k >> 2] |= (uint32_t)(uint8_t)fresh_voxel(p, y, x, z) << (8 * (k & 3));
+    }
+  }
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Write env b's fresh world to HBM (and to LDS when lds != nullptr).
+template <int NT>
+__device__ __forceinline__ void store_fresh_world(const Params& p, int8_t* gworld, int8_t* lds,
+                                                  int nvox, int tid) {
+  if ((nvox & 15) == 0) {
+    uint4* g = reinterpret_cast<uint4*>(gworld);
+    uint4* l = reinterpret_cast<uint4*>(lds);
+    for (int c = tid; c < (nvox >> 4); c += NT) {
+      const uint4 v = fresh_chunk(p, c << 4);
+      g[c] = v;
+      if (lds) l[c] = v;
+    }
+  } else {
+    const int XZ = p.X * p.Z;
+    for (int i = tid; i < nvox; i += NT) {
+      const int y = i / XZ, rem = i - y * XZ, x = rem / p.Z, z = rem - x * p.Z;
+      const int8_t v = fresh_voxel(p, y, x, z);
+      gworld[i] = v;
+      if (lds) lds[i] = v;
+    }
+  }
+}
+
+// HBM -> LDS copy of one env's world.
+template <int NT>
+__device__ __forceinline__ void stage_world(int8_t* lds, const int8_t* gworld, int nvox, int tid) {
+  if ((nvox & 15) == 0) {
+    const uint4* g = reinterpret_cast<const uint4*>(gworld);
+    uint4* l = reinterpret_cast<uint4*>(lds);
+    for (int c = tid; c < (nvox >> 4); c += NT) l[c] = g[c];
+  } else {
+    for (int i = tid; i < nvox; i += NT) lds[i] = gworld[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Camera frame (src/kernels.cu:326-345 eye + basis), plus the inside-solid
+// shortcut: with the eye strictly inside the world box, every ray passes the
+// AABB test with t_entry < 0, and with max_steps >= 1 and view_distance > 0 the
+// first DDA iteration (t = 0, face -1) hits the eye voxel when it is solid; the
+// shaded colour is then col * 1 * (1 - 0) + sky * 0 == col for every pixel.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ Frame make_frame(const Params& p, float x, float y, float z,
+                                            float sin_yaw, float cos_yaw, float sin_pitch,
+                                            float cos_pitch, const int8_t* world) {
+  Frame f;
+  f.ex = x;
+  f.ey = y + 1.6f;
+  f.ez = z;
+  f.fx = sin_yaw * cos_pitch;
+  f.fy = sin_pitch;
+  f.fz = cos_yaw * cos_pitch;
+  f.rx = cos_yaw;
+  f.rz = -sin_yaw;
+  f.ux = -sin_yaw * sin_pitch;
+  f.uy = cos_pitch;
+  f.uz = -cos_yaw * sin_pitch;
+  f.fill_block = -1;
+  if (p.max_steps >= 1 && p.view_distance > 0.0f && f.ex > 0.0f && f.ex < (float)p.X &&
+      f.ey > 0.0f && f.ey < (float)p.Y && f.ez > 0.0f && f.ez < (float)p.Z) {
+    const int vx = (int)floorf(f.ex), vy = (int)floorf(f.ey), vz = (int)floorf(f.ez);
+    const int8_t blk = world[vy * p.X * p.Z + vx * p.Z + vz];
+    if (blk >= 0 && blk < 13) f.fill_block = blk;
+  }
+  return f;
+}
+
+// ---------------------------------------------------------------------------
+// One ray: AABB cull + DDA + face shade + fog (src/kernels.cu:51-79, 354-469).
+// Writes the shaded colour in [0,1] to rgb.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, float v,
+                                      const int8_t* __restrict__ world,
+                                      const float* __restrict__ pal, float rgb[3]) {
+  float ray_x = f.fx + u * f.rx + v * f.ux;
+  float ray_y = f.fy + v * f.uy;
+  float ray_z = f.fz + u * f.rz + v * f.uz;
+  const float ray_len = sqrtf(ray_x * ray_x + ray_y * ray_y + ray_z * ray_z);
+  ray_x /= ray_len;
+  ray_y /= ray_len;
+  ray_z /= ray_len;
+  // One reciprocal per axis serves both the slab test and the DDA: the
+  // reference computes the same expression twice (kernels.cu:57-59, 395-397).
+  const float inv_x = (ray_x != 0.0f) ? 1.0f / ray_x : 1e30f;
+  const float inv_y = (ray_y != 0.0f) ? 1.0f / ray_y : 1e30f;
+  const float inv_z = (ray_z != 0.0f) ? 1.0f / ray_z : 1e30f;
+  const float t1 = (0.0f - f.ex) * inv_x, t2 = ((float)p.X - f.ex) * inv_x;
+  const float t3 = (0.0f - f.ey) * inv_y, t4 = ((float)p.Y - f.ey) * inv_y;
+  const float t5 = (0.0f - f.ez) * inv_z, t6 = ((float)p.Z - f.ez) * inv_z;
+  const float t_entry = fmaxf(fmaxf(fminf(t1, t2), fminf(t3, t4)), fminf(t5, t6));
+  const float t_exit = fminf(fminf(fmaxf(t1, t2), fmaxf(t3, t4)), fmaxf(t5, t6));
+  const float vd = p.view_distance;
+  int hit_block = -1;
+  int hit_face = -1;
+  float t = 0.0f;
+  int step_y = (ray_y >= 0) ? 1 : -1;
+  if (!(t_exit < 0 || t_entry > t_exit) && !(t_entry > vd)) {
+    int vx = (int)floorf(f.ex), vy = (int)floorf(f.ey), vz = (int)floorf(f.ez);
+    const int step_x = (ray_x >= 0) ? 1 : -1;
+    const int step_z = (ray_z >= 0) ? 1 : -1;
+    const float td_x = fabsf(inv_x), td_y = fabsf(inv_y), td_z = fabsf(inv_z);
+    float tm_x = (ray_x >= 0) ? ((float)(vx + 1) - f.ex) * inv_x : ((float)vx - f.ex) * inv_x;
+    float tm_y = (ray_y >= 0) ? ((float)(vy + 1) - f.ey) * inv_y : ((float)vy - f.ey) * inv_y;
+    float tm_z = (ray_z >= 0) ? ((float)(vz + 1) - f.ez) * inv_z : ((float)vz - f.ez) * inv_z;
+    const int XZ = p.X * p.Z;
+    for (int step = 0; !(step >= p.max_steps || t >= vd); ++step) {
+      if ((unsigned)vx < (unsigned)p.X && (unsigned)vy < (unsigned)p.Y &&
+          (unsigned)vz < (unsigned)p.Z) {
+        const int8_t blk = world[vy * XZ + vx * p.Z + vz];
+        if (blk >= 0) {
+          hit_block = blk;
+          break;
+        }
+      } else if (t > t_exit) {
+        break;
+      }
+      if (tm_x < tm_y && tm_x < tm_z) {
+        t = tm_x; tm_x += td_x; vx += step_x; hit_face = 0;
+      } else if (tm_y < tm_z) {
+        t = tm_y; tm_y += td_y; vy += step_y; hit_face = 1;
+      } else {
+        t = tm_z; tm_z += td_z; vz += step_z; hit_face = 2;
+      }
+    }
+  }
+  if (hit_block >= 0 && hit_block < 13) {
+    float shade = 1.0f;
+    if (hit_face == 0) shade = 0.8f;
+    else if (hit_face == 2) shade = 0.9f;
+    else if (hit_face == 1 && step_y < 0) shade = 0.6f;
+    float fog = fminf(t / vd, 1.0f);
+    fog = fog * fog;
+    const float keep = 1.0f - fog;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rgb[c] = pal[hit_block * 3 + c] * shade * keep + pal[kSkyBase + c] * fog;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rgb[c] = pal[kSkyBase + c];
+  }
+}
+
+__device__ __forceinline__ uint32_t to_u8(float c) { return (uint32_t)(c * 255.0f) & 0xffu; }
+
+// ---------------------------------------------------------------------------
+// Render every pixel of env b into obs (layout [B,H,W,3]).
+// OBS: 0 = uint8, 1 = float32.
+// ---------------------------------------------------------------------------
+template <int OBS, int NT>
+__device__ __forceinline__ void render_env(const Params& p, int b, const int8_t* world,
+                                           const float* pal, const Frame& f, int tid) {
+  const int W = p.W, H = p.H, npix = W * H;
+  const float hf = p.half_fov, aspect = p.aspect;
+  if (OBS == 0) {
+    uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
+    if (f.fill_block >= 0 && ((npix * 3) & 15) == 0) {
+      // Constant colour: three 16-byte phases of the repeating RGB pattern.
+      const uint32_t r = to_u8(pal[f.fill_block * 3]), g = to_u8(pal[f.fill_block * 3 + 1]),
+                     bl = to_u8(pal[f.fill_block * 3 + 2]);
+      const uint32_t w0 = r | (g << 8) | (bl << 16) | (r << 24);
+      const uint32_t w1 = g | (bl << 8) | (r << 16) | (g << 24);
+      const uint32_t w2 = bl | (r << 8) | (g << 16) | (bl << 24);
+      // Chunk c starts at byte 16c, i.e. at channel phase (16c mod 3) = c mod 3.
+      uint4* o4 = reinterpret_cast<uint4*>(out);
+      for (int c = tid; c < (npix * 3) >> 4; c += NT) {
+        const int ph = c % 3;
+        const uint32_t a = (ph == 0) ? w0 : (ph == 1) ? w1 : w2;
+        const uint32_t bq = (ph == 0) ? w1 : (ph == 1) ? w2 : w0;
+        const uint32_t cq = (ph == 0) ? w2 : (ph == 1) ? w0 : w1;
+        o4[c] = make_uint4(a, bq, cq, a);
+      }
+      return;
+    }
+    if ((W & 3) == 0) {
+      for (int q = tid; q < (npix >> 2); q += NT) {
+        const int p0 = q << 2, py = p0 / W, px0 = p0 - py * W;
+        const float v = (1.0f - 2.0f * (py + 0.5f) / H) * hf;
+        // Packed 0x00BBGGRR per pixel.
+        auto shade = [&](int j) -> uint32_t {
+          const float u = (2.0f * (px0 + j + 0.5f) / W - 1.0f) * hf * aspect;
+          float rgb[3];
+          trace(p, f, u, v, world, pal, rgb);
+          return to_u8(rgb[0]) | (to_u8(rgb[1]) << 8) | (to_u8(rgb[2]) << 16);
+        };
+        const uint32_t c0 = shade(0), c1 = shade(1), c2 = shade(2), c3 = shade(3);
+        uint32_t* o = reinterpret_cast<uint32_t*>(out + (int64_t)p0 * 3);
+        o[0] = c0 | (c1 << 24);
+        o[1] = (c1 >> 8) | (c2 << 16);
+        o[2] = (c2 >> 16) | (c3 << 8);
+      }
+    } else {
+      for (int pi = tid; pi < npix; pi += NT) {
+        const int py = pi / W, px = pi - py * W;
+        const float v = (1.0f - 2.0f * (py + 0.5f) / H) * hf;
+        const float u = (2.0f * (px + 0.5f) / W - 1.0f) * hf * aspect;
+        float rgb[3];
+        if (f.fill_block >= 0) {
+          for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
+        } else {
+          trace(p, f, u, v, world, pal, rgb);
+        }
+        for (int c = 0; c < 3; ++c) out[(int64_t)pi * 3 + c] = (uint8_t)to_u8(rgb[c]);
+      }
+    }
+  } else {
+    float* out = reinterpret_cast<float*>(p.obs) + (int64_t)b * npix * 3;
+    if ((W & 3) == 0) {
+      for (int q = tid; q < (npix >> 2); q += NT) {
+        const int p0 = q << 2, py = p0 / W, px0 = p0 - py * W;
+        const float v = (1.0f - 2.0f * (py + 0.5f) / H) * hf;
+        float vals[12];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float u = (2.0f * (px0 + j + 0.5f) / W - 1.0f) * hf * aspect;
+          if (f.fill_block >= 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) vals[j * 3 + c] = pal[f.fill_block * 3 + c];
+          } else {
+            trace(p, f, u, v, world, pal, &vals[j * 3]);
+          }
+        }
+        float4* o = reinterpret_cast<float4*>(out + (int64_t)p0 * 3);
+        o[0] = make_float4(vals[0], vals[1], vals[2], vals[3]);
+        o[1] = make_float4(vals[4], vals[5], vals[6], vals[7]);
+        o[2] = make_float4(vals[8], vals[9], vals[10], vals[11]);
+      }
+    } else {
+      for (int pi = tid; pi < npix; pi += NT) {
+        const int py = pi / W, px = pi - py * W;
+        const float v = (1.0f - 2.0f * (py + 0.5f) / H) * hf;
+        const float u = (2.0f * (px + 0.5f) / W - 1.0f) * hf * aspect;
+        float rgb[3];
+        if (f.fill_block >= 0) {
+          for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
+        } else {
+          trace(p, f, u, v, world, pal, rgb);
+        }
+        for (int c = 0; c < 3; ++c) out[(int64_t)pi * 3 + c] = rgb[c];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-env step logic, executed by the 64 lanes of wave 0 (values uniform
+// across lanes except in the raycast).  Order: episode check
+// (minecu/__init__.py:363-366), decode (kernels.cu:1477-1503), physics
+// (1536-1583), raycast break/place (1609-1679), masked reset (1775-1785),
+// camera (2330-2334), step_count update (minecu/__init__.py:435-436).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ Frame env_update(const Params& p, int b, int8_t* world,
+                                            int8_t* gworld, bool any_done) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int sc = (p.done_mode == 0) ? p.step_count[b] : 0;
+  const bool done = (p.done_mode == 0) ? ((p.episode_length >= 0) && (sc >= p.episode_length))
+                                       : (p.do_reset[b] != 0);
+
+  // Decode.
+  int8_t bt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  float look_yaw = 0.0f, look_pitch = 0.0f;
+  if (p.action_kind == 1) {
+    const int64_t a = reinterpret_cast<const int64_t*>(p.actions)[b];
+    if (a >= 1 && a <= 6) bt[a - 1] = 1;
+    else if (a == 7) look_yaw = p.look_delta;
+    else if (a == 8) look_yaw = -p.look_delta;
+    else if (a == 9) look_pitch = p.look_delta;
+    else if (a == 10) look_pitch = -p.look_delta;
+  } else {
+    const int8_t* src = reinterpret_cast<const int8_t*>(p.actions) + (int64_t)b * 8;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) bt[k] = src[k];
+    look_yaw = p.look[b * 2 + 0];
+    look_pitch = p.look[b * 2 + 1];
+  }
+  const float fwd = (float)bt[0] - (float)bt[1];
+  const float str = (float)bt[3] - (float)bt[2];
+  const bool jump = bt[4] != 0, brk = bt[5] != 0, place = bt[6] != 0;
+  const float speed_mult = (bt[0] && bt[7]) ? p.sprint_mult : 1.0f;
+
+  // Physics.
+  float yaw = p.yaws[b] + look_yaw;
+  const float new_pitch = p.pitches[b] + look_pitch;
+  float pitch = fminf(fmaxf(new_pitch, -1.56f), 1.56f);
+  float sin_yaw, cos_yaw;
+  mc_sincosf(yaw, &sin_yaw, &cos_yaw);
+  const float speed = p.walk_speed * speed_mult;
+  const float move_x = sin_yaw * fwd * speed * p.dt + cos_yaw * str * p.walk_speed * p.dt;
+  const float move_z = cos_yaw * fwd * speed * p.dt - sin_yaw * str * p.walk_speed * p.dt;
+  float vel_y = p.velocities[b * 3 + 1];
+  bool og = p.on_ground[b] != 0;
+  if (jump && og) vel_y = p.jump_vel;
+  vel_y += p.gravity * p.dt;
+  float nx = p.positions[b * 3 + 0] + move_x;
+  float ny = p.positions[b * 3 + 1] + vel_y * p.dt;
+  float nz = p.positions[b * 3 + 2] + move_z;
+  const float ground_y = (float)(p.Y / 4);
+  if (ny <= ground_y) {
+    ny = ground_y;
+    vel_y = 0.0f;
+    og = true;
+  } else {
+    og = false;
+  }
+  nx = fminf(fmaxf(nx, 1.0f), (float)p.X - 1.0f);
+  nz = fminf(fmaxf(nz, 1.0f), (float)p.Z - 1.0f);
+
+  // Raycast: sample k of 32 on lane k, first solid in-bounds sample by ballot.
+  float reward = 0.0f;
+  float sin_pitch = 0.0f, cos_pitch = 1.0f;
+  bool have_pitch = false;
+  if (brk || place) {
+    mc_sincosf(pitch, &sin_pitch, &cos_pitch);
+    have_pitch = true;
+    const float dx = sin_yaw * cos_pitch, dy = sin_pitch, dz = cos_yaw * cos_pitch;
+    const float ex = nx, ey = ny + 1.6f, ez = nz;
+    bool inb = false, solid = false;
+    int idx = 0;
+    int8_t blk = kAir;
+    if (lane < 32) {
+      const float t = 0.1f + (float)lane * (4.0f / 32.0f);
+      const int vx = (int)floorf(ex + dx * t);
+      const int vy = (int)floorf(ey + dy * t);
+      const int vz = (int)floorf(ez + dz * t);
+      inb = vx >= 0 && vx < p.X && vy >= 0 && vy < p.Y && vz >= 0 && vz < p.Z;
+      if (inb) {
+        idx = vy * p.X * p.Z + vx * p.Z + vz;
+        blk = world[idx];
+        solid = blk >= 0;
+      }
+    }
+    const uint64_t hit = __ballot(solid);
+    const uint64_t inb_mask = __ballot(inb);
+    if (hit) {
+      const int k = __ffsll((unsigned long long)hit) - 1;
+      const int8_t hit_blk = (int8_t)__shfl((int)blk, k);
+      if (brk) {
+        if (p.target_block < 0 || (int)hit_blk == p.target_block) reward = p.reward_value;
+        if (lane == k) {
+          world[idx] = kAir;
+          if (!any_done) gworld[idx] = kAir;
+        }
+      } else {
+        const uint64_t before = inb_mask & ((1ull << k) - 1ull);
+        if (before) {
+          const int pl = 63 - __clzll((long long)before);
+          if (lane == pl) {
+            world[idx] = kDirt;
+            if (!any_done) gworld[idx] = kDirt;
+          }
+        }
+      }
+    }
+  }
+
+  // Masked respawn (after physics and raycast, as launch_step_uint8 orders it).
+  if (done) {
+    nx = p.spawn_x; ny = p.spawn_y; nz = p.spawn_z;
+    vel_y = 0.0f;
+    yaw = 0.0f;
+    pitch = 0.0f;
+    og = false;
+  }
+  if (lane == 0) {
+    p.positions[b * 3 + 0] = nx;
+    p.positions[b * 3 + 1] = ny;
+    p.positions[b * 3 + 2] = nz;
+    p.velocities[b * 3 + 1] = vel_y;
+    if (done) {
+      p.velocities[b * 3 + 0] = 0.0f;
+      p.velocities[b * 3 + 2] = 0.0f;
+    }
+    p.yaws[b] = yaw;
+    p.pitches[b] = pitch;
+    p.on_ground[b] = og ? 1 : 0;
+    p.rewards[b] = reward;
+    if (p.done_mode == 0) {
+      const int32_t nsc = done ? 1 : (int32_t)((uint32_t)sc + 1u);
+      p.step_count[b] = nsc;
+      p.do_reset[b] = done ? 1 : 0;
+      if (p.dones) p.dones[b] = done ? 1 : 0;
+      // Idempotent 0 -> 1 store: racing workgroups all write the same value.
+      if (p.ring && p.episode_length >= 0 && nsc >= p.episode_length)
+        __hip_atomic_store(&p.ring[(p.slot + 1) % 3], 1u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (p.forward_in) {
+      p.forward_in[b] = fwd;
+      p.strafe_in[b] = str;
+      p.delta_yaw_in[b] = look_yaw;
+      p.delta_pitch_in[b] = look_pitch;
+      p.jump_in[b] = jump ? 1 : 0;
+      p.do_break[b] = brk ? 1 : 0;
+      p.do_place[b] = place ? 1 : 0;
+      p.speed_mult_in[b] = speed_mult;
+    }
+    p.cameras[b * 5 + 0] = nx;
+    p.cameras[b * 5 + 1] = ny;
+    p.cameras[b * 5 + 2] = nz;
+    p.cameras[b * 5 + 3] = yaw;
+    p.cameras[b * 5 + 4] = pitch;
+  }
+  // Camera basis from the post-reset orientation (yaw = pitch = 0 after a
+  // respawn, where sin/cos are exactly 0 and 1).
+  if (done) {
+    sin_yaw = 0.0f; cos_yaw = 1.0f; sin_pitch = 0.0f; cos_pitch = 1.0f;
+  } else if (!have_pitch) {
+    mc_sincosf(pitch, &sin_pitch, &cos_pitch);
+  }
+  return make_frame(p, nx, ny, nz, sin_yaw, cos_yaw, sin_pitch, cos_pitch, world);
+}
+
+__device__ __forceinline__ void load_palette(float* pal, int tid) {
+  if (tid < kPaletteLen) pal[tid] = palette_entry(tid);
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Fused env step; one workgroup (NT threads) per env.
+template <int OBS, int NT>
+__global__ __launch_bounds__(NT) void k_step(Params p) {
+  extern __shared__ __align__(16) int8_t s_world[];
+  __shared__ float s_pal[kPaletteLen];
+  __shared__ Frame s_frame;
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int nvox = p.X * p.Y * p.Z;
+  int8_t* gworld = p.voxels + (int64_t)b * nvox;
+  load_palette(s_pal, tid);
+  stage_world<NT>(s_world, gworld, nvox, tid);
+  const bool any_done = (p.done_mode == 0 && p.ring) ? (p.ring[p.slot] != 0u) : false;
+  if (p.done_mode == 0 && p.ring && b == 0 && tid == 0) p.ring[(p.slot + 2) % 3] = 0u;
+  __syncthreads();
+  if (tid < kWave) {
+    const Frame f = env_update(p, b, s_world, gworld, any_done);
+    if (tid == 0) s_frame = f;
+  }
+  __syncthreads();
+  const Frame f = s_frame;
+  render_env<OBS, NT>(p, b, s_world, s_pal, f, tid);
+  if (any_done) store_fresh_world<NT>(p, gworld, nullptr, nvox, tid);
+}
+
+// MineEnv.reset: masked respawn, step_count = 0, regeneration when flagged,
+// camera update, render.  One workgroup per env.
+template <int OBS, int NT>
+__global__ __launch_bounds__(NT) void k_reset_render(Params p) {
+  extern __shared__ __align__(16) int8_t s_world[];
+  __shared__ float s_pal[kPaletteLen];
+  __shared__ Frame s_frame;
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int nvox = p.X * p.Y * p.Z;
+  int8_t* gworld = p.voxels + (int64_t)b * nvox;
+  load_palette(s_pal, tid);
+  const bool regen = p.ring[3] != 0u;
+  if (regen) store_fresh_world<NT>(p, gworld, s_world, nvox, tid);
+  else stage_world<NT>(s_world, gworld, nvox, tid);
+  __syncthreads();
+  if (tid < kWave) {
+    const bool r = p.mask ? (p.mask[b] != 0) : true;
+    float x = p.positions[b * 3 + 0], y = p.positions[b * 3 + 1], z = p.positions[b * 3 + 2];
+    float yaw = p.yaws[b], pitch = p.pitches[b];
+    if (r) {
+      x = p.spawn_x; y = p.spawn_y; z = p.spawn_z;
+      yaw = 0.0f; pitch = 0.0f;
+    }
+    if (tid == 0) {
+      p.do_reset[b] = r ? 1 : 0;
+      if (r) {
+        p.positions[b * 3 + 0] = x;
+        p.positions[b * 3 + 1] = y;
+        p.positions[b * 3 + 2] = z;
+        p.velocities[b * 3 + 0] = 0.0f;
+        p.velocities[b * 3 + 1] = 0.0f;
+        p.velocities[b * 3 + 2] = 0.0f;
+        p.yaws[b] = 0.0f;
+        p.pitches[b] = 0.0f;
+        p.on_ground[b] = 0;
+        p.step_count[b] = 0;
+      }
+      const int32_t sc = r ? 0 : p.step_count[b];
+      if (p.episode_length >= 0 && sc >= p.episode_length)
+        __hip_atomic_store(&p.ring[p.slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      p.cameras[b * 5 + 0] = x;
+      p.cameras[b * 5 + 1] = y;
+      p.cameras[b * 5 + 2] = z;
+      p.cameras[b * 5 + 3] = yaw;
+      p.cameras[b * 5 + 4] = pitch;
+    }
+    float sy, cy, sp, cp;
+    mc_sincosf(yaw, &sy, &cy);
+    mc_sincosf(pitch, &sp, &cp);
+    const Frame f = make_frame(p, x, y, z, sy, cy, sp, cp, s_world);
+    if (tid == 0) s_frame = f;
+  }
+  __syncthreads();
+  const Frame f = s_frame;
+  render_env<OBS, NT>(p, b, s_world, s_pal, f, tid);
+}
+
+// Render from cameras [B,5] (from_state = 0) or from the agent state, which
+// also refreshes cameras (from_state = 1, MineEnv._render).
+template <int OBS, int NT>
+__global__ __launch_bounds__(NT) void k_render(Params p, int from_state) {
+  extern __shared__ __align__(16) int8_t s_world[];
+  __shared__ float s_pal[kPaletteLen];
+  __shared__ Frame s_frame;
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int nvox = p.X * p.Y * p.Z;
+  const int8_t* gworld = p.voxels + (int64_t)b * nvox;
+  load_palette(s_pal, tid);
+  stage_world<NT>(s_world, gworld, nvox, tid);
+  __syncthreads();
+  if (tid < kWave) {
+    float x, y, z, yaw, pitch;
+    if (from_state) {
+      x = p.positions[b * 3 + 0]; y = p.positions[b * 3 + 1]; z = p.positions[b * 3 + 2];
+      yaw = p.yaws[b]; pitch = p.pitches[b];
+      if (tid == 0) {
+        p.cameras[b * 5 + 0] = x;
+        p.cameras[b * 5 + 1] = y;
+        p.cameras[b * 5 + 2] = z;
+        p.cameras[b * 5 + 3] = yaw;
+        p.cameras[b * 5 + 4] = pitch;
+      }
+    } else {
+      x = p.cameras[b * 5 + 0]; y = p.cameras[b * 5 + 1]; z = p.cameras[b * 5 + 2];
+      yaw = p.cameras[b * 5 + 3]; pitch = p.cameras[b * 5 + 4];
+    }
+    float sy, cy, sp, cp;
+    mc_sincosf(yaw, &sy, &cy);
+    mc_sincosf(pitch, &sp, &cp);
+    const Frame f = make_frame(p, x, y, z, sy, cy, sp, cp, s_world);
+    if (tid == 0) s_frame = f;
+  }
+  __syncthreads();
+  const Frame f = s_frame;
+  render_env<OBS, NT>(p, b, s_world, s_pal, f, tid);
+}
+
+// Episode-ring set-up, one workgroup of 1024 threads.
+//   mode 0: ring[slot] = any(step_count >= L), ring[(slot+1)%3] = 0
+//   mode 1 (before a reset): ring[3] = any(mask) (1 when mask is NULL),
+//           ring[slot] = ring[(slot+1)%3] = 0
+__global__ __launch_bounds__(1024) void k_ring_setup(Params p, int mode) {
+  uint32_t any = 0;
+  if (mode == 0) {
+    if (p.episode_length >= 0)
+      for (int i = threadIdx.x; i < p.B; i += 1024) any |= (p.step_count[i] >= p.episode_length);
+  } else {
+    if (!p.mask) any = 1;
+    else
+      for (int i = threadIdx.x; i < p.B; i += 1024) any |= (p.mask[i] != 0);
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) {
+    if (mode == 0) {
+      p.ring[p.slot] = any ? 1u : 0u;
+    } else {
+      p.ring[3] = any ? 1u : 0u;
+      p.ring[p.slot] = 0u;
+    }
+    p.ring[(p.slot + 1) % 3] = 0u;
+  }
+}
+
+// Masked regeneration of whole worlds (mask NULL = all); one workgroup per env.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_regen(Params p) {
+  const int b = blockIdx.x;
+  if (p.mask && !p.mask[b]) return;
+  const int nvox = p.X * p.Y * p.Z;
+  store_fresh_world<NT>(p, p.voxels + (int64_t)b * nvox, nullptr, nvox, threadIdx.x);
+}
+
+// generate_world_kernel (src/kernels.cu:1793-1828), 16 voxels per thread.
+__global__ void k_generate_world(int8_t* voxels, int64_t total, int XZ, int Y, int gh) {
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  const int64_t wxyz = (int64_t)XZ * Y;
+  if ((wxyz & 15) == 0) {
+    const int64_t i0 = i << 4;
+    if (i0 >= total) return;
+    int8_t v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = layer_block((int)(((i0 + k) % wxyz) / XZ), gh);
+    uint4 r;
+    r.x = pack4(v[0], v[1], v[2], v[3]);
+    r.y = pack4(v[4], v[5], v[6], v[7]);
+    r.z = pack4(v[8], v[9], v[10], v[11]);
+    r.w = pack4(v[12], v[13], v[14], v[15]);
+    reinterpret_cast<uint4*>(voxels)[i] = r;
+  } else {
+    if (i >= total) return;
+    voxels[i] = layer_block((int)((i % wxyz) / XZ), gh);
+  }
+}
+
+// place_tree_kernel (src/kernels.cu:1831-1865); x/z outside the world are skipped.
+__global__ void k_place_tree(int8_t* voxels, int B, int X, int Y, int Z, int tx, int tz, int gh) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B || tx < 0 || tx >= X || tz < 0 || tz >= Z) return;
+  const int64_t base = (int64_t)b * X * Y * Z;
+  for (int dy = 0; dy < 4; ++dy) {
+    const int y = gh + dy;
+    if (y >= 0 && y < Y) voxels[base + (int64_t)y * X * Z + tx * Z + tz] = (dy < 3) ? kOakLog : kLeaves;
+  }
+}
+
+// place_house_kernel (src/kernels.cu:1868-1946) via the closed form above.
+__global__ void k_place_house(int8_t* voxels, int B, int X, int Y, int Z, int hx, int hz, int gh) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int64_t base = (int64_t)b * X * Y * Z;
+  for (int dy = 0; dy <= 4; ++dy)
+    for (int dx = 0; dx < 5; ++dx)
+      for (int dz = 0; dz < 4; ++dz) {
+        const int vx = hx + dx, vy = gh + dy, vz = hz + dz;
+        if (vx < 0 || vx >= X || vy < 0 || vy >= Y || vz < 0 || vz >= Z) continue;
+        const int8_t h = house_block(dy, dx, dz);
+        if (h != kNoWrite) voxels[base + (int64_t)vy * X * Z + vx * Z + vz] = h;
+      }
+}
+
+// reset_kernel (src/kernels.cu:1760-1786).
+__global__ void k_reset(float* pos, float* vel, float* yaw, float* pitch, uint8_t* og,
+                        const uint8_t* do_reset, int B, float sx, float sy, float sz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B || !do_reset[i]) return;
+  pos[i * 3 + 0] = sx; pos[i * 3 + 1] = sy; pos[i * 3 + 2] = sz;
+  vel[i * 3 + 0] = 0.0f; vel[i * 3 + 1] = 0.0f; vel[i * 3 + 2] = 0.0f;
+  yaw[i] = 0.0f; pitch[i] = 0.0f; og[i] = 0;
+}
+
+__global__ void k_debug_sincos(const float* x, float* s, float* c, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) mc_sincosf(x[i], &s[i], &c[i]);
+}
+
+}  // namespace mc

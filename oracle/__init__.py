@@ -1,0 +1,252 @@
+"""Oracle for the batched environment step -- TEST INFRASTRUCTURE ONLY.
+
+A scalar CPU restatement of Infatoshi/mine.cu's env step, used as the checker
+for the gfx950 HIP kernels.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py`` may import this package; the product
+(``mine.cu_amd/minecu``) never does.
+
+* ``mc_oracle.cpp`` (built by ``Makefile`` into ``_build/libmc_oracle.so``)
+  restates the reference kernels one by one (src/kernels.cu, cited per
+  function).
+* :class:`OracleEnv` restates the ``MineEnv`` orchestration
+  (minecu/__init__.py:141-466) over numpy arrays, calling those kernels in the
+  reference's order.
+
+Pinning: every function here is checked by CPU tests against closed forms and
+invariants (tests/test_oracle_*.py): worldgen layer/tree/house counts, free-fall
+landing steps, shaded colour table, eye-inside-solid and all-air renders, an
+independent float64 Amanatides-Woo DDA, raycast known answers and the episode
+period.  The reference ships no golden vectors (it has no tests), and its CUDA
+source cannot be built here, so nothing is compared with the reference binary.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libmc_oracle.so")
+_lib = None
+
+# Block ids (src/kernels.cu:11-24).
+AIR, GRASS, DIRT, STONE, OAKLOG, LEAVES = -1, 0, 1, 2, 3, 4
+BRICK, COBBLESTONE, PLANKS, BEDROCK = 8, 9, 10, 12
+
+
+def build() -> str:
+    """Compile the oracle library (g++, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    """Load (building first if absent) the oracle shared library."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        i, f, u = ctypes.c_int, ctypes.c_float, ctypes.c_uint
+        L.orc_sincosf.argtypes = [P, P, P, i]
+        L.orc_half_fov_tan.argtypes = [f]
+        L.orc_half_fov_tan.restype = f
+        L.orc_generate_world.argtypes = [P, i, i, i, i, i, u]
+        L.orc_place_tree.argtypes = [P, i, i, i, i, i, i, i]
+        L.orc_place_house.argtypes = [P, i, i, i, i, i, i, i]
+        L.orc_reset.argtypes = [P, P, P, P, P, P, i, f, f, f]
+        L.orc_discrete_to_buttons.argtypes = [P, i, f, P, P]
+        L.orc_decode_actions.argtypes = [P, P, P, P, P, P, P, P, P, P, i, f]
+        L.orc_physics.argtypes = [P, P, P, P, P, P, P, P, P, P, P, i, i, i, i, f, f, f, f]
+        L.orc_raycast_interact.argtypes = [P, P, P, P, P, P, P, i, i, i, i, i, f]
+        L.orc_update_cameras.argtypes = [P, P, P, P, i]
+        L.orc_render.argtypes = [P, P, P, i, i, i, i, i, i, i, i, f, f, P, i]
+        _lib = L
+    return _lib
+
+
+def _p(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be C-contiguous"
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def default_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+# ---------------------------------------------------------------------------
+# Kernel-level wrappers
+# ---------------------------------------------------------------------------
+
+def sincosf(x: np.ndarray):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    lib().orc_sincosf(_p(x), _p(s), _p(c), x.size)
+    return s, c
+
+
+def half_fov_tan(fov_degrees: float) -> float:
+    return float(lib().orc_half_fov_tan(fov_degrees))
+
+
+def fresh_world(B, X, Y, Z, ground_height, tree_xz, house_xz, seed=42) -> np.ndarray:
+    """generate_world + place_tree + place_house, the reference's regen order
+    (minecu/__init__.py:258-268)."""
+    v = np.empty((B, Y, X, Z), dtype=np.int8)
+    L = lib()
+    L.orc_generate_world(_p(v), B, X, Y, Z, ground_height, seed)
+    L.orc_place_tree(_p(v), B, X, Y, Z, tree_xz[0], tree_xz[1], ground_height)
+    L.orc_place_house(_p(v), B, X, Y, Z, house_xz[0], house_xz[1], ground_height)
+    return v
+
+
+def render(voxels, cameras, width, height, max_steps=64, view_distance=32.0,
+           fov_degrees=70.0, out_u8=True, want_hits=False, nthreads=None):
+    B, Y, X, Z = voxels.shape
+    out = np.empty((B, height, width, 3), dtype=np.uint8 if out_u8 else np.float32)
+    hits = np.empty((B, height, width), dtype=np.int32) if want_hits else None
+    lib().orc_render(_p(voxels), _p(cameras), _p(out), int(out_u8), B, X, Y, Z, width, height,
+                     max_steps, view_distance, fov_degrees, _p(hits),
+                     nthreads or default_threads())
+    return (out, hits) if want_hits else out
+
+
+def discrete_to_buttons(actions: np.ndarray, look_delta: float):
+    actions = np.ascontiguousarray(actions, dtype=np.int64)
+    B = actions.shape[0]
+    buttons = np.empty((B, 8), dtype=np.int8)
+    look = np.empty((B, 2), dtype=np.float32)
+    lib().orc_discrete_to_buttons(_p(actions), B, look_delta, _p(buttons), _p(look))
+    return buttons, look
+
+
+# ---------------------------------------------------------------------------
+# MineEnv orchestration restated (minecu/__init__.py)
+# ---------------------------------------------------------------------------
+
+class OracleEnv:
+    """numpy restatement of ``MineEnv`` for the shaded ``uint8``/``float32``
+    observation modes (minecu/__init__.py:141-466)."""
+
+    def __init__(self, batch_size=4096, world_size=32, world_x=None, world_y=None, world_z=None,
+                 render_width=64, render_height=64, max_steps=64, view_distance=32.0,
+                 fov_degrees=70.0, dt=0.05, gravity=-20.0, walk_speed=4.0, sprint_mult=1.5,
+                 jump_vel=8.0, ground_height=8, target_block=OAKLOG, reward_value=1.0,
+                 episode_length=None, seed=42, obs_dtype="uint8", look_delta=0.1,
+                 nthreads=None):
+        assert obs_dtype in ("uint8", "float32")
+        self.batch_size = B = batch_size
+        self.world_x = world_x if world_x is not None else world_size
+        self.world_y = world_y if world_y is not None else world_size
+        self.world_z = world_z if world_z is not None else world_size
+        self.render_width, self.render_height = render_width, render_height
+        self.max_steps, self.view_distance, self.fov_degrees = max_steps, view_distance, fov_degrees
+        self.dt, self.gravity, self.walk_speed = dt, gravity, walk_speed
+        self.sprint_mult, self.jump_vel = sprint_mult, jump_vel
+        self.ground_height, self.target_block, self.reward_value = ground_height, target_block, reward_value
+        self.episode_length, self.seed, self.obs_dtype = episode_length, seed, obs_dtype
+        self.look_delta = look_delta
+        self.nthreads = nthreads or default_threads()
+        # Spawn / tree / house (minecu/__init__.py:196-206).
+        self.spawn_x = self.world_x / 2.0
+        self.spawn_y = float(ground_height + 2)
+        self.spawn_z = self.world_z / 2.0
+        self.tree_x, self.tree_z = int(self.spawn_x), int(self.spawn_z) + 2
+        self.house_x, self.house_z = int(self.spawn_x) + 6, int(self.spawn_z) - 2
+        # State (minecu/__init__.py:214-256).
+        X, Y, Z = self.world_x, self.world_y, self.world_z
+        self.positions = np.zeros((B, 3), np.float32)
+        self.velocities = np.zeros((B, 3), np.float32)
+        self.yaws = np.zeros(B, np.float32)
+        self.pitches = np.zeros(B, np.float32)
+        self.on_ground = np.ones(B, np.uint8)
+        self.obs_buffer = np.zeros((B, render_height, render_width, 3),
+                                   np.uint8 if obs_dtype == "uint8" else np.float32)
+        self.cameras = np.zeros((B, 5), np.float32)
+        self.rewards = np.zeros(B, np.float32)
+        self.step_count = np.zeros(B, np.int32)
+        self.do_reset = np.zeros(B, np.uint8)
+        self.voxels = self._fresh()
+        self.hits = None
+
+    def _fresh(self):
+        return fresh_world(self.batch_size, self.world_x, self.world_y, self.world_z,
+                           self.ground_height, (self.tree_x, self.tree_z),
+                           (self.house_x, self.house_z), self.seed)
+
+    def _regen(self):
+        self.voxels[...] = self._fresh()
+
+    def _render(self, want_hits=False):
+        L = lib()
+        L.orc_update_cameras(_p(self.cameras), _p(self.positions), _p(self.yaws), _p(self.pitches),
+                             self.batch_size)
+        r = render(self.voxels, self.cameras, self.render_width, self.render_height,
+                   self.max_steps, self.view_distance, self.fov_degrees,
+                   out_u8=self.obs_dtype == "uint8", want_hits=want_hits, nthreads=self.nthreads)
+        if want_hits:
+            self.obs_buffer[...], self.hits = r
+        else:
+            self.obs_buffer[...] = r
+        return self.obs_buffer
+
+    # minecu/__init__.py:270-302
+    def reset(self, mask: Optional[np.ndarray] = None, want_hits=False):
+        if mask is None:
+            self.do_reset[:] = 1
+        else:
+            self.do_reset[:] = np.asarray(mask).astype(np.uint8)
+        lib().orc_reset(_p(self.positions), _p(self.velocities), _p(self.yaws), _p(self.pitches),
+                        _p(self.on_ground), _p(self.do_reset), self.batch_size, self.spawn_x,
+                        self.spawn_y, self.spawn_z)
+        self.step_count[self.do_reset.astype(bool)] = 0
+        if mask is None or np.asarray(mask).any():
+            self._regen()
+        return self._render(want_hits)
+
+    # minecu/__init__.py:304-444 (shaded uint8 / float32 modes)
+    def step(self, buttons, look=None, want_hits=False):
+        B = self.batch_size
+        if look is None:
+            buttons, look = discrete_to_buttons(np.asarray(buttons), self.look_delta)
+        buttons = np.ascontiguousarray(buttons, dtype=np.int8)
+        look = np.ascontiguousarray(look, dtype=np.float32)
+        if self.episode_length is not None:
+            self.do_reset = (self.step_count >= self.episode_length).astype(np.uint8)
+        else:
+            self.do_reset[:] = 0
+        fwd, strafe, dyaw, dpitch = (np.empty(B, np.float32) for _ in range(4))
+        jump, brk, place = (np.empty(B, np.uint8) for _ in range(3))
+        speed = np.empty(B, np.float32)
+        L = lib()
+        L.orc_decode_actions(_p(buttons), _p(look), _p(fwd), _p(strafe), _p(dyaw), _p(dpitch),
+                             _p(jump), _p(brk), _p(place), _p(speed), B, self.sprint_mult)
+        X, Y, Z = self.world_x, self.world_y, self.world_z
+        L.orc_physics(_p(self.positions), _p(self.velocities), _p(self.yaws), _p(self.pitches),
+                      _p(self.on_ground), _p(fwd), _p(strafe), _p(dyaw), _p(dpitch), _p(jump),
+                      _p(speed), B, X, Y, Z, self.dt, self.gravity, self.walk_speed, self.jump_vel)
+        L.orc_raycast_interact(_p(self.voxels), _p(self.positions), _p(self.yaws),
+                               _p(self.pitches), _p(brk), _p(place), _p(self.rewards), B, X, Y, Z,
+                               self.target_block, self.reward_value)
+        L.orc_reset(_p(self.positions), _p(self.velocities), _p(self.yaws), _p(self.pitches),
+                    _p(self.on_ground), _p(self.do_reset), B, self.spawn_x, self.spawn_y,
+                    self.spawn_z)
+        self._render(want_hits)
+        self.step_count += 1
+        self.step_count[self.do_reset.astype(bool)] = 1
+        if self.do_reset.any():
+            self._regen()
+        return self.obs_buffer, self.rewards, self.do_reset.astype(bool).copy()
+
+    def state(self):
+        """Copy of every state array, keyed like MineEnv's attributes."""
+        return {k: getattr(self, k).copy() for k in
+                ("voxels", "positions", "velocities", "yaws", "pitches", "on_ground",
+                 "step_count", "cameras", "rewards", "obs_buffer")}

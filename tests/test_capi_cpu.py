@@ -1,0 +1,95 @@
+"""The C ABI without a GPU: the library loads, exports every symbol the header
+declares, and rejects bad arguments with a status and a message before any
+device work.  Also the Python-level argument errors the reference raises."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "minecu_hip.h")
+
+
+def _lib():
+    from minecu import _lib
+    return _lib, _lib.load()
+
+
+def test_library_exports_every_declared_symbol():
+    _l, L = _lib()
+    text = open(HEADER).read()
+    declared = sorted(set(re.findall(r"^(?:int|const char\*)\s+(mc_[a-z0-9_]+)\(", text, re.M)))
+    assert len(declared) >= 20
+    missing = [n for n in declared if not hasattr(L, n)]
+    assert not missing, missing
+    # the ctypes table covers exactly the declared API
+    assert sorted(_l.SIGNATURES) == declared
+
+
+def test_struct_layouts_match_header():
+    _l, _ = _lib()
+    # 28 four-byte fields, 11 pointers
+    assert ctypes.sizeof(_l.McConfig) == 28 * 4
+    assert ctypes.sizeof(_l.McState) == 11 * 8
+
+
+def _err(L):
+    return L.mc_last_error().decode()
+
+
+def test_null_and_range_errors():
+    _l, L = _lib()
+    assert L.mc_step(None, None, None, 0, None, None, None, None) == _l.MC_E_INVALID
+    assert "config" in _err(L)
+    assert L.mc_generate_world(None, 1, 8, 8, 8, 8, 0, None) == _l.MC_E_INVALID
+    v = ctypes.c_void_p(0x1000)  # never dereferenced: validation fails first
+    assert L.mc_generate_world(v, 0, 8, 8, 8, 8, 0, None) == _l.MC_E_INVALID
+    assert L.mc_render(v, v, v, 0, 4, 64, 64, 64, 8, 8, 64, 32.0, 70.0, None) == _l.MC_E_UNSUPPORTED
+    assert "LDS" in _err(L)
+    assert L.mc_render(v, v, v, 7, 4, 8, 8, 8, 8, 8, 64, 32.0, 70.0, None) == _l.MC_E_INVALID
+    assert L.mc_render(v, v, v, 0, 4, 8, 8, 8, 0, 8, 64, 32.0, 70.0, None) == _l.MC_E_INVALID
+    assert L.mc_env_create(None, None, None) == _l.MC_E_INVALID
+    out = ctypes.c_void_p()
+    cfg = _l.McConfig(batch_size=4, world_x=8, world_y=8, world_z=8, width=8, height=8,
+                      obs_mode=0)
+    st = _l.McState()
+    assert L.mc_env_create(ctypes.byref(cfg), ctypes.byref(st), ctypes.byref(out)) == _l.MC_E_INVALID
+    assert "non-NULL" in _err(L) and not out.value
+    assert L.mc_env_step(None, v, 0, v, None, None) == _l.MC_E_INVALID
+    assert L.mc_debug_sincosf(None, None, None, 4, None) == _l.MC_E_INVALID
+
+
+def test_reference_step_rejects_partial_decode_buffers():
+    _l, L = _lib()
+    v = ctypes.c_void_p(0x1000)
+    args = [v] * 20
+    args[9] = None  # forward_in NULL while the other decode buffers are set
+    st = L.mc_launch_step_uint8(*args, 4, 8, 8, 8, 64, 32.0, 70.0, 0.05, -20.0, 4.0, 1.5, 8.0,
+                                3, 1.0, 4.0, 10.0, 4.0, None)
+    assert st == _l.MC_E_INVALID and "decode buffers" in _err(L)
+
+
+def test_check_raises_with_message():
+    _l, L = _lib()
+    L.mc_env_create(None, None, None)
+    with pytest.raises(_l.MineCuError, match="invalid argument"):
+        _l.check(_l.MC_E_INVALID, "x")
+
+
+def test_mineenv_argument_errors():
+    import minecu
+    with pytest.raises(AssertionError):
+        minecu.MineEnv(batch_size=4, device="cpu")
+    with pytest.raises(AssertionError):
+        minecu.MineEnv(batch_size=4, obs_dtype="int16")
+    with pytest.raises(NotImplementedError):
+        minecu.MineEnv(batch_size=4, obs_dtype="uint8_fp16")
+
+
+def test_config_presets_match_reference():
+    from minecu.config import DEFAULT_ENV, FAST_ENV, QUALITY_ENV, EnvConfig
+    assert EnvConfig().episode_length == 60 and EnvConfig().world_size == 32
+    assert (FAST_ENV.batch_size, FAST_ENV.render_width, FAST_ENV.max_steps) == (32768, 32, 32)
+    assert (QUALITY_ENV.render_height, QUALITY_ENV.view_distance) == (128, 64.0)
+    assert DEFAULT_ENV.memory_bytes() == EnvConfig().memory_bytes()
