@@ -221,6 +221,41 @@ int mc_env_step(mc_env* env, const void* actions, int action_kind, const float* 
 int mc_env_reset(mc_env* env, const uint8_t* mask, void* stream);
 int mc_env_render(mc_env* env, void* stream);
 
+/* Launch shape of a group's step: `threads` per workgroup (64, 256 or 512)
+ * and `waves_per_env` (1, 2, 4 or 8) 64-lane waves per env, so a workgroup
+ * holds threads / (64 waves_per_env) envs.  (0, 0) restores the default; the
+ * autotuner below sets it.  get returns the shape after the LDS fit. */
+int mc_env_set_shape(mc_env* env, int threads, int waves_per_env);
+int mc_env_get_shape(const mc_env* env, int* threads, int* waves_per_env);
+
+/* ---- batched groups (no reference counterpart: the reference runs one
+ * fixed-shape MineEnv per process-global graph) --------------------------- */
+
+typedef struct mc_batch mc_batch;
+
+int mc_batch_create(mc_batch** out);
+int mc_batch_destroy(mc_batch* b);
+/* 1 (default): persistent launch with an atomic work counter; 0: one
+ * workgroup per work item. */
+int mc_batch_set_persistent(mc_batch* b, int persistent);
+
+/* One step of n (<= 64) env groups -- any mix of batch sizes, world sizes,
+ * resolutions and obs modes -- in ONE kernel launch.  Per group: actions,
+ * action kind, look (buttons only; `looks` may be NULL when every group is
+ * discrete) and dones (NULL entries or a NULL array: not written).  Each
+ * group's results equal an mc_env_step of that group alone, bit for bit. */
+int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
+                  const int* action_kinds, const float* const* looks, uint8_t* const* dones,
+                  int n, void* stream);
+
+/* Autotuner: for every distinct (world, image, obs) shape among envs, times
+ * each candidate launch shape on a scratch group (never the caller's state)
+ * for `steps` steps and keeps the fastest; sets the shape of every listed env.
+ * Results are cached in the batch object.  Synchronises `stream`. */
+int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* stream);
+int mc_batch_tuned_shape(const mc_batch* b, const mc_config* cfg, int* threads,
+                         int* waves_per_env);
+
 /* ---- test hooks -------------------------------------------------------- */
 
 /* Device evaluation of the shared sin/cos (mc_math.h) on n inputs. */

@@ -6,7 +6,11 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+#include <map>
 #include <new>
+#include <tuple>
+#include <vector>
 
 #include "mc_device.h"
 #include "minecu_hip.h"
@@ -30,12 +34,6 @@ int check_launch(const char* what) {
 }
 
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
-
-// Workgroup size per env: one wave while an observation has at most 1024
-// pixels, four waves above that.
-constexpr int kSmallImage = 1024;
-
-int world_lds_bytes(int X, int Y, int Z) { return ((X * Y * Z) + 15) & ~15; }
 
 int check_dims(int B, int X, int Y, int Z) {
   if (B < 1) return fail(MC_E_INVALID, "batch_size must be >= 1 (got %d)", B);
@@ -121,33 +119,61 @@ mc::Params make_params(const mc_config* c, const mc_state* s) {
   return p;
 }
 
-// Launch one of the per-env kernels with the right template arguments.
-enum class Kind { Step, ResetRender, Render };
+// Launch shape of the per-env work: NT threads per workgroup, wpe waves per
+// env, so one workgroup holds epw = NT / (64 wpe) envs.
+struct Shape {
+  int nt;
+  int wpe;
+};
 
-template <int OBS, int NT>
-int launch_env_kernel(Kind kind, const mc::Params& p, int from_state, hipStream_t st) {
-  const size_t lds = (size_t)world_lds_bytes(p.X, p.Y, p.Z);
-  switch (kind) {
-    case Kind::Step:
-      hipLaunchKernelGGL((mc::k_step<OBS, NT>), dim3(p.B), dim3(NT), lds, st, p);
-      break;
-    case Kind::ResetRender:
-      hipLaunchKernelGGL((mc::k_reset_render<OBS, NT>), dim3(p.B), dim3(NT), lds, st, p);
-      break;
-    case Kind::Render:
-      hipLaunchKernelGGL((mc::k_render<OBS, NT>), dim3(p.B), dim3(NT), lds, st, p, from_state);
-      break;
-  }
+bool valid_shape(Shape s) {
+  return (s.nt == 64 || s.nt == 256 || s.nt == 512) &&
+         (s.wpe == 1 || s.wpe == 2 || s.wpe == 4 || s.wpe == 8) && s.wpe * 64 <= s.nt;
+}
+
+// Default before any tuning: one wave per env up to 1024 pixels, four above.
+Shape default_shape(int W, int H) {
+  return ((int64_t)W * H <= 1024) ? Shape{64, 1} : Shape{256, 4};
+}
+
+int epw_of(Shape s) { return (s.nt / mc::kWave) / s.wpe; }
+
+// Raise wpe until the envs of one workgroup fit the LDS budget.
+Shape fit_lds(Shape s, int X, int Y, int Z) {
+  const int stride = mc::world_stride_bytes(X, Y, Z);
+  while (epw_of(s) * stride > mc::kMaxLdsWorld && s.wpe * 64 < s.nt) s.wpe *= 2;
+  return s;
+}
+
+template <int NT, int KIND>
+int launch_kind(const mc::Params& p, int obs_mode, int wpe, hipStream_t st) {
+  const int epw = (NT / mc::kWave) / wpe;
+  const int grid = (p.B + epw - 1) / epw;
+  const size_t lds = (size_t)epw * mc::world_stride_bytes(p.X, p.Y, p.Z);
+  hipLaunchKernelGGL((mc::k_env<NT, KIND>), dim3(grid), dim3(NT), lds, st, p, obs_mode, wpe);
   return check_launch("env kernel launch");
 }
 
-int launch_env(Kind kind, const mc::Params& p, int obs_mode, int from_state, hipStream_t st) {
-  const bool small = (int64_t)p.W * p.H <= kSmallImage;
-  if (obs_mode == MC_OBS_U8)
-    return small ? launch_env_kernel<0, 64>(kind, p, from_state, st)
-                 : launch_env_kernel<0, 256>(kind, p, from_state, st);
-  return small ? launch_env_kernel<1, 64>(kind, p, from_state, st)
-               : launch_env_kernel<1, 256>(kind, p, from_state, st);
+template <int NT>
+int launch_nt(int kind, const mc::Params& p, int obs_mode, int wpe, hipStream_t st) {
+  switch (kind) {
+    case mc::kItemStep: return launch_kind<NT, mc::kItemStep>(p, obs_mode, wpe, st);
+    case mc::kItemResetRender: return launch_kind<NT, mc::kItemResetRender>(p, obs_mode, wpe, st);
+    case mc::kItemRenderCameras:
+      return launch_kind<NT, mc::kItemRenderCameras>(p, obs_mode, wpe, st);
+    default: return launch_kind<NT, mc::kItemRenderState>(p, obs_mode, wpe, st);
+  }
+}
+
+int launch_env(int kind, const mc::Params& p, int obs_mode, Shape s, hipStream_t st) {
+  s = fit_lds(s, p.X, p.Y, p.Z);
+  if (epw_of(s) * mc::world_stride_bytes(p.X, p.Y, p.Z) > mc::kMaxLdsWorld)
+    return fail(MC_E_UNSUPPORTED, "world does not fit the LDS budget");
+  switch (s.nt) {
+    case 64: return launch_nt<64>(kind, p, obs_mode, s.wpe, st);
+    case 256: return launch_nt<256>(kind, p, obs_mode, s.wpe, st);
+    default: return launch_nt<512>(kind, p, obs_mode, s.wpe, st);
+  }
 }
 
 }  // namespace
@@ -156,8 +182,10 @@ struct mc_env {
   mc_config cfg;
   mc_state st;
   mc::Params base;
-  uint32_t* ring;   // device words: episode flag ring [0..2] + reset flag [3]
-  int slot;         // ring slot the next step launch reads
+  uint32_t* ring;     // device words: episode flag ring [0..2] + reset flag [3]
+  mc::Params* desc;   // device copy of `base` (with ring), read by batched launches
+  int slot;           // ring slot the next step launch reads
+  Shape shape;        // launch shape for this group (default or tuned)
 };
 
 extern "C" {
@@ -204,7 +232,7 @@ int mc_regen_worlds(const mc_config* cfg, int8_t* voxels, const uint8_t* mask, v
   mc::Params p = make_params(cfg, nullptr);
   p.voxels = voxels;
   p.mask = mask;
-  hipLaunchKernelGGL((mc::k_regen<256>), dim3(p.B), dim3(256), 0, S(stream), p);
+  hipLaunchKernelGGL(mc::k_regen, dim3(p.B), dim3(256), 0, S(stream), p);
   return check_launch("regen_worlds");
 }
 
@@ -242,7 +270,7 @@ int mc_render(const int8_t* voxels, const float* cameras, void* obs, int obs_mod
   p.voxels = const_cast<int8_t*>(voxels);
   p.cameras = const_cast<float*>(cameras);
   p.obs = obs;
-  return launch_env(Kind::Render, p, obs_mode, 0, S(stream));
+  return launch_env(mc::kItemRenderCameras, p, obs_mode, default_shape(W, H), S(stream));
 }
 
 int mc_render_state(const mc_config* cfg, const mc_state* st, void* stream) {
@@ -251,7 +279,8 @@ int mc_render_state(const mc_config* cfg, const mc_state* st, void* stream) {
   r = validate_state(st);
   if (r) return r;
   mc::Params p = make_params(cfg, st);
-  return launch_env(Kind::Render, p, cfg->obs_mode, 1, S(stream));
+  return launch_env(mc::kItemRenderState, p, cfg->obs_mode,
+                    default_shape(cfg->width, cfg->height), S(stream));
 }
 
 int mc_step(const mc_config* cfg, const mc_state* st, const void* actions, int action_kind,
@@ -275,12 +304,13 @@ int mc_step(const mc_config* cfg, const mc_state* st, const void* actions, int a
   hipLaunchKernelGGL(mc::k_ring_setup, dim3(1), dim3(1024), 0, S(stream), p, 0);
   r = check_launch("episode pre-pass");
   if (r) return r;
-  return launch_env(Kind::Step, p, cfg->obs_mode, 0, S(stream));
+  return launch_env(mc::kItemStep, p, cfg->obs_mode, default_shape(cfg->width, cfg->height),
+                    S(stream));
 }
 
 namespace {
 int reset_with_ring(const mc_config* cfg, const mc_state* st, const uint8_t* mask,
-                    uint32_t* ring, int slot, void* stream) {
+                    uint32_t* ring, int slot, Shape shape, void* stream) {
   mc::Params p = make_params(cfg, st);
   p.mask = mask;
   p.ring = ring;
@@ -288,7 +318,7 @@ int reset_with_ring(const mc_config* cfg, const mc_state* st, const uint8_t* mas
   hipLaunchKernelGGL(mc::k_ring_setup, dim3(1), dim3(1024), 0, S(stream), p, 1);
   const int r = check_launch("reset pre-pass");
   if (r) return r;
-  return launch_env(Kind::ResetRender, p, cfg->obs_mode, 0, S(stream));
+  return launch_env(mc::kItemResetRender, p, cfg->obs_mode, shape, S(stream));
 }
 }  // namespace
 
@@ -299,7 +329,8 @@ int mc_env_reset_state(const mc_config* cfg, const mc_state* st, const uint8_t* 
   r = validate_state(st);
   if (r) return r;
   if (!workspace) return fail(MC_E_INVALID, "workspace is NULL");
-  return reset_with_ring(cfg, st, mask, reinterpret_cast<uint32_t*>(workspace), 0, stream);
+  return reset_with_ring(cfg, st, mask, reinterpret_cast<uint32_t*>(workspace), 0,
+                         default_shape(cfg->width, cfg->height), stream);
 }
 
 int mc_env_create(const mc_config* cfg, const mc_state* st, mc_env** out) {
@@ -315,15 +346,46 @@ int mc_env_create(const mc_config* cfg, const mc_state* st, mc_env** out) {
   e->st = *st;
   e->base = make_params(cfg, st);
   e->ring = nullptr;
+  e->desc = nullptr;
   e->slot = 0;
+  e->shape = default_shape(cfg->width, cfg->height);
   hipError_t he = hipMalloc(reinterpret_cast<void**>(&e->ring), MC_WORKSPACE_BYTES);
   if (he == hipSuccess) he = hipMemset(e->ring, 0, MC_WORKSPACE_BYTES);
+  if (he == hipSuccess) {
+    e->base.ring = e->ring;
+    he = hipMalloc(reinterpret_cast<void**>(&e->desc), sizeof(mc::Params));
+  }
+  if (he == hipSuccess)
+    he = hipMemcpy(e->desc, &e->base, sizeof(mc::Params), hipMemcpyHostToDevice);
   if (he != hipSuccess) {
     if (e->ring) (void)hipFree(e->ring);
+    if (e->desc) (void)hipFree(e->desc);
     delete e;
     return fail(MC_E_HIP, "workspace: %s", hipGetErrorString(he));
   }
   *out = e;
+  return MC_OK;
+}
+
+int mc_env_set_shape(mc_env* env, int threads, int waves_per_env) {
+  if (!env) return fail(MC_E_INVALID, "env is NULL");
+  if (threads == 0 && waves_per_env == 0) {
+    env->shape = default_shape(env->cfg.width, env->cfg.height);
+    return MC_OK;
+  }
+  const Shape s{threads, waves_per_env};
+  if (!valid_shape(s))
+    return fail(MC_E_INVALID, "shape (%d threads, %d waves/env) not supported", threads,
+                waves_per_env);
+  env->shape = s;
+  return MC_OK;
+}
+
+int mc_env_get_shape(const mc_env* env, int* threads, int* waves_per_env) {
+  if (!env || !threads || !waves_per_env) return fail(MC_E_INVALID, "NULL argument");
+  const Shape s = fit_lds(env->shape, env->cfg.world_x, env->cfg.world_y, env->cfg.world_z);
+  *threads = s.nt;
+  *waves_per_env = s.wpe;
   return MC_OK;
 }
 
@@ -339,6 +401,7 @@ int mc_env_sync(mc_env* env, void* stream) {
 int mc_env_destroy(mc_env* env) {
   if (!env) return MC_OK;
   if (env->ring) (void)hipFree(env->ring);
+  if (env->desc) (void)hipFree(env->desc);
   delete env;
   return MC_OK;
 }
@@ -357,20 +420,20 @@ int mc_env_step(mc_env* env, const void* actions, int action_kind, const float* 
   p.dones = dones_out;
   p.ring = env->ring;
   p.slot = env->slot;
-  const int r = launch_env(Kind::Step, p, env->cfg.obs_mode, 0, S(stream));
+  const int r = launch_env(mc::kItemStep, p, env->cfg.obs_mode, env->shape, S(stream));
   if (r == MC_OK) env->slot = (env->slot + 1) % 3;
   return r;
 }
 
 int mc_env_reset(mc_env* env, const uint8_t* mask, void* stream) {
   if (!env) return fail(MC_E_INVALID, "env is NULL");
-  return reset_with_ring(&env->cfg, &env->st, mask, env->ring, env->slot, stream);
+  return reset_with_ring(&env->cfg, &env->st, mask, env->ring, env->slot, env->shape, stream);
 }
 
 int mc_env_render(mc_env* env, void* stream) {
   if (!env) return fail(MC_E_INVALID, "env is NULL");
   mc::Params p = env->base;
-  return launch_env(Kind::Render, p, env->cfg.obs_mode, 1, S(stream));
+  return launch_env(mc::kItemRenderState, p, env->cfg.obs_mode, env->shape, S(stream));
 }
 
 namespace {
@@ -444,7 +507,7 @@ int launch_step_ref(int obs_mode, int8_t* voxels, float* positions, float* veloc
     p.do_place = do_place;
     p.speed_mult_in = speed_mult;
   }
-  return launch_env(Kind::Step, p, obs_mode, 0, S(stream));
+  return launch_env(mc::kItemStep, p, obs_mode, default_shape(W, H), S(stream));
 }
 }  // namespace
 
@@ -497,6 +560,338 @@ int mc_launch_render(const int8_t* voxels, const float* cameras, float* output, 
                      float fov_degrees, void* stream) {
   return mc_render(voxels, cameras, output, MC_OBS_F32, batch_size, world_size, world_size,
                    world_size, width, height, max_steps, view_distance, fov_degrees, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Batched groups: one persistent launch over several env groups.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+constexpr int kEntryRing = 8;   // descriptor tables in flight
+
+using BucketKey = std::tuple<int, int, int, int, int, int>;  // X, Y, Z, W, H, obs_mode
+
+BucketKey bucket_of(const mc_config& c) {
+  return BucketKey{c.world_x, c.world_y, c.world_z, c.width, c.height, c.obs_mode};
+}
+
+struct Occupancy {
+  int nt, lds, blocks;
+};
+}  // namespace
+
+struct mc_batch {
+  uint32_t* counters = nullptr;               // device [2]
+  mc::MultiEntry* dev_entries = nullptr;      // [kEntryRing][kMaxGroups]
+  mc::MultiEntry* host_entries = nullptr;     // pinned, same shape
+  hipEvent_t ev[kEntryRing] = {};
+  bool ev_live[kEntryRing] = {};
+  int next = 0;
+  int persistent = 1;
+  int cus = 0;
+  std::vector<Occupancy> occ;
+  std::map<BucketKey, Shape> tuned;
+};
+
+namespace {
+template <int NT>
+int occupancy_blocks(mc_batch* b, int lds) {
+  for (const auto& o : b->occ)
+    if (o.nt == NT && o.lds == lds) return o.blocks;
+  int nb = 0;
+  const hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mc::k_multi<NT>, NT, (size_t)lds);
+  if (e != hipSuccess || nb < 1) nb = 1;
+  b->occ.push_back({NT, lds, nb});
+  return nb;
+}
+
+template <int NT>
+int launch_multi(mc_batch* b, const mc::MultiArgs& a, int lds, hipStream_t st) {
+  int grid = a.total_items;
+  if (a.persistent) grid = std::min(grid, occupancy_blocks<NT>(b, lds) * b->cus);
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL((mc::k_multi<NT>), dim3(grid), dim3(NT), (size_t)lds, st, a);
+  return check_launch("batched step launch");
+}
+}  // namespace
+
+extern "C" {
+
+int mc_batch_create(mc_batch** out) {
+  if (!out) return fail(MC_E_INVALID, "out is NULL");
+  *out = nullptr;
+  mc_batch* b = new (std::nothrow) mc_batch;
+  if (!b) return fail(MC_E_INVALID, "out of host memory");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const size_t tbl = sizeof(mc::MultiEntry) * mc::kMaxGroups * kEntryRing;
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->counters), 2 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemset(b->counters, 0, 2 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->dev_entries), tbl);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->host_entries), tbl, 0);
+  for (int i = 0; i < kEntryRing && e == hipSuccess; ++i)
+    e = hipEventCreateWithFlags(&b->ev[i], hipEventDisableTiming);
+  if (e != hipSuccess) {
+    mc_batch_destroy(b);
+    return fail(MC_E_HIP, "batch create: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return MC_OK;
+}
+
+int mc_batch_destroy(mc_batch* b) {
+  if (!b) return MC_OK;
+  for (int i = 0; i < kEntryRing; ++i)
+    if (b->ev[i]) {
+      if (b->ev_live[i]) (void)hipEventSynchronize(b->ev[i]);
+      (void)hipEventDestroy(b->ev[i]);
+    }
+  if (b->counters) (void)hipFree(b->counters);
+  if (b->dev_entries) (void)hipFree(b->dev_entries);
+  if (b->host_entries) (void)hipHostFree(b->host_entries);
+  delete b;
+  return MC_OK;
+}
+
+int mc_batch_set_persistent(mc_batch* b, int persistent) {
+  if (!b) return fail(MC_E_INVALID, "batch is NULL");
+  b->persistent = persistent ? 1 : 0;
+  return MC_OK;
+}
+
+int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
+                  const int* action_kinds, const float* const* looks, uint8_t* const* dones,
+                  int n, void* stream) {
+  if (!b || !envs || !actions || !action_kinds) return fail(MC_E_INVALID, "NULL argument");
+  if (n < 1 || n > mc::kMaxGroups)
+    return fail(MC_E_INVALID, "group count %d outside [1, %d]", n, mc::kMaxGroups);
+  // Launch width: 512 threads if some group wants 8 waves per env, else 256.
+  int nt = 256;
+  for (int i = 0; i < n; ++i) {
+    if (!envs[i] || !actions[i]) return fail(MC_E_INVALID, "group %d: NULL env/actions", i);
+    const int k = action_kinds[i];
+    if (k != MC_ACT_BUTTONS && k != MC_ACT_DISCRETE)
+      return fail(MC_E_INVALID, "group %d: bad action kind %d", i, k);
+    if (k == MC_ACT_BUTTONS && (!looks || !looks[i]))
+      return fail(MC_E_INVALID, "group %d: look is NULL", i);
+    for (int j = 0; j < i; ++j)
+      if (envs[j] == envs[i]) return fail(MC_E_INVALID, "group %d listed twice", i);
+    if (envs[i]->shape.wpe == 8) nt = 512;
+  }
+  struct G {
+    int idx, wpe, items;
+    double cost;
+  };
+  std::vector<G> gs(n);
+  int lds = 0;
+  for (int i = 0; i < n; ++i) {
+    const mc_config& c = envs[i]->cfg;
+    Shape s{nt, std::min(envs[i]->shape.wpe, nt / mc::kWave)};
+    s = fit_lds(s, c.world_x, c.world_y, c.world_z);
+    const int stride = mc::world_stride_bytes(c.world_x, c.world_y, c.world_z);
+    if (epw_of(s) * stride > mc::kMaxLdsWorld)
+      return fail(MC_E_UNSUPPORTED, "group %d: world does not fit the LDS budget", i);
+    lds = std::max(lds, epw_of(s) * stride);
+    const int epw = epw_of(s);
+    const double per_env = (double)c.width * c.height * (1.0 + c.world_y / 16.0) + stride;
+    gs[i] = G{i, s.wpe, (c.batch_size + epw - 1) / epw, per_env * epw};
+  }
+  // Most expensive items first, so the tail of the persistent loop is short.
+  std::stable_sort(gs.begin(), gs.end(), [](const G& x, const G& y) { return x.cost > y.cost; });
+  const int slot = b->next;
+  b->next = (b->next + 1) % kEntryRing;
+  if (b->ev_live[slot]) {
+    const hipError_t e = hipEventSynchronize(b->ev[slot]);
+    if (e != hipSuccess) return fail(MC_E_HIP, "batch table wait: %s", hipGetErrorString(e));
+  }
+  mc::MultiEntry* he = b->host_entries + (size_t)slot * mc::kMaxGroups;
+  mc::MultiEntry* de = b->dev_entries + (size_t)slot * mc::kMaxGroups;
+  int items = 0;
+  for (int k = 0; k < n; ++k) {
+    const int i = gs[k].idx;
+    mc::MultiEntry& m = he[k];
+    m.desc = envs[i]->desc;
+    m.actions = actions[i];
+    m.look = looks ? looks[i] : nullptr;
+    m.dones = dones ? dones[i] : nullptr;
+    m.action_kind = action_kinds[i];
+    m.slot = envs[i]->slot;
+    m.item_begin = items;
+    m.wpe = gs[k].wpe;
+    m.obs_mode = envs[i]->cfg.obs_mode;
+    m.pad = 0;
+    items += gs[k].items;
+  }
+  hipError_t e = hipMemcpyAsync(de, he, sizeof(mc::MultiEntry) * n, hipMemcpyHostToDevice,
+                                S(stream));
+  if (e != hipSuccess) return fail(MC_E_HIP, "batch table upload: %s", hipGetErrorString(e));
+  mc::MultiArgs a;
+  a.n_groups = n;
+  a.total_items = items;
+  a.persistent = b->persistent;
+  a.counters = b->counters;
+  a.entries = de;
+  const int r = (nt == 512) ? launch_multi<512>(b, a, lds, S(stream))
+                            : launch_multi<256>(b, a, lds, S(stream));
+  if (r) return r;
+  e = hipEventRecord(b->ev[slot], S(stream));
+  if (e != hipSuccess) return fail(MC_E_HIP, "batch event: %s", hipGetErrorString(e));
+  b->ev_live[slot] = true;
+  for (int i = 0; i < n; ++i) envs[i]->slot = (envs[i]->slot + 1) % 3;
+  return MC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Scratch copy of one env group for timing candidates without touching the
+// caller's state.
+struct Scratch {
+  mc_state st{};
+  std::vector<void*> allocs;
+  ~Scratch() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+  bool alloc(void** p, size_t n) {
+    if (hipMalloc(p, n) != hipSuccess) return false;
+    allocs.push_back(*p);
+    return hipMemset(*p, 0, n) == hipSuccess;
+  }
+};
+
+// Deterministic discrete actions (uniform ids 0..10) for tuning runs.
+std::vector<int64_t> tuning_actions(int steps, int B) {
+  std::vector<int64_t> a((size_t)steps * B);
+  uint64_t x = 0x9e3779b97f4a7c15ull;
+  for (auto& v : a) {
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    v = (int64_t)(x % 11u);
+  }
+  return a;
+}
+}  // namespace
+
+extern "C" {
+
+int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* stream) {
+  if (!b || !envs || n < 1) return fail(MC_E_INVALID, "NULL argument");
+  if (steps < 1) steps = 12;
+  std::map<BucketKey, const mc_env*> rep;  // largest group of each bucket
+  for (int i = 0; i < n; ++i) {
+    if (!envs[i]) return fail(MC_E_INVALID, "env %d is NULL", i);
+    const BucketKey k = bucket_of(envs[i]->cfg);
+    auto it = rep.find(k);
+    if (it == rep.end() || it->second->cfg.batch_size < envs[i]->cfg.batch_size) rep[k] = envs[i];
+  }
+  const hipStream_t st = S(stream);
+  const Shape cands[] = {{64, 1}, {256, 1}, {256, 2}, {256, 4}, {512, 8}};
+  for (const auto& kv : rep) {
+    if (b->tuned.count(kv.first)) continue;
+    mc_config c = kv.second->cfg;
+    c.batch_size = std::min(c.batch_size, 4096);
+    const int B = c.batch_size;
+    const size_t nvox = (size_t)c.world_x * c.world_y * c.world_z;
+    const size_t obs_b = (size_t)c.width * c.height * 3 * (c.obs_mode == MC_OBS_F32 ? 4 : 1);
+    Scratch sc, bk;
+    mc_state& s = sc.st;
+    bool ok = sc.alloc((void**)&s.voxels, B * nvox) && sc.alloc((void**)&s.positions, B * 12) &&
+              sc.alloc((void**)&s.velocities, B * 12) && sc.alloc((void**)&s.yaws, B * 4) &&
+              sc.alloc((void**)&s.pitches, B * 4) && sc.alloc((void**)&s.on_ground, B) &&
+              sc.alloc((void**)&s.step_count, B * 4) && sc.alloc((void**)&s.do_reset, B) &&
+              sc.alloc((void**)&s.cameras, B * 20) && sc.alloc(&s.obs, B * obs_b) &&
+              sc.alloc((void**)&s.rewards, B * 4);
+    void *save_vox = nullptr, *save_agent = nullptr, *acts = nullptr, *ring = nullptr;
+    const size_t agent_b = (size_t)B * (12 + 12 + 4 + 4 + 1 + 4);
+    ok = ok && bk.alloc(&save_vox, B * nvox) && bk.alloc(&save_agent, agent_b) &&
+         bk.alloc(&ring, MC_WORKSPACE_BYTES) && bk.alloc(&acts, (size_t)(steps + 8) * B * 8);
+    if (!ok) return fail(MC_E_HIP, "autotune: scratch allocation failed");
+    const std::vector<int64_t> ha = tuning_actions(steps + 8, B);
+    if (hipMemcpy(acts, ha.data(), ha.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(MC_E_HIP, "autotune: action upload failed");
+    int r = mc_regen_worlds(&c, s.voxels, nullptr, stream);
+    if (!r) r = reset_with_ring(&c, &s, nullptr, (uint32_t*)ring, 0, default_shape(c.width, c.height), stream);
+    mc::Params p = make_params(&c, &s);
+    p.ring = (uint32_t*)ring;
+    p.action_kind = MC_ACT_DISCRETE;
+    int slot = 0;
+    // Warm-up steps bring the group into its steady state.
+    for (int t = 0; t < 8 && !r; ++t) {
+      p.actions = (const int64_t*)acts + (size_t)t * B;
+      p.slot = slot;
+      r = launch_env(mc::kItemStep, p, c.obs_mode, default_shape(c.width, c.height), st);
+      slot = (slot + 1) % 3;
+    }
+    if (r) return r;
+    // Snapshot voxels + agent state (+ ring) so every candidate starts equal.
+    auto copy_agent = [&](bool save) -> hipError_t {
+      char* a = (char*)save_agent;
+      struct F { void* p; size_t n; } fs[] = {{s.positions, (size_t)B * 12}, {s.velocities, (size_t)B * 12},
+                                             {s.yaws, (size_t)B * 4}, {s.pitches, (size_t)B * 4},
+                                             {s.on_ground, (size_t)B}, {s.step_count, (size_t)B * 4}};
+      hipError_t e = save ? hipMemcpyAsync(save_vox, s.voxels, B * nvox, hipMemcpyDeviceToDevice, st)
+                          : hipMemcpyAsync(s.voxels, save_vox, B * nvox, hipMemcpyDeviceToDevice, st);
+      for (auto& f : fs) {
+        if (e != hipSuccess) break;
+        e = save ? hipMemcpyAsync(a, f.p, f.n, hipMemcpyDeviceToDevice, st)
+                 : hipMemcpyAsync(f.p, a, f.n, hipMemcpyDeviceToDevice, st);
+        a += f.n;
+      }
+      return e;
+    };
+    if (copy_agent(true) != hipSuccess) return fail(MC_E_HIP, "autotune: snapshot failed");
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+      return fail(MC_E_HIP, "autotune: event create failed");
+    Shape best = default_shape(c.width, c.height);
+    float best_ms = 1e30f;
+    for (const Shape& cand : cands) {
+      const Shape f = fit_lds(cand, c.world_x, c.world_y, c.world_z);
+      if (f.wpe != cand.wpe) continue;  // does not fit as asked
+      if (copy_agent(false) != hipSuccess) break;
+      // The saved ring slot state: recompute from step_count.
+      mc::Params q = p;
+      q.slot = slot;
+      hipLaunchKernelGGL(mc::k_ring_setup, dim3(1), dim3(1024), 0, st, q, 0);
+      int s2 = slot;
+      (void)hipEventRecord(e0, st);
+      for (int t = 0; t < steps && !r; ++t) {
+        q.actions = (const int64_t*)acts + (size_t)(8 + t) * B;
+        q.slot = s2;
+        r = launch_env(mc::kItemStep, q, c.obs_mode, cand, st);
+        s2 = (s2 + 1) % 3;
+      }
+      (void)hipEventRecord(e1, st);
+      if (r || hipEventSynchronize(e1) != hipSuccess) break;
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = cand;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r) return r;
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(MC_E_HIP, "autotune: sync failed");
+    b->tuned[kv.first] = best;
+  }
+  for (int i = 0; i < n; ++i) envs[i]->shape = b->tuned[bucket_of(envs[i]->cfg)];
+  return MC_OK;
+}
+
+int mc_batch_tuned_shape(const mc_batch* b, const mc_config* cfg, int* threads,
+                         int* waves_per_env) {
+  if (!b || !cfg || !threads || !waves_per_env) return fail(MC_E_INVALID, "NULL argument");
+  auto it = b->tuned.find(bucket_of(*cfg));
+  if (it == b->tuned.end()) return fail(MC_E_INVALID, "shape not tuned");
+  *threads = it->second.nt;
+  *waves_per_env = it->second.wpe;
+  return MC_OK;
 }
 
 int mc_debug_sincosf(const float* x, float* s, float* c, int n, void* stream) {

@@ -20,7 +20,9 @@
 namespace mc {
 
 constexpr int kWave = 64;
-constexpr int kMaxLdsWorld = 65536;   // bytes of world staged in LDS per workgroup
+// Bytes of world staged in LDS per workgroup: 64 KiB less room for the
+// kernels' static LDS (palette, frames, work-item bookkeeping).
+constexpr int kMaxLdsWorld = 65536 - 1024;
 
 // Block ids (src/kernels.cu:11-24).
 constexpr int8_t kAir = -1, kGrass = 0, kDirt = 1, kOakLog = 3, kLeaves = 4, kBrick = 8,
@@ -190,20 +192,19 @@ __device__ __forceinline__ uint4 fresh_chunk(const Params& p, int i0) {
 }
 
 // Write env b's fresh world to HBM (and to LDS when lds != nullptr).
-template <int NT>
 __device__ __forceinline__ void store_fresh_world(const Params& p, int8_t* gworld, int8_t* lds,
-                                                  int nvox, int tid) {
+                                                  int nvox, int tid, int nt) {
   if ((nvox & 15) == 0) {
     uint4* g = reinterpret_cast<uint4*>(gworld);
     uint4* l = reinterpret_cast<uint4*>(lds);
-    for (int c = tid; c < (nvox >> 4); c += NT) {
+    for (int c = tid; c < (nvox >> 4); c += nt) {
       const uint4 v = fresh_chunk(p, c << 4);
       g[c] = v;
       if (lds) l[c] = v;
     }
   } else {
     const int XZ = p.X * p.Z;
-    for (int i = tid; i < nvox; i += NT) {
+    for (int i = tid; i < nvox; i += nt) {
       const int y = i / XZ, rem = i - y * XZ, x = rem / p.Z, z = rem - x * p.Z;
       const int8_t v = fresh_voxel(p, y, x, z);
       gworld[i] = v;
@@ -213,14 +214,14 @@ __device__ __forceinline__ void store_fresh_world(const Params& p, int8_t* gworl
 }
 
 // HBM -> LDS copy of one env's world.
-template <int NT>
-__device__ __forceinline__ void stage_world(int8_t* lds, const int8_t* gworld, int nvox, int tid) {
+__device__ __forceinline__ void stage_world(int8_t* lds, const int8_t* gworld, int nvox, int tid,
+                                            int nt) {
   if ((nvox & 15) == 0) {
     const uint4* g = reinterpret_cast<const uint4*>(gworld);
     uint4* l = reinterpret_cast<uint4*>(lds);
-    for (int c = tid; c < (nvox >> 4); c += NT) l[c] = g[c];
+    for (int c = tid; c < (nvox >> 4); c += nt) l[c] = g[c];
   } else {
-    for (int i = tid; i < nvox; i += NT) lds[i] = gworld[i];
+    for (int i = tid; i < nvox; i += nt) lds[i] = gworld[i];
   }
 }
 
@@ -336,9 +337,9 @@ __device__ __forceinline__ uint32_t to_u8(float c) { return (uint32_t)(c * 255.0
 // Render every pixel of env b into obs (layout [B,H,W,3]).
 // OBS: 0 = uint8, 1 = float32.
 // ---------------------------------------------------------------------------
-template <int OBS, int NT>
+template <int OBS>
 __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t* world,
-                                           const float* pal, const Frame& f, int tid) {
+                                           const float* pal, const Frame& f, int tid, int nt) {
   const int W = p.W, H = p.H, npix = W * H;
   const float hf = p.half_fov, aspect = p.aspect;
   if (OBS == 0) {
@@ -352,7 +353,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       const uint32_t w2 = bl | (r << 8) | (g << 16) | (bl << 24);
       // Chunk c starts at byte 16c, i.e. at channel phase (16c mod 3) = c mod 3.
       uint4* o4 = reinterpret_cast<uint4*>(out);
-      for (int c = tid; c < (npix * 3) >> 4; c += NT) {
+      for (int c = tid; c < (npix * 3) >> 4; c += nt) {
         const int ph = c % 3;
         const uint32_t a = (ph == 0) ? w0 : (ph == 1) ? w1 : w2;
         const uint32_t bq = (ph == 0) ? w1 : (ph == 1) ? w2 : w0;
@@ -362,7 +363,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       return;
     }
     if ((W & 3) == 0) {
-      for (int q = tid; q < (npix >> 2); q += NT) {
+      for (int q = tid; q < (npix >> 2); q += nt) {
         const int p0 = q << 2, py = p0 / W, px0 = p0 - py * W;
         const float v = (1.0f - 2.0f * (py + 0.5f) / H) * hf;
         // Packed 0x00BBGGRR per pixel.
@@ -379,7 +380,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
         o[2] = (c2 >> 16) | (c3 << 8);
       }
     } else {
-      for (int pi = tid; pi < npix; pi += NT) {
+      for (int pi = tid; pi < npix; pi += nt) {
         const int py = pi / W, px = pi - py * W;
         const float v = (1.0f - 2.0f * (py + 0.5f) / H) * hf;
         const float u = (2.0f * (px + 0.5f) / W - 1.0f) * hf * aspect;
@@ -395,7 +396,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
   } else {
     float* out = reinterpret_cast<float*>(p.obs) + (int64_t)b * npix * 3;
     if ((W & 3) == 0) {
-      for (int q = tid; q < (npix >> 2); q += NT) {
+      for (int q = tid; q < (npix >> 2); q += nt) {
         const int p0 = q << 2, py = p0 / W, px0 = p0 - py * W;
         const float v = (1.0f - 2.0f * (py + 0.5f) / H) * hf;
         float vals[12];
@@ -415,7 +416,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
         o[2] = make_float4(vals[8], vals[9], vals[10], vals[11]);
       }
     } else {
-      for (int pi = tid; pi < npix; pi += NT) {
+      for (int pi = tid; pi < npix; pi += nt) {
         const int py = pi / W, px = pi - py * W;
         const float v = (1.0f - 2.0f * (py + 0.5f) / H) * hf;
         const float u = (2.0f * (px + 0.5f) / W - 1.0f) * hf * aspect;
@@ -603,132 +604,210 @@ __device__ __forceinline__ void load_palette(float* pal, int tid) {
   if (tid < kPaletteLen) pal[tid] = palette_entry(tid);
 }
 
-// ---------------------------------------------------------------------------
-// Kernels
-// ---------------------------------------------------------------------------
-
-// Fused env step; one workgroup (NT threads) per env.
-template <int OBS, int NT>
-__global__ __launch_bounds__(NT) void k_step(Params p) {
-  extern __shared__ __align__(16) int8_t s_world[];
-  __shared__ float s_pal[kPaletteLen];
-  __shared__ Frame s_frame;
-  const int b = blockIdx.x;
-  const int tid = threadIdx.x;
-  const int nvox = p.X * p.Y * p.Z;
-  int8_t* gworld = p.voxels + (int64_t)b * nvox;
-  load_palette(s_pal, tid);
-  stage_world<NT>(s_world, gworld, nvox, tid);
-  const bool any_done = (p.done_mode == 0 && p.ring) ? (p.ring[p.slot] != 0u) : false;
-  if (p.done_mode == 0 && p.ring && b == 0 && tid == 0) p.ring[(p.slot + 2) % 3] = 0u;
-  __syncthreads();
-  if (tid < kWave) {
-    const Frame f = env_update(p, b, s_world, gworld, any_done);
-    if (tid == 0) s_frame = f;
+// Wave-0 logic of MineEnv.reset for env b (minecu/__init__.py:270-302):
+// respawn when masked (reset_kernel, kernels.cu:1775-1785), step_count = 0,
+// report "done on the next step" into the episode ring, refresh the camera.
+__device__ __forceinline__ Frame reset_update(const Params& p, int b, const int8_t* world) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const bool r = p.mask ? (p.mask[b] != 0) : true;
+  float x = p.positions[b * 3 + 0], y = p.positions[b * 3 + 1], z = p.positions[b * 3 + 2];
+  float yaw = p.yaws[b], pitch = p.pitches[b];
+  if (r) {
+    x = p.spawn_x; y = p.spawn_y; z = p.spawn_z;
+    yaw = 0.0f; pitch = 0.0f;
   }
-  __syncthreads();
-  const Frame f = s_frame;
-  render_env<OBS, NT>(p, b, s_world, s_pal, f, tid);
-  if (any_done) store_fresh_world<NT>(p, gworld, nullptr, nvox, tid);
+  if (lane == 0) {
+    p.do_reset[b] = r ? 1 : 0;
+    if (r) {
+      p.positions[b * 3 + 0] = x;
+      p.positions[b * 3 + 1] = y;
+      p.positions[b * 3 + 2] = z;
+      p.velocities[b * 3 + 0] = 0.0f;
+      p.velocities[b * 3 + 1] = 0.0f;
+      p.velocities[b * 3 + 2] = 0.0f;
+      p.yaws[b] = 0.0f;
+      p.pitches[b] = 0.0f;
+      p.on_ground[b] = 0;
+      p.step_count[b] = 0;
+    }
+    const int32_t sc = r ? 0 : p.step_count[b];
+    if (p.episode_length >= 0 && sc >= p.episode_length)
+      __hip_atomic_store(&p.ring[p.slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    p.cameras[b * 5 + 0] = x;
+    p.cameras[b * 5 + 1] = y;
+    p.cameras[b * 5 + 2] = z;
+    p.cameras[b * 5 + 3] = yaw;
+    p.cameras[b * 5 + 4] = pitch;
+  }
+  float sy, cy, sp, cp;
+  mc_sincosf(yaw, &sy, &cy);
+  mc_sincosf(pitch, &sp, &cp);
+  return make_frame(p, x, y, z, sy, cy, sp, cp, world);
 }
 
-// MineEnv.reset: masked respawn, step_count = 0, regeneration when flagged,
-// camera update, render.  One workgroup per env.
-template <int OBS, int NT>
-__global__ __launch_bounds__(NT) void k_reset_render(Params p) {
-  extern __shared__ __align__(16) int8_t s_world[];
-  __shared__ float s_pal[kPaletteLen];
-  __shared__ Frame s_frame;
-  const int b = blockIdx.x;
-  const int tid = threadIdx.x;
-  const int nvox = p.X * p.Y * p.Z;
-  int8_t* gworld = p.voxels + (int64_t)b * nvox;
-  load_palette(s_pal, tid);
-  const bool regen = p.ring[3] != 0u;
-  if (regen) store_fresh_world<NT>(p, gworld, s_world, nvox, tid);
-  else stage_world<NT>(s_world, gworld, nvox, tid);
-  __syncthreads();
-  if (tid < kWave) {
-    const bool r = p.mask ? (p.mask[b] != 0) : true;
-    float x = p.positions[b * 3 + 0], y = p.positions[b * 3 + 1], z = p.positions[b * 3 + 2];
-    float yaw = p.yaws[b], pitch = p.pitches[b];
-    if (r) {
-      x = p.spawn_x; y = p.spawn_y; z = p.spawn_z;
-      yaw = 0.0f; pitch = 0.0f;
-    }
-    if (tid == 0) {
-      p.do_reset[b] = r ? 1 : 0;
-      if (r) {
-        p.positions[b * 3 + 0] = x;
-        p.positions[b * 3 + 1] = y;
-        p.positions[b * 3 + 2] = z;
-        p.velocities[b * 3 + 0] = 0.0f;
-        p.velocities[b * 3 + 1] = 0.0f;
-        p.velocities[b * 3 + 2] = 0.0f;
-        p.yaws[b] = 0.0f;
-        p.pitches[b] = 0.0f;
-        p.on_ground[b] = 0;
-        p.step_count[b] = 0;
-      }
-      const int32_t sc = r ? 0 : p.step_count[b];
-      if (p.episode_length >= 0 && sc >= p.episode_length)
-        __hip_atomic_store(&p.ring[p.slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Wave-0 camera of env b: from cameras [B,5], or from the agent state, which
+// also refreshes cameras (MineEnv._render, minecu/__init__.py:446-466).
+__device__ __forceinline__ Frame camera_update(const Params& p, int b, const int8_t* world,
+                                               bool from_state) {
+  const int lane = threadIdx.x & (kWave - 1);
+  float x, y, z, yaw, pitch;
+  if (from_state) {
+    x = p.positions[b * 3 + 0]; y = p.positions[b * 3 + 1]; z = p.positions[b * 3 + 2];
+    yaw = p.yaws[b]; pitch = p.pitches[b];
+    if (lane == 0) {
       p.cameras[b * 5 + 0] = x;
       p.cameras[b * 5 + 1] = y;
       p.cameras[b * 5 + 2] = z;
       p.cameras[b * 5 + 3] = yaw;
       p.cameras[b * 5 + 4] = pitch;
     }
-    float sy, cy, sp, cp;
-    mc_sincosf(yaw, &sy, &cy);
-    mc_sincosf(pitch, &sp, &cp);
-    const Frame f = make_frame(p, x, y, z, sy, cy, sp, cp, s_world);
-    if (tid == 0) s_frame = f;
+  } else {
+    x = p.cameras[b * 5 + 0]; y = p.cameras[b * 5 + 1]; z = p.cameras[b * 5 + 2];
+    yaw = p.cameras[b * 5 + 3]; pitch = p.cameras[b * 5 + 4];
   }
-  __syncthreads();
-  const Frame f = s_frame;
-  render_env<OBS, NT>(p, b, s_world, s_pal, f, tid);
+  float sy, cy, sp, cp;
+  mc_sincosf(yaw, &sy, &cy);
+  mc_sincosf(pitch, &sp, &cp);
+  return make_frame(p, x, y, z, sy, cy, sp, cp, world);
 }
 
-// Render from cameras [B,5] (from_state = 0) or from the agent state, which
-// also refreshes cameras (from_state = 1, MineEnv._render).
-template <int OBS, int NT>
-__global__ __launch_bounds__(NT) void k_render(Params p, int from_state) {
-  extern __shared__ __align__(16) int8_t s_world[];
-  __shared__ float s_pal[kPaletteLen];
-  __shared__ Frame s_frame;
-  const int b = blockIdx.x;
-  const int tid = threadIdx.x;
+// ---------------------------------------------------------------------------
+// Work items.  A workgroup of NT threads processes one item: epw = NT/(64 wpe)
+// consecutive envs b0.. of one group, with wpe waves (64 wpe threads) per env.
+// The envs of an item share the workgroup barriers; each has its own LDS world
+// slot (world_stride bytes apart) and camera frame.
+// ---------------------------------------------------------------------------
+enum ItemKind { kItemStep = 0, kItemResetRender = 1, kItemRenderCameras = 2, kItemRenderState = 3 };
+
+__host__ __device__ __forceinline__ int world_stride_bytes(int X, int Y, int Z) {
+  return (X * Y * Z + 15) & ~15;
+}
+
+template <int NT, int KIND>
+__device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, int wpe,
+                                         int8_t* lds, const float* pal, Frame* frames) {
+  const int tid = threadIdx.x, wave = tid / kWave;
+  const int epw = (NT / kWave) / wpe;
+  const int e = wave / wpe;                                 // env slot within the item
+  const int et = tid - e * wpe * kWave;                     // thread index within the env
+  const int ent = wpe * kWave;                              // threads per env
+  const int b = b0 + e;
+  const bool active = (e < epw) && (b < p.B);
   const int nvox = p.X * p.Y * p.Z;
-  const int8_t* gworld = p.voxels + (int64_t)b * nvox;
-  load_palette(s_pal, tid);
-  stage_world<NT>(s_world, gworld, nvox, tid);
-  __syncthreads();
-  if (tid < kWave) {
-    float x, y, z, yaw, pitch;
-    if (from_state) {
-      x = p.positions[b * 3 + 0]; y = p.positions[b * 3 + 1]; z = p.positions[b * 3 + 2];
-      yaw = p.yaws[b]; pitch = p.pitches[b];
-      if (tid == 0) {
-        p.cameras[b * 5 + 0] = x;
-        p.cameras[b * 5 + 1] = y;
-        p.cameras[b * 5 + 2] = z;
-        p.cameras[b * 5 + 3] = yaw;
-        p.cameras[b * 5 + 4] = pitch;
-      }
-    } else {
-      x = p.cameras[b * 5 + 0]; y = p.cameras[b * 5 + 1]; z = p.cameras[b * 5 + 2];
-      yaw = p.cameras[b * 5 + 3]; pitch = p.cameras[b * 5 + 4];
-    }
-    float sy, cy, sp, cp;
-    mc_sincosf(yaw, &sy, &cy);
-    mc_sincosf(pitch, &sp, &cp);
-    const Frame f = make_frame(p, x, y, z, sy, cy, sp, cp, s_world);
-    if (tid == 0) s_frame = f;
+  int8_t* world = lds + e * world_stride_bytes(p.X, p.Y, p.Z);
+  int8_t* gworld = p.voxels + (int64_t)b * nvox;
+  bool any_done = false;
+  if (KIND == kItemStep && p.done_mode == 0 && p.ring) {
+    any_done = p.ring[p.slot] != 0u;
+    if (b == 0 && et == 0) p.ring[(p.slot + 2) % 3] = 0u;
+  }
+  const bool regen = (KIND == kItemResetRender) && (p.ring[3] != 0u);
+  if (active) {
+    if (regen) store_fresh_world(p, gworld, world, nvox, et, ent);
+    else stage_world(world, gworld, nvox, et, ent);
   }
   __syncthreads();
-  const Frame f = s_frame;
-  render_env<OBS, NT>(p, b, s_world, s_pal, f, tid);
+  if (active && et < kWave) {
+    Frame f;
+    if (KIND == kItemStep) f = env_update(p, b, world, gworld, any_done);
+    else if (KIND == kItemResetRender) f = reset_update(p, b, world);
+    else f = camera_update(p, b, world, KIND == kItemRenderState);
+    if (et == 0) frames[e] = f;
+  }
+  __syncthreads();
+  if (active) {
+    const Frame f = frames[e];
+    if (obs_mode == 0) render_env<0>(p, b, world, pal, f, et, ent);
+    else render_env<1>(p, b, world, pal, f, et, ent);
+    if (KIND == kItemStep && any_done) store_fresh_world(p, gworld, nullptr, nvox, et, ent);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// One group, one item per workgroup: grid = ceil(B / epw).
+template <int NT, int KIND>
+__global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
+  extern __shared__ __align__(16) int8_t s_lds[];
+  __shared__ float s_pal[kPaletteLen];
+  __shared__ Frame s_frames[NT / kWave];
+  load_palette(s_pal, threadIdx.x);
+  const int epw = (NT / kWave) / wpe;
+  run_item<NT, KIND>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_frames);
+}
+
+// Several env groups (uneven batch sizes, world sizes, resolutions) in one
+// persistent launch.  Items are numbered group by group (groups ordered by the
+// host, most expensive first); each workgroup takes the next item from an
+// atomic counter until none is left, and the last workgroup to finish resets
+// the counters for the next launch.
+constexpr int kMaxGroups = 64;
+
+struct MultiEntry {
+  const Params* desc;      // static per-group parameters (device memory)
+  const void* actions;
+  const float* look;
+  uint8_t* dones;
+  int action_kind;
+  int slot;
+  int item_begin;          // first global item of this group
+  int wpe;
+  int obs_mode;
+  int pad;
+};
+
+struct MultiArgs {
+  int n_groups;
+  int total_items;
+  int persistent;
+  uint32_t* counters;          // [2] device words, zero between launches
+  const MultiEntry* entries;   // [n_groups] device memory
+};
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
+  extern __shared__ __align__(16) int8_t s_lds[];
+  __shared__ float s_pal[kPaletteLen];
+  __shared__ Frame s_frames[NT / kWave];
+  __shared__ int s_begin[kMaxGroups];
+  __shared__ int s_item;
+  load_palette(s_pal, threadIdx.x);
+  if (threadIdx.x < a.n_groups) s_begin[threadIdx.x] = a.entries[threadIdx.x].item_begin;
+  int item = blockIdx.x;
+  for (;;) {
+    if (a.persistent) {
+      __syncthreads();  // the previous item is done with LDS and s_item
+      if (threadIdx.x == 0) s_item = (int)atomicAdd(&a.counters[0], 1u);
+    }
+    __syncthreads();
+    if (a.persistent) item = s_item;
+    if (item >= a.total_items) break;
+    int g = 0;
+    while (g + 1 < a.n_groups && s_begin[g + 1] <= item) ++g;
+    const MultiEntry& me = a.entries[g];
+    Params p = *me.desc;
+    p.actions = me.actions;
+    p.look = me.look;
+    p.dones = me.dones;
+    p.action_kind = me.action_kind;
+    p.slot = me.slot;
+    const int wpe = me.wpe;
+    const int epw = (NT / kWave) / wpe;
+    run_item<NT, kItemStep>(p, me.obs_mode, (item - s_begin[g]) * epw, wpe, s_lds, s_pal,
+                            s_frames);
+    if (!a.persistent) break;
+  }
+  if (a.persistent && threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t prev = atomicAdd(&a.counters[1], 1u);
+    if (prev == gridDim.x - 1) {
+      a.counters[0] = 0u;
+      a.counters[1] = 0u;
+      __threadfence();
+    }
+  }
 }
 
 // Episode-ring set-up, one workgroup of 1024 threads.
@@ -758,12 +837,11 @@ __global__ __launch_bounds__(1024) void k_ring_setup(Params p, int mode) {
 }
 
 // Masked regeneration of whole worlds (mask NULL = all); one workgroup per env.
-template <int NT>
-__global__ __launch_bounds__(NT) void k_regen(Params p) {
+__global__ __launch_bounds__(256) void k_regen(Params p) {
   const int b = blockIdx.x;
   if (p.mask && !p.mask[b]) return;
   const int nvox = p.X * p.Y * p.Z;
-  store_fresh_world<NT>(p, p.voxels + (int64_t)b * nvox, nullptr, nvox, threadIdx.x);
+  store_fresh_world(p, p.voxels + (int64_t)b * nvox, nullptr, nvox, threadIdx.x, 256);
 }
 
 // generate_world_kernel (src/kernels.cu:1793-1828), 16 voxels per thread.

@@ -4,9 +4,9 @@ Drop-in for Infatoshi/mine.cu's Python API (minecu/__init__.py): same
 ``MineEnv`` constructor, ``reset(mask)``, ``step(buttons, look)``, tensor
 attributes and layouts, ``BlockType``, ``EnvConfig``/``TrainConfig``.  Every
 step runs as hand-written HIP kernels for CDNA4 through libminecu_hip.so:
-one episode pre-pass plus one fused step kernel per call (decode, physics,
-break/place raycast, respawn, render from an LDS-staged world, regeneration),
-with no host synchronisation.
+one fused kernel launch per call (decode, physics, break/place raycast,
+respawn, render from an LDS-staged world, regeneration), with no host
+synchronisation.  ``EnvBatch`` steps several groups in one launch.
 
 Example::
 
@@ -35,7 +35,7 @@ _lib.load()  # fail loudly at import when the HIP library is missing
 from . import _C  # noqa: E402  (reference-named tensor functions)
 
 __version__ = "0.1.0"
-__all__ = ["MineEnv", "BlockType", "EnvConfig", "TrainConfig"]
+__all__ = ["MineEnv", "BlockType", "EnvConfig", "TrainConfig", "EnvBatch"]
 
 
 class BlockType:
@@ -274,19 +274,33 @@ class MineEnv:
         """
         B = self.batch_size
         dones = torch.empty((B,), dtype=torch.bool, device=self.device)
-        if look is None:
-            acts = self._as_device(buttons, torch.int64, (B,), "actions")
-            st = _lib.load().mc_env_step(self._handle, ctypes.c_void_p(acts.data_ptr()),
-                                         _lib.MC_ACT_DISCRETE, None,
-                                         ctypes.c_void_p(dones.data_ptr()), self._stream())
-        else:
-            bt = self._as_device(buttons, torch.int8, (B, 8), "buttons")
-            lk = self._as_device(look, torch.float32, (B, 2), "look")
-            st = _lib.load().mc_env_step(self._handle, ctypes.c_void_p(bt.data_ptr()),
-                                         _lib.MC_ACT_BUTTONS, ctypes.c_void_p(lk.data_ptr()),
-                                         ctypes.c_void_p(dones.data_ptr()), self._stream())
+        a, kind, lk = self._prepare_actions(buttons, look)
+        st = _lib.load().mc_env_step(self._handle, ctypes.c_void_p(a.data_ptr()), kind,
+                                     ctypes.c_void_p(lk.data_ptr()) if lk is not None else None,
+                                     ctypes.c_void_p(dones.data_ptr()), self._stream())
         _lib.check(st, "step")
         return self.obs_buffer, self.rewards, dones
+
+    def _prepare_actions(self, buttons, look):
+        """Validate one step's actions; returns (actions, kind, look)."""
+        B = self.batch_size
+        if look is None:
+            return self._as_device(buttons, torch.int64, (B,), "actions"), _lib.MC_ACT_DISCRETE, None
+        bt = self._as_device(buttons, torch.int8, (B, 8), "buttons")
+        lk = self._as_device(look, torch.float32, (B, 2), "look")
+        return bt, _lib.MC_ACT_BUTTONS, lk
+
+    @property
+    def shape(self) -> Tuple[int, int]:
+        """Launch shape (threads per workgroup, waves per env) of this group."""
+        t, w = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().mc_env_get_shape(self._handle, ctypes.byref(t), ctypes.byref(w)))
+        return t.value, w.value
+
+    def set_shape(self, threads: int = 0, waves_per_env: int = 0) -> None:
+        """Override the launch shape ((0, 0) restores the default).  Results
+        do not depend on it."""
+        _lib.check(_lib.load().mc_env_set_shape(self._handle, threads, waves_per_env), "set_shape")
 
     def _render(self) -> torch.Tensor:
         """minecu/__init__.py:446-466: cameras from state, then render."""
@@ -346,3 +360,6 @@ class MineEnv:
         elif t.dtype != dtype:
             raise RuntimeError(f"{name} must be {dtype} (got {t.dtype})")
         return t.to(device=self.device, dtype=dtype).contiguous()
+
+
+from .batch import EnvBatch  # noqa: E402
