@@ -256,6 +256,51 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
 int mc_batch_tuned_shape(const mc_batch* b, const mc_config* cfg, int* threads,
                          int* waves_per_env);
 
+/* ---- serving (no reference counterpart) ----------------------------------
+ * A continuous-batching server: a loop thread owns n_streams HIP streams and
+ * merges the pending step requests of every admitted group (at most one in
+ * flight per group; up to 64 groups per launch) into one mc_batch_step
+ * launch per free stream, so requests arriving while a launch runs join the
+ * next one and overlap it on the device.  Groups are mc_env handles whose
+ * state stays resident; admission is FIFO under an HBM budget (`bytes` per
+ * group as the caller accounts it).  Latency = completion seen by the loop
+ * minus submit time.  Action/look/dones pointers must stay valid until the
+ * request completes; a group's outputs are those of its last completed step.
+ * mc_server_autotune must not run while requests are pending. */
+
+typedef struct mc_server mc_server;
+
+typedef struct mc_server_record {
+  int64_t req_id;
+  int64_t group;
+  int64_t t_submit_ns;   /* steady clock */
+  int64_t t_launch_ns;
+  int64_t t_done_ns;     /* -1: the launch failed */
+} mc_server_record;
+
+enum { MC_GROUP_WAITING = 0, MC_GROUP_ADMITTED = 1, MC_GROUP_REMOVED = 2 };
+
+int mc_server_create(int64_t hbm_budget_bytes, int n_streams, mc_server** out);
+int mc_server_destroy(mc_server* s);
+const char* mc_server_last_error(mc_server* s);
+int mc_server_add_group(mc_server* s, mc_env* env, int64_t bytes, int batch_size, int* gid);
+int mc_server_group_state(mc_server* s, int gid, int* state);
+int mc_server_remove_group(mc_server* s, int gid);
+int mc_server_submit(mc_server* s, int gid, const void* actions, int action_kind,
+                     const float* look, uint8_t* dones, int64_t* req_id);
+/* Closed loop: after each completion the group's next of `count` steps is
+ * submitted at once (actions at actions_base + k * stride_bytes). */
+int mc_server_set_replay(mc_server* s, int gid, const void* actions_base, int64_t stride_bytes,
+                         int64_t count, int action_kind, const float* look_base,
+                         int64_t look_stride_bytes, uint8_t* dones);
+int mc_server_start_replay(mc_server* s, int gid);
+/* Blocks until no request is pending, in flight or left to replay. */
+int mc_server_drain(mc_server* s);
+int mc_server_take_records(mc_server* s, mc_server_record* out, int max, int* n);
+int mc_server_stats(mc_server* s, int64_t* launches, int64_t* env_steps, int64_t* requests,
+                    int64_t* used_bytes);
+int mc_server_autotune(mc_server* s, int steps, void* stream);
+
 /* ---- test hooks -------------------------------------------------------- */
 
 /* Device evaluation of the shared sin/cos (mc_math.h) on n inputs. */

@@ -1,0 +1,434 @@
+// mc_server.cpp - continuous-batching step server, built only on the public
+// C ABI (minecu_hip.h): a loop thread merges the pending step requests of all
+// admitted env groups into one mc_batch_step launch per free stream, retires
+// launches as their completion events fire, and records per-request latency.
+//
+// Groups stay resident on the device for their whole session.  Admission is
+// by an HBM budget: a group whose bytes do not fit waits (FIFO) until earlier
+// groups are removed.  A group has at most one request on the device at a
+// time (its steps are sequential); requests that arrive meanwhile queue and
+// join the next launch, which can run on another stream while the current one
+// drains.
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "minecu_hip.h"
+
+namespace {
+
+int64_t now_ns() {
+  return std::chrono::duration_cast<std::chrono::nanoseconds>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+thread_local char g_serr[256] = "";
+
+int sfail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_serr, sizeof g_serr, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+struct Req {
+  int64_t id;
+  int gid;
+  const void* actions;
+  int kind;
+  const float* look;
+  uint8_t* dones;
+  int64_t t_submit;
+};
+
+enum GroupState { kWaiting = 0, kAdmitted = 1, kRemoved = 2 };
+
+struct Group {
+  mc_env* env;
+  int64_t bytes;
+  int batch;
+  int state;
+  bool busy;       // a request of this group is on the device
+  int pending;     // queued requests
+  // Closed-loop replay: after each completion, submit the next of `rep_count`
+  // steps whose actions start at rep_base (+ k * rep_stride bytes).
+  const char* rep_base;
+  int64_t rep_stride;
+  int64_t rep_count;
+  int64_t rep_next;
+  int rep_kind;
+  const char* rep_look;
+  int64_t rep_look_stride;
+  uint8_t* rep_dones;
+};
+
+struct Launch {
+  hipEvent_t ev;
+  int stream;
+  std::vector<Req> reqs;
+  int64_t t_launch;
+};
+
+}  // namespace
+
+struct mc_server {
+  std::mutex mu;
+  std::condition_variable cv;
+  int64_t budget = 0;
+  int64_t used = 0;
+  std::vector<Group> groups;
+  std::deque<int> waiting;
+  std::deque<Req> pending;
+  std::deque<Launch> inflight;
+  std::vector<hipStream_t> streams;
+  std::vector<char> stream_busy;
+  std::vector<hipEvent_t> events;
+  mc_batch* batch = nullptr;
+  std::thread th;
+  bool running = false;
+  int device = 0;
+  std::vector<mc_server_record> done;
+  int64_t next_id = 1;
+  int64_t launches = 0, env_steps = 0, grouped = 0, failed = 0;
+  std::string last_err;
+
+  void admit_waiting() {
+    while (!waiting.empty()) {
+      Group& g = groups[waiting.front()];
+      if (used + g.bytes > budget) break;
+      used += g.bytes;
+      g.state = kAdmitted;
+      waiting.pop_front();
+    }
+  }
+
+  void retire_locked(int64_t t) {
+    for (auto it = inflight.begin(); it != inflight.end();) {
+      if (hipEventQuery(it->ev) != hipSuccess) {
+        ++it;
+        continue;
+      }
+      for (const Req& r : it->reqs) {
+        done.push_back(mc_server_record{r.id, r.gid, r.t_submit, it->t_launch, t});
+        Group& g = groups[r.gid];
+        g.busy = false;
+        if (g.rep_base && g.rep_next < g.rep_count) {
+          const int64_t k = g.rep_next++;
+          pending.push_back(Req{next_id++, r.gid, g.rep_base + k * g.rep_stride, g.rep_kind,
+                                g.rep_look ? (const float*)(g.rep_look + k * g.rep_look_stride)
+                                           : nullptr,
+                                g.rep_dones, t});
+          g.pending++;
+        }
+      }
+      stream_busy[it->stream] = 0;
+      events.push_back(it->ev);
+      it = inflight.erase(it);
+    }
+  }
+
+  void loop() {
+    (void)hipSetDevice(device);
+    std::vector<Req> take;
+    std::vector<mc_env*> envs;
+    std::vector<const void*> acts;
+    std::vector<int> kinds;
+    std::vector<const float*> looks;
+    std::vector<uint8_t*> dns;
+    for (;;) {
+      int stream = -1;
+      hipEvent_t ev = nullptr;
+      take.clear();
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        retire_locked(now_ns());
+        if (!running && pending.empty() && inflight.empty()) break;
+        for (size_t i = 0; i < streams.size(); ++i)
+          if (!stream_busy[i]) {
+            stream = (int)i;
+            break;
+          }
+        if (stream >= 0 && !events.empty()) {
+          for (auto it = pending.begin();
+               it != pending.end() && (int)take.size() < 64;) {
+            Group& g = groups[it->gid];
+            bool dup = false;
+            for (const Req& q : take) dup |= (q.gid == it->gid);
+            if (g.state == kAdmitted && !g.busy && !dup) {
+              take.push_back(*it);
+              g.busy = true;
+              g.pending--;
+              it = pending.erase(it);
+            } else {
+              ++it;
+            }
+          }
+        }
+        if (take.empty()) {
+          if (!inflight.empty()) {
+            lk.unlock();
+            std::this_thread::yield();
+          } else {
+            cv.wait_for(lk, std::chrono::milliseconds(2));
+          }
+          continue;
+        }
+        stream_busy[stream] = 1;
+        ev = events.back();
+        events.pop_back();
+      }
+      envs.clear();
+      acts.clear();
+      kinds.clear();
+      looks.clear();
+      dns.clear();
+      int64_t steps = 0;
+      for (const Req& r : take) {
+        envs.push_back(groups[r.gid].env);
+        acts.push_back(r.actions);
+        kinds.push_back(r.kind);
+        looks.push_back(r.look);
+        dns.push_back(r.dones);
+        steps += groups[r.gid].batch;
+      }
+      const int64_t t_launch = now_ns();
+      int st = mc_batch_step(batch, envs.data(), acts.data(), kinds.data(), looks.data(),
+                             dns.data(), (int)take.size(), streams[stream]);
+      if (st == MC_OK && hipEventRecord(ev, streams[stream]) != hipSuccess) st = MC_E_HIP;
+      std::lock_guard<std::mutex> lk(mu);
+      if (st != MC_OK) {
+        last_err = mc_last_error();
+        failed += (int64_t)take.size();
+        for (const Req& r : take) {
+          groups[r.gid].busy = false;
+          done.push_back(mc_server_record{r.id, r.gid, r.t_submit, t_launch, -1});
+        }
+        stream_busy[stream] = 0;
+        events.push_back(ev);
+        continue;
+      }
+      launches++;
+      grouped += (int64_t)take.size();
+      env_steps += steps;
+      inflight.push_back(Launch{ev, stream, take, t_launch});
+    }
+  }
+};
+
+extern "C" {
+
+const char* mc_server_last_error(mc_server* s) {
+  if (!s) return g_serr;
+  std::lock_guard<std::mutex> lk(s->mu);
+  return s->last_err.empty() ? g_serr : s->last_err.c_str();
+}
+
+int mc_server_create(int64_t hbm_budget_bytes, int n_streams, mc_server** out) {
+  if (!out) return sfail(MC_E_INVALID, "out is NULL");
+  *out = nullptr;
+  if (hbm_budget_bytes <= 0) return sfail(MC_E_INVALID, "budget must be positive");
+  if (n_streams < 1 || n_streams > 8) return sfail(MC_E_INVALID, "n_streams must be in [1, 8]");
+  mc_server* s = new (std::nothrow) mc_server;
+  if (!s) return sfail(MC_E_INVALID, "out of host memory");
+  s->budget = hbm_budget_bytes;
+  hipError_t e = hipGetDevice(&s->device);
+  for (int i = 0; i < n_streams && e == hipSuccess; ++i) {
+    hipStream_t st = nullptr;
+    e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+      s->streams.push_back(st);
+      s->stream_busy.push_back(0);
+    }
+  }
+  for (int i = 0; i < 2 * n_streams && e == hipSuccess; ++i) {
+    hipEvent_t ev = nullptr;
+    e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e == hipSuccess) s->events.push_back(ev);
+  }
+  int r = (e == hipSuccess) ? mc_batch_create(&s->batch) : MC_E_HIP;
+  if (e != hipSuccess || r != MC_OK) {
+    mc_server_destroy(s);
+    return sfail(MC_E_HIP, "server create failed");
+  }
+  s->running = true;
+  s->th = std::thread([s] { s->loop(); });
+  *out = s;
+  return MC_OK;
+}
+
+int mc_server_destroy(mc_server* s) {
+  if (!s) return MC_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->running = false;
+    // Requests that were never launched are dropped.
+    s->pending.clear();
+  }
+  s->cv.notify_all();
+  if (s->th.joinable()) s->th.join();
+  for (hipEvent_t ev : s->events) (void)hipEventDestroy(ev);
+  for (auto& l : s->inflight) (void)hipEventDestroy(l.ev);
+  for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
+  if (s->batch) mc_batch_destroy(s->batch);
+  delete s;
+  return MC_OK;
+}
+
+int mc_server_add_group(mc_server* s, mc_env* env, int64_t bytes, int batch_size, int* gid) {
+  if (!s || !env || !gid || bytes < 0 || batch_size < 1) return sfail(MC_E_INVALID, "bad argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (bytes > s->budget) return sfail(MC_E_INVALID, "group needs more than the whole budget");
+  Group g{};
+  g.env = env;
+  g.bytes = bytes;
+  g.batch = batch_size;
+  g.state = kWaiting;
+  *gid = (int)s->groups.size();
+  s->groups.push_back(g);
+  s->waiting.push_back(*gid);
+  s->admit_waiting();
+  return MC_OK;
+}
+
+int mc_server_group_state(mc_server* s, int gid, int* state) {
+  if (!s || !state) return sfail(MC_E_INVALID, "bad argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (gid < 0 || gid >= (int)s->groups.size()) return sfail(MC_E_INVALID, "no group %d", gid);
+  *state = s->groups[gid].state;
+  return MC_OK;
+}
+
+int mc_server_remove_group(mc_server* s, int gid) {
+  if (!s) return sfail(MC_E_INVALID, "server is NULL");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (gid < 0 || gid >= (int)s->groups.size()) return sfail(MC_E_INVALID, "no group %d", gid);
+  Group& g = s->groups[gid];
+  if (g.busy || g.pending > 0) return sfail(MC_E_INVALID, "group %d has requests in flight", gid);
+  if (g.state == kAdmitted) s->used -= g.bytes;
+  if (g.state == kWaiting)
+    s->waiting.erase(std::remove(s->waiting.begin(), s->waiting.end(), gid), s->waiting.end());
+  g.state = kRemoved;
+  g.rep_base = nullptr;
+  s->admit_waiting();
+  return MC_OK;
+}
+
+int mc_server_submit(mc_server* s, int gid, const void* actions, int action_kind,
+                     const float* look, uint8_t* dones, int64_t* req_id) {
+  if (!s || !actions) return sfail(MC_E_INVALID, "bad argument");
+  if (action_kind == MC_ACT_BUTTONS && !look) return sfail(MC_E_INVALID, "look is NULL");
+  int64_t id;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (gid < 0 || gid >= (int)s->groups.size()) return sfail(MC_E_INVALID, "no group %d", gid);
+    if (s->groups[gid].state != kAdmitted)
+      return sfail(MC_E_INVALID, "group %d is not admitted", gid);
+    id = s->next_id++;
+    s->pending.push_back(Req{id, gid, actions, action_kind, look, dones, now_ns()});
+    s->groups[gid].pending++;
+  }
+  s->cv.notify_one();
+  if (req_id) *req_id = id;
+  return MC_OK;
+}
+
+int mc_server_set_replay(mc_server* s, int gid, const void* actions_base, int64_t stride_bytes,
+                         int64_t count, int action_kind, const float* look_base,
+                         int64_t look_stride_bytes, uint8_t* dones) {
+  if (!s || !actions_base || count < 0) return sfail(MC_E_INVALID, "bad argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (gid < 0 || gid >= (int)s->groups.size()) return sfail(MC_E_INVALID, "no group %d", gid);
+  Group& g = s->groups[gid];
+  g.rep_base = (const char*)actions_base;
+  g.rep_stride = stride_bytes;
+  g.rep_count = count;
+  g.rep_next = 0;
+  g.rep_kind = action_kind;
+  g.rep_look = (const char*)look_base;
+  g.rep_look_stride = look_stride_bytes;
+  g.rep_dones = dones;
+  return MC_OK;
+}
+
+int mc_server_start_replay(mc_server* s, int gid) {
+  if (!s) return sfail(MC_E_INVALID, "server is NULL");
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (gid < 0 || gid >= (int)s->groups.size()) return sfail(MC_E_INVALID, "no group %d", gid);
+    Group& g = s->groups[gid];
+    if (!g.rep_base || g.rep_next >= g.rep_count || g.state != kAdmitted)
+      return sfail(MC_E_INVALID, "group %d has no replay to start", gid);
+    const int64_t k = g.rep_next++;
+    s->pending.push_back(Req{s->next_id++, gid, g.rep_base + k * g.rep_stride, g.rep_kind,
+                             g.rep_look ? (const float*)(g.rep_look + k * g.rep_look_stride) : nullptr,
+                             g.rep_dones, now_ns()});
+    g.pending++;
+  }
+  s->cv.notify_one();
+  return MC_OK;
+}
+
+int mc_server_drain(mc_server* s) {
+  if (!s) return sfail(MC_E_INVALID, "server is NULL");
+  for (;;) {
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      bool replay_left = false;
+      for (const Group& g : s->groups) replay_left |= (g.rep_base && g.rep_next < g.rep_count);
+      if (s->pending.empty() && s->inflight.empty() && !replay_left) break;
+    }
+    std::this_thread::sleep_for(std::chrono::microseconds(100));
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  return s->failed ? sfail(MC_E_HIP, "%lld requests failed: %s", (long long)s->failed,
+                           s->last_err.c_str())
+                   : MC_OK;
+}
+
+int mc_server_take_records(mc_server* s, mc_server_record* out, int max, int* n) {
+  if (!s || !n || (max > 0 && !out)) return sfail(MC_E_INVALID, "bad argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  const int k = std::min<int>(max, (int)s->done.size());
+  std::copy(s->done.begin(), s->done.begin() + k, out);
+  s->done.erase(s->done.begin(), s->done.begin() + k);
+  *n = k;
+  return MC_OK;
+}
+
+int mc_server_stats(mc_server* s, int64_t* launches, int64_t* env_steps, int64_t* requests,
+                    int64_t* used_bytes) {
+  if (!s) return sfail(MC_E_INVALID, "server is NULL");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (launches) *launches = s->launches;
+  if (env_steps) *env_steps = s->env_steps;
+  if (requests) *requests = s->grouped;
+  if (used_bytes) *used_bytes = s->used;
+  return MC_OK;
+}
+
+int mc_server_autotune(mc_server* s, int steps, void* stream) {
+  if (!s) return sfail(MC_E_INVALID, "server is NULL");
+  std::vector<mc_env*> envs;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (const Group& g : s->groups)
+      if (g.state == kAdmitted && !g.busy && g.pending == 0) envs.push_back(g.env);
+  }
+  if (envs.empty()) return MC_OK;
+  return mc_batch_autotune(s->batch, envs.data(), (int)envs.size(), steps, stream);
+}
+
+}  // extern "C"

@@ -76,6 +76,19 @@ SIGNATURES = {
     "mc_batch_step": ([_P, _P, _P, _P, _P, _P, _I, _P], _I),
     "mc_batch_autotune": ([_P, _P, _I, _I, _P], _I),
     "mc_batch_tuned_shape": ([_P, _P, _P, _P], _I),
+    "mc_server_create": ([ctypes.c_int64, _I, _P], _I),
+    "mc_server_destroy": ([_P], _I),
+    "mc_server_last_error": ([_P], ctypes.c_char_p),
+    "mc_server_add_group": ([_P, _P, ctypes.c_int64, _I, _P], _I),
+    "mc_server_group_state": ([_P, _I, _P], _I),
+    "mc_server_remove_group": ([_P, _I], _I),
+    "mc_server_submit": ([_P, _I, _P, _I, _P, _P, _P], _I),
+    "mc_server_set_replay": ([_P, _I, _P, ctypes.c_int64, ctypes.c_int64, _I, _P, ctypes.c_int64, _P], _I),
+    "mc_server_start_replay": ([_P, _I], _I),
+    "mc_server_drain": ([_P], _I),
+    "mc_server_take_records": ([_P, _P, _I, _P], _I),
+    "mc_server_stats": ([_P, _P, _P, _P, _P], _I),
+    "mc_server_autotune": ([_P, _I, _P], _I),
     "mc_debug_sincosf": ([_P, _P, _P, _I, _P], _I),
 }
 

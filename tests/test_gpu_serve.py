@@ -1,0 +1,116 @@
+"""Continuous-batching server: every group's result is the same alone and in
+the served mix (bit for bit), HBM-budget admission, request timestamps."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [
+    dict(batch_size=48, world_size=16, render_width=32, render_height=32, obs_dtype="uint8",
+         episode_length=9),
+    dict(batch_size=16, world_size=32, render_width=64, render_height=64, obs_dtype="uint8",
+         episode_length=13),
+    dict(batch_size=100, world_size=16, render_width=64, render_height=64, obs_dtype="uint8",
+         episode_length=None),
+    dict(batch_size=20, world_size=32, render_width=32, render_height=32, obs_dtype="uint8",
+         episode_length=6),
+]
+
+
+def _make(kw):
+    import minecu
+    e = minecu.MineEnv(device="cuda", **kw)
+    e.reset()
+    return e
+
+
+def _state(e):
+    return {k: getattr(e, k).cpu().numpy().copy() for k in e._STATE_KEYS}
+
+
+def _same(a, b, where):
+    for k in a:
+        assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), f"{where}: {k}"
+
+
+def _acts(T, B, seed):
+    g = torch.Generator().manual_seed(seed)
+    return torch.randint(0, 11, (T, B), generator=g).cuda()
+
+
+def test_served_equals_alone_interleaved_submits():
+    from minecu.serve import StepServer
+    T = 14
+    served = [_make(kw) for kw in SHAPES]
+    alone = [_make(kw) for kw in SHAPES]
+    acts = [_acts(T, kw["batch_size"], 10 + i) for i, kw in enumerate(SHAPES)]
+    torch.cuda.synchronize()
+    srv = StepServer(n_streams=2)
+    gids = [srv.add_group(e) for e in served]
+    # interleave: group i's step t is submitted in a shuffled global order
+    # that keeps each group's own steps in sequence
+    rng = np.random.default_rng(0)
+    order = np.concatenate([np.full(T, i) for i in range(len(SHAPES))])
+    rng.shuffle(order)
+    nxt = [0] * len(SHAPES)
+    for i in order:
+        srv.submit(gids[i], acts[i][nxt[i]])
+        nxt[i] += 1
+    srv.drain()
+    rec = srv.records()
+    assert len(rec) == T * len(SHAPES)
+    assert (rec["t_done_ns"] >= rec["t_launch_ns"]).all()
+    assert (rec["t_launch_ns"] >= rec["t_submit_ns"]).all()
+    st = srv.stats()
+    assert st["requests"] == T * len(SHAPES) and st["launches"] <= st["requests"]
+    for i, e in enumerate(alone):
+        for t in range(T):
+            e.step(acts[i][t])
+    torch.cuda.synchronize()
+    for i in range(len(SHAPES)):
+        _same(_state(alone[i]), _state(served[i]), f"group {i}")
+    srv.close()
+
+
+def test_served_closed_loop_replay_equals_alone():
+    from minecu.serve import StepServer
+    T = 20
+    served = [_make(kw) for kw in SHAPES]
+    alone = [_make(kw) for kw in SHAPES]
+    acts = [_acts(T, kw["batch_size"], 30 + i) for i, kw in enumerate(SHAPES)]
+    torch.cuda.synchronize()
+    srv = StepServer(n_streams=2)
+    gids = [srv.add_group(e) for e in served]
+    for g, a in zip(gids, acts):
+        srv.replay(g, a)
+    srv.drain()
+    assert len(srv.records()) == T * len(SHAPES)
+    for i, e in enumerate(alone):
+        for t in range(T):
+            e.step(acts[i][t])
+    torch.cuda.synchronize()
+    for i in range(len(SHAPES)):
+        _same(_state(alone[i]), _state(served[i]), f"group {i}")
+    srv.close()
+
+
+def test_admission_by_hbm_budget():
+    from minecu.serve import ADMITTED, REMOVED, WAITING, StepServer, env_bytes
+    envs = [_make(SHAPES[0]) for _ in range(3)]
+    nb = env_bytes(envs[0])
+    srv = StepServer(hbm_budget_bytes=2 * nb + nb // 2, n_streams=1)
+    g = [srv.add_group(e) for e in envs]
+    assert [srv.state(x) for x in g] == [ADMITTED, ADMITTED, WAITING]
+    a = _acts(1, envs[0].batch_size, 0)[0]
+    with pytest.raises(Exception):
+        srv.submit(g[2], a)
+    srv.submit(g[0], a)
+    srv.drain()
+    srv.remove_group(g[0])
+    assert [srv.state(x) for x in g] == [REMOVED, ADMITTED, ADMITTED]
+    assert srv.stats()["used_bytes"] == 2 * nb
+    srv.submit(g[2], a)
+    srv.drain()
+    assert len(srv.records()) == 2
+    srv.close()
