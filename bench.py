@@ -122,6 +122,103 @@ def run_gpu(w, obs, steps, warmup):
     return total_ms, wall, per_launch_ms
 
 
+def serving_clients(seed=4, n_clients=64):
+    """64 clients: batch log-uniform in [16, 4096], world 16^3 or 32^3, obs
+    32x32 or 64x64 uint8."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_clients):
+        B = int(round(float(np.exp(rng.uniform(np.log(16), np.log(4096))))))
+        out.append(dict(batch=B, world=int(rng.choice([16, 32])), res=int(rng.choice([32, 64]))))
+    return out
+
+
+def run_serving(seconds=60.0, rate=200.0, seed=4, budget_gb=64, closed_steps=200):
+    """Open loop: Poisson step requests per client at `rate`/s for `seconds`
+    through the continuous-batching server; per-request latency = completion
+    seen by the server loop minus submit.  Closed loop: every client steps
+    back-to-back (`closed_steps` each) to show capacity."""
+    import numpy as np
+    import torch
+    import minecu
+    from minecu.serve import StepServer
+    dev = torch.device("cuda", 0)
+    clients = serving_clients(seed)
+    envs, acts = [], []
+    g = torch.Generator(device=dev).manual_seed(seed)
+    for c in clients:
+        e = minecu.MineEnv(batch_size=c["batch"], world_size=c["world"], render_width=c["res"],
+                           render_height=c["res"], device=str(dev), obs_dtype="uint8",
+                           episode_length=60)
+        e.reset()
+        envs.append(e)
+        acts.append(torch.randint(0, 11, (256, c["batch"]), generator=g, device=dev))
+    torch.cuda.synchronize()
+    srv = StepServer(hbm_budget_bytes=budget_gb << 30, n_streams=2)
+    gids = [srv.add_group(e) for e in envs]
+    srv.autotune(8)
+    torch.cuda.synchronize()
+    # Arrival schedule: independent Poisson processes, merged.
+    rng = np.random.default_rng(seed)
+    times, who = [], []
+    for i in range(len(clients)):
+        n = rng.poisson(rate * seconds * 1.2) + 16
+        t = np.cumsum(rng.exponential(1.0 / rate, n))
+        t = t[t < seconds]
+        times.append(t)
+        who.append(np.full(len(t), i))
+    times = np.concatenate(times)
+    who = np.concatenate(who)
+    order = np.argsort(times, kind="stable")
+    times, who = times[order], who[order]
+    count = [0] * len(clients)
+    t0 = time.perf_counter()
+    for t, i in zip(times.tolist(), who.tolist()):
+        while True:
+            dt = t0 + t - time.perf_counter()
+            if dt <= 0:
+                break
+            if dt > 0.002:
+                time.sleep(dt - 0.001)
+        srv.submit(gids[i], acts[i][count[i] % 256])
+        count[i] += 1
+    srv.drain()
+    wall = time.perf_counter() - t0
+    rec = srv.records()
+    lat_ms = (rec["t_done_ns"] - rec["t_submit_ns"]) / 1e6
+    batch_of = np.array([c["batch"] for c in clients])
+    served = int(batch_of[rec["group"]].sum())
+    span_s = (rec["t_done_ns"].max() - rec["t_submit_ns"].min()) / 1e9
+    st = srv.stats()
+    open_loop = {
+        "requests": int(len(rec)), "offered_env_steps_per_s": float(batch_of[who].sum() / seconds),
+        "served_env_steps_per_s": served / span_s, "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)), "max_ms": float(lat_ms.max()),
+        "launches": st["launches"], "requests_per_launch": st["requests"] / max(1, st["launches"]),
+        "wall_s": wall}
+    # Closed loop (capacity): replay every client back to back.
+    reps = [acts[i][:min(256, closed_steps)].repeat((closed_steps + 255) // 256, 1)[:closed_steps]
+            .contiguous() for i in range(len(clients))]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for gid_, r in zip(gids, reps):
+        srv.replay(gid_, r)
+    srv.drain()
+    wall2 = time.perf_counter() - t1
+    rec2 = srv.records()
+    lat2 = (rec2["t_done_ns"] - rec2["t_submit_ns"]) / 1e6
+    closed = {"env_steps_per_s": float(batch_of.sum() * closed_steps / wall2),
+              "p50_ms": float(np.percentile(lat2, 50)), "p99_ms": float(np.percentile(lat2, 99)),
+              "steps_per_client": closed_steps}
+    srv.close()
+    return {"clients": len(clients), "total_batch": int(batch_of.sum()), "rate_per_client": rate,
+            "seconds": seconds, "budget_gb": budget_gb, "open_loop": open_loop,
+            "closed_loop": closed,
+            "workload": "64 clients, batch log-uniform [16,4096], world 16^3/32^3, obs 32^2/64^2 "
+                        "uint8, Poisson 200/s per client, seed 4"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +228,8 @@ def main():
     ap.add_argument("--obs", default="uint8", choices=["uint8", "float32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32", action="store_true", help="skip the float32-obs side measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -174,6 +273,9 @@ def main():
     if args.obs == "uint8" and not args.no_f32:
         f_ms, _, f_launch = run_gpu(w, "float32", min(K, 200), args.warmup)
         out["float32_obs_value"] = B * min(K, 200) / (f_ms / 1e3)
+    if not args.no_serve:
+        out["served"] = run_serving(args.serve_seconds)
+        out["served"]["closed_loop_frac_of_plain"] = out["served"]["closed_loop"]["env_steps_per_s"] / value
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args.obs)
     print(json.dumps(out))

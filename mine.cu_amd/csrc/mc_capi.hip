@@ -582,7 +582,9 @@ struct Occupancy {
 }  // namespace
 
 struct mc_batch {
-  uint32_t* counters = nullptr;               // device [2]
+  // Work counters, one pair per descriptor slot: launches in flight at the
+  // same time (on different streams) never share a counter.
+  uint32_t* counters = nullptr;               // device [kEntryRing][2]
   mc::MultiEntry* dev_entries = nullptr;      // [kEntryRing][kMaxGroups]
   mc::MultiEntry* host_entries = nullptr;     // pinned, same shape
   hipEvent_t ev[kEntryRing] = {};
@@ -628,8 +630,9 @@ int mc_batch_create(mc_batch** out) {
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev);
   const size_t tbl = sizeof(mc::MultiEntry) * mc::kMaxGroups * kEntryRing;
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->counters), 2 * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMemset(b->counters, 0, 2 * sizeof(uint32_t));
+  const size_t ctr = 2 * sizeof(uint32_t) * kEntryRing;
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->counters), ctr);
+  if (e == hipSuccess) e = hipMemset(b->counters, 0, ctr);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->dev_entries), tbl);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->host_entries), tbl, 0);
   for (int i = 0; i < kEntryRing && e == hipSuccess; ++i)
@@ -732,7 +735,7 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
   a.n_groups = n;
   a.total_items = items;
   a.persistent = b->persistent;
-  a.counters = b->counters;
+  a.counters = b->counters + 2 * slot;
   a.entries = de;
   const int r = (nt == 512) ? launch_multi<512>(b, a, lds, S(stream))
                             : launch_multi<256>(b, a, lds, S(stream));
