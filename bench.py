@@ -230,11 +230,23 @@ def main():
     ap.add_argument("--no-f32", action="store_true", help="skip the float32-obs side measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(envs shard with no per-step exchange; see the sharding notes)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe_mib:
+        import torch
+        n = args.copy_probe_mib << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "dispatches": 5}))
+        return
     w = WORKLOADS[args.workload]
     obs_bytes = 1 if args.obs == "uint8" else 4
     total_ms, wall, per_launch = run_gpu(w, args.obs, args.steps, args.warmup)

@@ -20,6 +20,15 @@
 namespace mc {
 
 constexpr int kWave = 64;
+
+// Global-address-space views: pointers loaded from memory (the batched
+// kernel's per-group descriptors) would otherwise be accessed as flat.
+#define MC_GLOBAL __attribute__((address_space(1)))
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef MC_GLOBAL v4u guint4;
+typedef MC_GLOBAL uint32_t guint32;
+typedef MC_GLOBAL v4f gfloat4;
 // Bytes of world staged in LDS per workgroup: 64 KiB less room for the
 // kernels' static LDS (palette, frames, work-item bookkeeping).
 constexpr int kMaxLdsWorld = 65536 - 1024;
@@ -156,7 +165,7 @@ __device__ __forceinline__ uint32_t pack4(int8_t a, int8_t b, int8_t c, int8_t d
 // 16 consecutive voxels starting at linear index i0 (i0 % 16 == 0), packed
 // little-endian into a uint4.  Every index below is a compile-time constant
 // after unrolling, so the chunk stays in registers.
-__device__ __forceinline__ uint4 fresh_chunk(const Params& p, int i0) {
+__device__ __forceinline__ v4u fresh_chunk(const Params& p, int i0) {
   const int XZ = p.X * p.Z;
   uint32_t w[4] = {0u, 0u, 0u, 0u};
   if ((p.Z & 15) == 0) {
@@ -188,17 +197,17 @@ __device__ __forceinline__ uint4 fresh_chunk(const Params& p, int i0) {
       w[k >> 2] |= (uint32_t)(uint8_t)fresh_voxel(p, y, x, z) << (8 * (k & 3));
     }
   }
-  return make_uint4(w[0], w[1], w[2], w[3]);
+  return v4u{w[0], w[1], w[2], w[3]};
 }
 
 // Write env b's fresh world to HBM (and to LDS when lds != nullptr).
 __device__ __forceinline__ void store_fresh_world(const Params& p, int8_t* gworld, int8_t* lds,
                                                   int nvox, int tid, int nt) {
   if ((nvox & 15) == 0) {
-    uint4* g = reinterpret_cast<uint4*>(gworld);
-    uint4* l = reinterpret_cast<uint4*>(lds);
+    guint4* g = (guint4*)gworld;
+    v4u* l = reinterpret_cast<v4u*>(lds);
     for (int c = tid; c < (nvox >> 4); c += nt) {
-      const uint4 v = fresh_chunk(p, c << 4);
+      const v4u v = fresh_chunk(p, c << 4);
       g[c] = v;
       if (lds) l[c] = v;
     }
@@ -213,13 +222,23 @@ __device__ __forceinline__ void store_fresh_world(const Params& p, int8_t* gworl
   }
 }
 
-// HBM -> LDS copy of one env's world.
+// HBM -> LDS copy of one env's world: up to 8 16-byte loads per lane in
+// flight before their LDS writes.
 __device__ __forceinline__ void stage_world(int8_t* lds, const int8_t* gworld, int nvox, int tid,
                                             int nt) {
   if ((nvox & 15) == 0) {
-    const uint4* g = reinterpret_cast<const uint4*>(gworld);
-    uint4* l = reinterpret_cast<uint4*>(lds);
-    for (int c = tid; c < (nvox >> 4); c += nt) l[c] = g[c];
+    const guint4* g = (const guint4*)gworld;
+    v4u* l = reinterpret_cast<v4u*>(lds);
+    const int n16 = nvox >> 4;
+    for (int c0 = tid; c0 < n16; c0 += 8 * nt) {
+      // Out-of-range slots reload the last chunk (in bounds) and are not stored.
+      v4u v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = g[min(c0 + k * nt, n16 - 1)];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (c0 + k * nt < n16) l[c0 + k * nt] = v[k];
+    }
   } else {
     for (int i = tid; i < nvox; i += nt) lds[i] = gworld[i];
   }
@@ -294,25 +313,49 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, 
     float tm_x = (ray_x >= 0) ? ((float)(vx + 1) - f.ex) * inv_x : ((float)vx - f.ex) * inv_x;
     float tm_y = (ray_y >= 0) ? ((float)(vy + 1) - f.ey) * inv_y : ((float)vy - f.ey) * inv_y;
     float tm_z = (ray_z >= 0) ? ((float)(vz + 1) - f.ez) * inv_z : ((float)vz - f.ez) * inv_z;
-    const int XZ = p.X * p.Z;
-    for (int step = 0; !(step >= p.max_steps || t >= vd); ++step) {
-      if ((unsigned)vx < (unsigned)p.X && (unsigned)vy < (unsigned)p.Y &&
-          (unsigned)vz < (unsigned)p.Z) {
-        const int8_t blk = world[vy * XZ + vx * p.Z + vz];
+    const int X = p.X, Y = p.Y, Z = p.Z, XZ = X * Z;
+    const int max_steps = p.max_steps;
+    // Linear voxel index kept in step with (vx, vy, vz); read only in bounds.
+    const int dix = step_x * Z, diy = step_y * XZ, diz = step_z;
+    int idx = vy * XZ + vx * Z + vz;
+    bool inb = (unsigned)vx < (unsigned)X && (unsigned)vy < (unsigned)Y && (unsigned)vz < (unsigned)Z;
+    int blk = inb ? (int)world[idx] : -1;
+    int face = -1;
+    // Each iteration is the reference's (exit checks, lookup, advance) with
+    // the advance computed branch-free and the next voxel's LDS read issued
+    // before the current voxel is tested, so its latency overlaps the test.
+    for (int step = 0;; ++step) {
+      if (step >= max_steps || t >= vd) break;
+      const bool cx = tm_x < tm_y && tm_x < tm_z;
+      const bool cy = !cx && (tm_y < tm_z);
+      const bool cz = !cx && !cy;
+      const float nt = cx ? tm_x : (cy ? tm_y : tm_z);
+      const float ntm_x = cx ? tm_x + td_x : tm_x;
+      const float ntm_y = cy ? tm_y + td_y : tm_y;
+      const float ntm_z = cz ? tm_z + td_z : tm_z;
+      const int nvx = cx ? vx + step_x : vx;
+      const int nvy = cy ? vy + step_y : vy;
+      const int nvz = cz ? vz + step_z : vz;
+      const int nidx = idx + (cx ? dix : (cy ? diy : diz));
+      const bool ninb = (unsigned)nvx < (unsigned)X && (unsigned)nvy < (unsigned)Y &&
+                        (unsigned)nvz < (unsigned)Z;
+      const int nblk = ninb ? (int)world[nidx] : -1;
+      if (inb) {
         if (blk >= 0) {
           hit_block = blk;
+          hit_face = face;
           break;
         }
       } else if (t > t_exit) {
         break;
       }
-      if (tm_x < tm_y && tm_x < tm_z) {
-        t = tm_x; tm_x += td_x; vx += step_x; hit_face = 0;
-      } else if (tm_y < tm_z) {
-        t = tm_y; tm_y += td_y; vy += step_y; hit_face = 1;
-      } else {
-        t = tm_z; tm_z += td_z; vz += step_z; hit_face = 2;
-      }
+      t = nt;
+      tm_x = ntm_x; tm_y = ntm_y; tm_z = ntm_z;
+      vx = nvx; vy = nvy; vz = nvz;
+      idx = nidx;
+      inb = ninb;
+      blk = nblk;
+      face = cx ? 0 : (cy ? 1 : 2);
     }
   }
   if (hit_block >= 0 && hit_block < 13) {
@@ -352,13 +395,13 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       const uint32_t w1 = g | (bl << 8) | (r << 16) | (g << 24);
       const uint32_t w2 = bl | (r << 8) | (g << 16) | (bl << 24);
       // Chunk c starts at byte 16c, i.e. at channel phase (16c mod 3) = c mod 3.
-      uint4* o4 = reinterpret_cast<uint4*>(out);
+      guint4* o4 = (guint4*)out;
       for (int c = tid; c < (npix * 3) >> 4; c += nt) {
         const int ph = c % 3;
         const uint32_t a = (ph == 0) ? w0 : (ph == 1) ? w1 : w2;
         const uint32_t bq = (ph == 0) ? w1 : (ph == 1) ? w2 : w0;
         const uint32_t cq = (ph == 0) ? w2 : (ph == 1) ? w0 : w1;
-        o4[c] = make_uint4(a, bq, cq, a);
+        o4[c] = v4u{a, bq, cq, a};
       }
       return;
     }
@@ -374,7 +417,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
           return to_u8(rgb[0]) | (to_u8(rgb[1]) << 8) | (to_u8(rgb[2]) << 16);
         };
         const uint32_t c0 = shade(0), c1 = shade(1), c2 = shade(2), c3 = shade(3);
-        uint32_t* o = reinterpret_cast<uint32_t*>(out + (int64_t)p0 * 3);
+        guint32* o = (guint32*)(out + (int64_t)p0 * 3);
         o[0] = c0 | (c1 << 24);
         o[1] = (c1 >> 8) | (c2 << 16);
         o[2] = (c2 >> 16) | (c3 << 8);
@@ -410,10 +453,10 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
             trace(p, f, u, v, world, pal, &vals[j * 3]);
           }
         }
-        float4* o = reinterpret_cast<float4*>(out + (int64_t)p0 * 3);
-        o[0] = make_float4(vals[0], vals[1], vals[2], vals[3]);
-        o[1] = make_float4(vals[4], vals[5], vals[6], vals[7]);
-        o[2] = make_float4(vals[8], vals[9], vals[10], vals[11]);
+        gfloat4* o = (gfloat4*)(out + (int64_t)p0 * 3);
+        o[0] = v4f{vals[0], vals[1], vals[2], vals[3]};
+        o[1] = v4f{vals[4], vals[5], vals[6], vals[7]};
+        o[2] = v4f{vals[8], vals[9], vals[10], vals[11]};
       }
     } else {
       for (int pi = tid; pi < npix; pi += nt) {
@@ -439,51 +482,90 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
 // (1536-1583), raycast break/place (1609-1679), masked reset (1775-1785),
 // camera (2330-2334), step_count update (minecu/__init__.py:435-436).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ Frame env_update(const Params& p, int b, int8_t* world,
-                                            int8_t* gworld, bool any_done) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int sc = (p.done_mode == 0) ? p.step_count[b] : 0;
-  const bool done = (p.done_mode == 0) ? ((p.episode_length >= 0) && (sc >= p.episode_length))
-                                       : (p.do_reset[b] != 0);
+// Inputs of the per-env logic, loaded before the world staging completes so
+// both HBM round trips overlap.
+struct EnvIn {
+  int32_t sc;
+  bool done;
+  uint32_t bt_lo, bt_hi;       // the 8 int8 buttons, little-endian
+  float look_yaw, look_pitch;
+  float yaw, pitch, px, py, pz, vy;
+  bool og;
+};
 
-  // Decode.
-  int8_t bt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  float look_yaw = 0.0f, look_pitch = 0.0f;
+__device__ __forceinline__ EnvIn load_env_in(const Params& p, int b) {
+  EnvIn in;
+  in.sc = (p.done_mode == 0) ? p.step_count[b] : 0;
+  in.done = (p.done_mode == 0) ? ((p.episode_length >= 0) && (in.sc >= p.episode_length))
+                               : (p.do_reset[b] != 0);
+  in.bt_lo = 0u;
+  in.bt_hi = 0u;
+  in.look_yaw = 0.0f;
+  in.look_pitch = 0.0f;
   if (p.action_kind == 1) {
+    // Discrete ids (docs/environment.md:3-17): 1..6 press one button.
     const int64_t a = reinterpret_cast<const int64_t*>(p.actions)[b];
-    if (a >= 1 && a <= 6) bt[a - 1] = 1;
-    else if (a == 7) look_yaw = p.look_delta;
-    else if (a == 8) look_yaw = -p.look_delta;
-    else if (a == 9) look_pitch = p.look_delta;
-    else if (a == 10) look_pitch = -p.look_delta;
+    if (a >= 1 && a <= 4) in.bt_lo = 1u << (8 * (int)(a - 1));
+    else if (a >= 5 && a <= 6) in.bt_hi = 1u << (8 * (int)(a - 5));
+    else if (a == 7) in.look_yaw = p.look_delta;
+    else if (a == 8) in.look_yaw = -p.look_delta;
+    else if (a == 9) in.look_pitch = p.look_delta;
+    else if (a == 10) in.look_pitch = -p.look_delta;
   } else {
-    const int8_t* src = reinterpret_cast<const int8_t*>(p.actions) + (int64_t)b * 8;
+    const uint8_t* src = reinterpret_cast<const uint8_t*>(p.actions) + (int64_t)b * 8;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) bt[k] = src[k];
-    look_yaw = p.look[b * 2 + 0];
-    look_pitch = p.look[b * 2 + 1];
+    for (int k = 0; k < 4; ++k) {
+      in.bt_lo |= (uint32_t)src[k] << (8 * k);
+      in.bt_hi |= (uint32_t)src[4 + k] << (8 * k);
+    }
+    in.look_yaw = p.look[b * 2 + 0];
+    in.look_pitch = p.look[b * 2 + 1];
   }
+  in.yaw = p.yaws[b];
+  in.pitch = p.pitches[b];
+  in.px = p.positions[b * 3 + 0];
+  in.py = p.positions[b * 3 + 1];
+  in.pz = p.positions[b * 3 + 2];
+  in.vy = p.velocities[b * 3 + 1];
+  in.og = p.on_ground[b] != 0;
+  return in;
+}
+
+__device__ __forceinline__ Frame env_update(const Params& p, int b, int8_t* world,
+                                            int8_t* gworld, bool any_done, const EnvIn& in) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int sc = in.sc;
+  const bool done = in.done;
+
+  // Decode (src/kernels.cu:1477-1503).
+  int8_t bt[8];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    bt[k] = (int8_t)(in.bt_lo >> (8 * k));
+    bt[4 + k] = (int8_t)(in.bt_hi >> (8 * k));
+  }
+  const float look_yaw = in.look_yaw, look_pitch = in.look_pitch;
   const float fwd = (float)bt[0] - (float)bt[1];
   const float str = (float)bt[3] - (float)bt[2];
   const bool jump = bt[4] != 0, brk = bt[5] != 0, place = bt[6] != 0;
   const float speed_mult = (bt[0] && bt[7]) ? p.sprint_mult : 1.0f;
 
   // Physics.
-  float yaw = p.yaws[b] + look_yaw;
-  const float new_pitch = p.pitches[b] + look_pitch;
+  float yaw = in.yaw + look_yaw;
+  const float new_pitch = in.pitch + look_pitch;
   float pitch = fminf(fmaxf(new_pitch, -1.56f), 1.56f);
   float sin_yaw, cos_yaw;
   mc_sincosf(yaw, &sin_yaw, &cos_yaw);
   const float speed = p.walk_speed * speed_mult;
   const float move_x = sin_yaw * fwd * speed * p.dt + cos_yaw * str * p.walk_speed * p.dt;
   const float move_z = cos_yaw * fwd * speed * p.dt - sin_yaw * str * p.walk_speed * p.dt;
-  float vel_y = p.velocities[b * 3 + 1];
-  bool og = p.on_ground[b] != 0;
+  float vel_y = in.vy;
+  bool og = in.og;
   if (jump && og) vel_y = p.jump_vel;
   vel_y += p.gravity * p.dt;
-  float nx = p.positions[b * 3 + 0] + move_x;
-  float ny = p.positions[b * 3 + 1] + vel_y * p.dt;
-  float nz = p.positions[b * 3 + 2] + move_z;
+  float nx = in.px + move_x;
+  float ny = in.py + vel_y * p.dt;
+  float nz = in.pz + move_z;
   const float ground_y = (float)(p.Y / 4);
   if (ny <= ground_y) {
     ny = ground_y;
@@ -702,6 +784,8 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     if (b == 0 && et == 0) p.ring[(p.slot + 2) % 3] = 0u;
   }
   const bool regen = (KIND == kItemResetRender) && (p.ring[3] != 0u);
+  EnvIn in;
+  if (KIND == kItemStep && active && et < kWave) in = load_env_in(p, b);
   if (active) {
     if (regen) store_fresh_world(p, gworld, world, nvox, et, ent);
     else stage_world(world, gworld, nvox, et, ent);
@@ -709,7 +793,7 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   __syncthreads();
   if (active && et < kWave) {
     Frame f;
-    if (KIND == kItemStep) f = env_update(p, b, world, gworld, any_done);
+    if (KIND == kItemStep) f = env_update(p, b, world, gworld, any_done, in);
     else if (KIND == kItemResetRender) f = reset_update(p, b, world);
     else f = camera_update(p, b, world, KIND == kItemRenderState);
     if (et == 0) frames[e] = f;
