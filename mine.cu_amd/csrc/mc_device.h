@@ -321,41 +321,34 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, 
     bool inb = (unsigned)vx < (unsigned)X && (unsigned)vy < (unsigned)Y && (unsigned)vz < (unsigned)Z;
     int blk = inb ? (int)world[idx] : -1;
     int face = -1;
-    // Each iteration is the reference's (exit checks, lookup, advance) with
-    // the advance computed branch-free and the next voxel's LDS read issued
-    // before the current voxel is tested, so its latency overlaps the test.
-    for (int step = 0;; ++step) {
-      if (step >= max_steps || t >= vd) break;
+    // The reference's iteration is (limit check, lookup, advance).  Here the
+    // limit check of iteration k+1 closes iteration k, the lookup decides a
+    // single loop exit, the advance is branch-free, and the next voxel's LDS
+    // read is issued before the loop comes back to test it.
+    bool go = !(0 >= max_steps || t >= vd);
+    int step = 0;
+    while (go) {
+      if (inb ? (blk >= 0) : (t > t_exit)) break;
       const bool cx = tm_x < tm_y && tm_x < tm_z;
       const bool cy = !cx && (tm_y < tm_z);
       const bool cz = !cx && !cy;
-      const float nt = cx ? tm_x : (cy ? tm_y : tm_z);
-      const float ntm_x = cx ? tm_x + td_x : tm_x;
-      const float ntm_y = cy ? tm_y + td_y : tm_y;
-      const float ntm_z = cz ? tm_z + td_z : tm_z;
-      const int nvx = cx ? vx + step_x : vx;
-      const int nvy = cy ? vy + step_y : vy;
-      const int nvz = cz ? vz + step_z : vz;
-      const int nidx = idx + (cx ? dix : (cy ? diy : diz));
-      const bool ninb = (unsigned)nvx < (unsigned)X && (unsigned)nvy < (unsigned)Y &&
-                        (unsigned)nvz < (unsigned)Z;
-      const int nblk = ninb ? (int)world[nidx] : -1;
-      if (inb) {
-        if (blk >= 0) {
-          hit_block = blk;
-          hit_face = face;
-          break;
-        }
-      } else if (t > t_exit) {
-        break;
-      }
-      t = nt;
-      tm_x = ntm_x; tm_y = ntm_y; tm_z = ntm_z;
-      vx = nvx; vy = nvy; vz = nvz;
-      idx = nidx;
-      inb = ninb;
-      blk = nblk;
+      t = cx ? tm_x : (cy ? tm_y : tm_z);
+      tm_x = cx ? tm_x + td_x : tm_x;
+      tm_y = cy ? tm_y + td_y : tm_y;
+      tm_z = cz ? tm_z + td_z : tm_z;
+      vx = cx ? vx + step_x : vx;
+      vy = cy ? vy + step_y : vy;
+      vz = cz ? vz + step_z : vz;
+      idx += cx ? dix : (cy ? diy : diz);
       face = cx ? 0 : (cy ? 1 : 2);
+      inb = (unsigned)vx < (unsigned)X && (unsigned)vy < (unsigned)Y && (unsigned)vz < (unsigned)Z;
+      blk = inb ? (int)world[idx] : -1;
+      ++step;
+      go = !(step >= max_steps || t >= vd);
+    }
+    if (go && inb && blk >= 0) {
+      hit_block = blk;
+      hit_face = face;
     }
   }
   if (hit_block >= 0 && hit_block < 13) {
