@@ -13,6 +13,7 @@
 // results are bit-identical to the scalar host restatement.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <limits.h>
 #include <stdint.h>
 
 #include "mc_math.h"
@@ -30,8 +31,8 @@ typedef MC_GLOBAL v4u guint4;
 typedef MC_GLOBAL uint32_t guint32;
 typedef MC_GLOBAL v4f gfloat4;
 // Bytes of world staged in LDS per workgroup: 64 KiB less room for the
-// kernels' static LDS (palette, frames, work-item bookkeeping).
-constexpr int kMaxLdsWorld = 65536 - 1024;
+// kernels' static LDS (palette, frames, u/v tables, work-item bookkeeping).
+constexpr int kMaxLdsWorld = 65536 - 3072;
 
 // Block ids (src/kernels.cu:11-24).
 constexpr int8_t kAir = -1, kGrass = 0, kDirt = 1, kOakLog = 3, kLeaves = 4, kBrick = 8,
@@ -118,6 +119,11 @@ struct Frame {
   float rx, rz;                // right (right_y == 0)
   float ux, uy, uz;            // up
   int fill_block;              // >= 0: eye inside this solid block, every ray hits it at t=0
+  // Extent of the solid voxels at heights >= floor(eye y): x in [hx0, hx1],
+  // z in [hz0, hz1], y <= hy1 (empty: hx0 > hx1, hy1 = INT_MIN).  Valid when
+  // `lift` is set; used to end upward rays early (see trace()).
+  int hx0, hx1, hz0, hz1, hy1;
+  int lift;
 };
 
 // ---------------------------------------------------------------------------
@@ -267,19 +273,86 @@ __device__ __forceinline__ Frame make_frame(const Params& p, float x, float y, f
   f.uy = cos_pitch;
   f.uz = -cos_yaw * sin_pitch;
   f.fill_block = -1;
+  f.hx0 = f.hz0 = INT_MAX;
+  f.hx1 = f.hz1 = f.hy1 = INT_MIN;
+  f.lift = 0;
   if (p.max_steps >= 1 && p.view_distance > 0.0f && f.ex > 0.0f && f.ex < (float)p.X &&
       f.ey > 0.0f && f.ey < (float)p.Y && f.ez > 0.0f && f.ez < (float)p.Z) {
     const int vx = (int)floorf(f.ex), vy = (int)floorf(f.ey), vz = (int)floorf(f.ez);
     const int8_t blk = world[vy * p.X * p.Z + vx * p.Z + vz];
     if (blk >= 0 && blk < 13) f.fill_block = blk;
   }
+  // The solid-extent scan reads whole 16-byte rows of z.
+  f.lift = (f.fill_block < 0) && ((p.Z & 15) == 0);
   return f;
+}
+
+// Extent of the solid voxels at heights >= y0 (16-byte LDS chunks; Z % 16 ==
+// 0), reduced over the env's threads into f (LDS) with LDS atomics.
+__device__ __forceinline__ void solid_extent(const Params& p, const int8_t* world, Frame* f,
+                                             int y0, int tid, int nt) {
+  const int X = p.X, Y = p.Y, zc = p.Z >> 4;
+  int xmin = INT_MAX, xmax = INT_MIN, zmin = INT_MAX, zmax = INT_MIN, ymax = INT_MIN;
+  if (y0 < 0) y0 = 0;
+  if (y0 < Y) {
+    const v4u* w4 = reinterpret_cast<const v4u*>(world);
+    const int n16 = Y * X * zc;
+    // chunk c = (y * X + x) * zc + zq, walked with per-lane stride nt
+    const int c_begin = y0 * X * zc + tid;
+    int zq = c_begin % zc, yx = c_begin / zc;
+    int x = yx % X, y = yx / X;
+    const int dzq = nt % zc, dyx = nt / zc, dx = dyx % X, dy = dyx / X;
+    for (int c = c_begin; c < n16; c += nt) {
+      const v4u w = w4[c];
+      const uint32_t m0 = ~w.x & 0x80808080u, m1 = ~w.y & 0x80808080u;
+      const uint32_t m2 = ~w.z & 0x80808080u, m3 = ~w.w & 0x80808080u;
+      if ((m0 | m1 | m2 | m3) != 0u) {
+        const uint64_t lo = (uint64_t)m0 | ((uint64_t)m1 << 32);
+        const uint64_t hi = (uint64_t)m2 | ((uint64_t)m3 << 32);
+        const int fb = lo ? ((__ffsll((unsigned long long)lo) - 1) >> 3)
+                          : 8 + ((__ffsll((unsigned long long)hi) - 1) >> 3);
+        const int lb = hi ? 15 - (__clzll((long long)hi) >> 3) : 7 - (__clzll((long long)lo) >> 3);
+        xmin = min(xmin, x);
+        xmax = max(xmax, x);
+        zmin = min(zmin, zq * 16 + fb);
+        zmax = max(zmax, zq * 16 + lb);
+        ymax = max(ymax, y);
+      }
+      zq += dzq;
+      x += dx;
+      y += dy;
+      if (zq >= zc) { zq -= zc; ++x; }
+      if (x >= X) { x -= X; ++y; }
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    xmin = min(xmin, __shfl_xor(xmin, o));
+    xmax = max(xmax, __shfl_xor(xmax, o));
+    zmin = min(zmin, __shfl_xor(zmin, o));
+    zmax = max(zmax, __shfl_xor(zmax, o));
+    ymax = max(ymax, __shfl_xor(ymax, o));
+  }
+  if ((tid & (kWave - 1)) == 0) {
+    atomicMin(&f->hx0, xmin);
+    atomicMax(&f->hx1, xmax);
+    atomicMin(&f->hz0, zmin);
+    atomicMax(&f->hz1, zmax);
+    atomicMax(&f->hy1, ymax);
+  }
 }
 
 // ---------------------------------------------------------------------------
 // One ray: AABB cull + DDA + face shade + fog (src/kernels.cu:51-79, 354-469).
 // Writes the shaded colour in [0,1] to rgb.
+//
+// LIFT: for a ray with ray_y >= 0 the DDA's voxel y never decreases, so once
+// the current voxel lies beyond every solid voxel at heights >= floor(eye y)
+// (above hy1, or past [hx0, hx1] / [hz0, hz1] in the direction of travel),
+// every voxel it can still visit is air or outside the world: the reference's
+// result is sky, and the loop stops there.
 // ---------------------------------------------------------------------------
+template <bool LIFT>
 __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, float v,
                                       const int8_t* __restrict__ world,
                                       const float* __restrict__ pal, float rgb[3]) {
@@ -319,16 +392,27 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, 
     const int dix = step_x * Z, diy = step_y * XZ, diz = step_z;
     int idx = vy * XZ + vx * Z + vz;
     bool inb = (unsigned)vx < (unsigned)X && (unsigned)vy < (unsigned)Y && (unsigned)vz < (unsigned)Z;
-    int blk = inb ? (int)world[idx] : -1;
+    // Out-of-bounds positions read voxel 0 (always valid) and discard it, so
+    // the read needs no branch.
+    int blk = inb ? (int)world[inb ? idx : 0] : -1;
     int face = -1;
     // The reference's iteration is (limit check, lookup, advance).  Here the
     // limit check of iteration k+1 closes iteration k, the lookup decides a
     // single loop exit, the advance is branch-free, and the next voxel's LDS
     // read is issued before the loop comes back to test it.
+    int by = INT_MAX, bxh = INT_MAX, bxl = INT_MIN, bzh = INT_MAX, bzl = INT_MIN;
+    if (LIFT && step_y > 0) {
+      by = f.hy1;
+      if (step_x > 0) bxh = f.hx1; else bxl = f.hx0;
+      if (step_z > 0) bzh = f.hz1; else bzl = f.hz0;
+    }
     bool go = !(0 >= max_steps || t >= vd);
     int step = 0;
     while (go) {
-      if (inb ? (blk >= 0) : (t > t_exit)) break;
+      // blk >= 0 implies in bounds (out-of-bounds blk is -1); a dead voxel is
+      // never solid, so a LIFT exit is a no-hit exit.
+      const bool dead = LIFT && ((vy > by) | (vx > bxh) | (vx < bxl) | (vz > bzh) | (vz < bzl));
+      if ((blk >= 0) | ((!inb) & (t > t_exit)) | dead) break;
       const bool cx = tm_x < tm_y && tm_x < tm_z;
       const bool cy = !cx && (tm_y < tm_z);
       const bool cz = !cx && !cy;
@@ -342,7 +426,8 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, 
       idx += cx ? dix : (cy ? diy : diz);
       face = cx ? 0 : (cy ? 1 : 2);
       inb = (unsigned)vx < (unsigned)X && (unsigned)vy < (unsigned)Y && (unsigned)vz < (unsigned)Z;
-      blk = inb ? (int)world[idx] : -1;
+      const int raw = (int)world[inb ? idx : 0];
+      blk = inb ? raw : -1;
       ++step;
       go = !(step >= max_steps || t >= vd);
     }
@@ -373,11 +458,32 @@ __device__ __forceinline__ uint32_t to_u8(float c) { return (uint32_t)(c * 255.0
 // Render every pixel of env b into obs (layout [B,H,W,3]).
 // OBS: 0 = uint8, 1 = float32.
 // ---------------------------------------------------------------------------
+// Image-plane coordinates of column px / row py (src/kernels.cu:351-352).
+__device__ __forceinline__ float u_of(const Params& p, int px) {
+  return (2.0f * (px + 0.5f) / p.W - 1.0f) * p.half_fov * p.aspect;
+}
+__device__ __forceinline__ float v_of(const Params& p, int py) {
+  return (1.0f - 2.0f * (py + 0.5f) / p.H) * p.half_fov;
+}
+
+// Per-workgroup tables of u (per column) and v (per row), filled before the
+// first barrier of an item when the image fits them.
+constexpr int kMaxUV = 256;
+__device__ __forceinline__ bool uv_tabled(const Params& p) { return p.W <= kMaxUV && p.H <= kMaxUV; }
+__device__ __forceinline__ void fill_uv(const Params& p, float* su, float* sv, int tid, int nt) {
+  if (!uv_tabled(p)) return;
+  for (int i = tid; i < p.W; i += nt) su[i] = u_of(p, i);
+  for (int i = tid; i < p.H; i += nt) sv[i] = v_of(p, i);
+}
+
 template <int OBS>
 __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t* world,
-                                           const float* pal, const Frame& f, int tid, int nt) {
+                                           const float* pal, const Frame& f, int tid, int nt,
+                                           const float* su, const float* sv) {
   const int W = p.W, H = p.H, npix = W * H;
-  const float hf = p.half_fov, aspect = p.aspect;
+  const bool tab = uv_tabled(p);
+  auto ucol = [&](int px) { return tab ? su[px] : u_of(p, px); };
+  auto vrow = [&](int py) { return tab ? sv[py] : v_of(p, py); };
   if (OBS == 0) {
     uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
     if (f.fill_block >= 0 && ((npix * 3) & 15) == 0) {
@@ -389,8 +495,9 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       const uint32_t w2 = bl | (r << 8) | (g << 16) | (bl << 24);
       // Chunk c starts at byte 16c, i.e. at channel phase (16c mod 3) = c mod 3.
       guint4* o4 = (guint4*)out;
-      for (int c = tid; c < (npix * 3) >> 4; c += nt) {
-        const int ph = c % 3;
+      const int d3 = nt % 3;
+      int ph = tid % 3;
+      for (int c = tid; c < (npix * 3) >> 4; c += nt, ph = (ph + d3 >= 3) ? ph + d3 - 3 : ph + d3) {
         const uint32_t a = (ph == 0) ? w0 : (ph == 1) ? w1 : w2;
         const uint32_t bq = (ph == 0) ? w1 : (ph == 1) ? w2 : w0;
         const uint32_t cq = (ph == 0) ? w2 : (ph == 1) ? w0 : w1;
@@ -399,14 +506,22 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       return;
     }
     if ((W & 3) == 0) {
+      // Quad q covers pixels 4q..4q+3 of one row; (row, quad column) advance
+      // incrementally instead of dividing per quad.
+      const int qpr = W >> 2, drow = nt / qpr, dcol = nt - drow * qpr;
+      int py = tid / qpr, qx = tid - py * qpr;
       for (int q = tid; q < (npix >> 2); q += nt) {
-        const int p0 = q << 2, py = p0 / W, px0 = p0 - py * W;
-        const float v = (1.0f - 2.0f * (py + 0.5f) / H) * hf;
+        const int px0 = qx << 2, p0 = py * W + px0;
+        const float v = vrow(py);
+        qx += dcol;
+        py += drow;
+        if (qx >= qpr) { qx -= qpr; ++py; }
         // Packed 0x00BBGGRR per pixel.
         auto shade = [&](int j) -> uint32_t {
-          const float u = (2.0f * (px0 + j + 0.5f) / W - 1.0f) * hf * aspect;
+          const float u = ucol(px0 + j);
           float rgb[3];
-          trace(p, f, u, v, world, pal, rgb);
+          if (f.lift) trace<true>(p, f, u, v, world, pal, rgb);
+          else trace<false>(p, f, u, v, world, pal, rgb);
           return to_u8(rgb[0]) | (to_u8(rgb[1]) << 8) | (to_u8(rgb[2]) << 16);
         };
         const uint32_t c0 = shade(0), c1 = shade(1), c2 = shade(2), c3 = shade(3);
@@ -418,13 +533,14 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
     } else {
       for (int pi = tid; pi < npix; pi += nt) {
         const int py = pi / W, px = pi - py * W;
-        const float v = (1.0f - 2.0f * (py + 0.5f) / H) * hf;
-        const float u = (2.0f * (px + 0.5f) / W - 1.0f) * hf * aspect;
+        const float v = vrow(py);
+        const float u = ucol(px);
         float rgb[3];
         if (f.fill_block >= 0) {
           for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
         } else {
-          trace(p, f, u, v, world, pal, rgb);
+          if (f.lift) trace<true>(p, f, u, v, world, pal, rgb);
+          else trace<false>(p, f, u, v, world, pal, rgb);
         }
         for (int c = 0; c < 3; ++c) out[(int64_t)pi * 3 + c] = (uint8_t)to_u8(rgb[c]);
       }
@@ -432,18 +548,24 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
   } else {
     float* out = reinterpret_cast<float*>(p.obs) + (int64_t)b * npix * 3;
     if ((W & 3) == 0) {
+      const int qpr = W >> 2, drow = nt / qpr, dcol = nt - drow * qpr;
+      int py = tid / qpr, qx = tid - py * qpr;
       for (int q = tid; q < (npix >> 2); q += nt) {
-        const int p0 = q << 2, py = p0 / W, px0 = p0 - py * W;
-        const float v = (1.0f - 2.0f * (py + 0.5f) / H) * hf;
+        const int px0 = qx << 2, p0 = py * W + px0;
+        const float v = vrow(py);
+        qx += dcol;
+        py += drow;
+        if (qx >= qpr) { qx -= qpr; ++py; }
         float vals[12];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const float u = (2.0f * (px0 + j + 0.5f) / W - 1.0f) * hf * aspect;
+          const float u = ucol(px0 + j);
           if (f.fill_block >= 0) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) vals[j * 3 + c] = pal[f.fill_block * 3 + c];
           } else {
-            trace(p, f, u, v, world, pal, &vals[j * 3]);
+            if (f.lift) trace<true>(p, f, u, v, world, pal, &vals[j * 3]);
+            else trace<false>(p, f, u, v, world, pal, &vals[j * 3]);
           }
         }
         gfloat4* o = (gfloat4*)(out + (int64_t)p0 * 3);
@@ -454,13 +576,14 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
     } else {
       for (int pi = tid; pi < npix; pi += nt) {
         const int py = pi / W, px = pi - py * W;
-        const float v = (1.0f - 2.0f * (py + 0.5f) / H) * hf;
-        const float u = (2.0f * (px + 0.5f) / W - 1.0f) * hf * aspect;
+        const float v = vrow(py);
+        const float u = ucol(px);
         float rgb[3];
         if (f.fill_block >= 0) {
           for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
         } else {
-          trace(p, f, u, v, world, pal, rgb);
+          if (f.lift) trace<true>(p, f, u, v, world, pal, rgb);
+          else trace<false>(p, f, u, v, world, pal, rgb);
         }
         for (int c = 0; c < 3; ++c) out[(int64_t)pi * 3 + c] = rgb[c];
       }
@@ -760,7 +883,8 @@ __host__ __device__ __forceinline__ int world_stride_bytes(int X, int Y, int Z) 
 
 template <int NT, int KIND>
 __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, int wpe,
-                                         int8_t* lds, const float* pal, Frame* frames) {
+                                         int8_t* lds, const float* pal, Frame* frames,
+                                         float* su, float* sv) {
   const int tid = threadIdx.x, wave = tid / kWave;
   const int epw = (NT / kWave) / wpe;
   const int e = wave / wpe;                                 // env slot within the item
@@ -779,6 +903,7 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   const bool regen = (KIND == kItemResetRender) && (p.ring[3] != 0u);
   EnvIn in;
   if (KIND == kItemStep && active && et < kWave) in = load_env_in(p, b);
+  fill_uv(p, su, sv, tid, NT);
   if (active) {
     if (regen) store_fresh_world(p, gworld, world, nvox, et, ent);
     else stage_world(world, gworld, nvox, et, ent);
@@ -792,10 +917,13 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     if (et == 0) frames[e] = f;
   }
   __syncthreads();
+  if (active && frames[e].lift)
+    solid_extent(p, world, &frames[e], (int)floorf(frames[e].ey), et, ent);
+  __syncthreads();
   if (active) {
     const Frame f = frames[e];
-    if (obs_mode == 0) render_env<0>(p, b, world, pal, f, et, ent);
-    else render_env<1>(p, b, world, pal, f, et, ent);
+    if (obs_mode == 0) render_env<0>(p, b, world, pal, f, et, ent, su, sv);
+    else render_env<1>(p, b, world, pal, f, et, ent, su, sv);
     if (KIND == kItemStep && any_done) store_fresh_world(p, gworld, nullptr, nvox, et, ent);
   }
 }
@@ -810,9 +938,10 @@ __global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
   __shared__ Frame s_frames[NT / kWave];
+  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
   load_palette(s_pal, threadIdx.x);
   const int epw = (NT / kWave) / wpe;
-  run_item<NT, KIND>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_frames);
+  run_item<NT, KIND>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_frames, s_u, s_v);
 }
 
 // Several env groups (uneven batch sizes, world sizes, resolutions) in one
@@ -850,6 +979,7 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
   __shared__ Frame s_frames[NT / kWave];
   __shared__ int s_begin[kMaxGroups];
   __shared__ int s_item;
+  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
   load_palette(s_pal, threadIdx.x);
   if (threadIdx.x < a.n_groups) s_begin[threadIdx.x] = a.entries[threadIdx.x].item_begin;
   int item = blockIdx.x;
@@ -873,7 +1003,7 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
     const int wpe = me.wpe;
     const int epw = (NT / kWave) / wpe;
     run_item<NT, kItemStep>(p, me.obs_mode, (item - s_begin[g]) * epw, wpe, s_lds, s_pal,
-                            s_frames);
+                            s_frames, s_u, s_v);
     if (!a.persistent) break;
   }
   if (a.persistent && threadIdx.x == 0) {
