@@ -221,12 +221,15 @@ int mc_env_step(mc_env* env, const void* actions, int action_kind, const float* 
 int mc_env_reset(mc_env* env, const uint8_t* mask, void* stream);
 int mc_env_render(mc_env* env, void* stream);
 
-/* Launch shape of a group's step: `threads` per workgroup (64, 256 or 512)
- * and `waves_per_env` (1, 2, 4 or 8) 64-lane waves per env, so a workgroup
- * holds threads / (64 waves_per_env) envs.  (0, 0) restores the default; the
- * autotuner below sets it.  get returns the shape after the LDS fit. */
-int mc_env_set_shape(mc_env* env, int threads, int waves_per_env);
-int mc_env_get_shape(const mc_env* env, int* threads, int* waves_per_env);
+/* Launch shape of a group's step: `threads` per workgroup (64, 256 or 512),
+ * `waves_per_env` (1, 2, 4 or 8) 64-lane waves per env, so a workgroup holds
+ * threads / (64 waves_per_env) envs, and `lift` (0/1; -1 keeps the current
+ * value): build in the exact early exit of upward rays past every solid voxel
+ * above eye height.  (0, 0, lift) restores the default size; the autotuner
+ * below sets all three.  Results do not depend on the shape.  get returns the
+ * shape after the LDS fit (lift may be NULL). */
+int mc_env_set_shape(mc_env* env, int threads, int waves_per_env, int lift);
+int mc_env_get_shape(const mc_env* env, int* threads, int* waves_per_env, int* lift);
 
 /* ---- batched groups (no reference counterpart: the reference runs one
  * fixed-shape MineEnv per process-global graph) --------------------------- */
@@ -254,7 +257,7 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
  * Results are cached in the batch object.  Synchronises `stream`. */
 int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* stream);
 int mc_batch_tuned_shape(const mc_batch* b, const mc_config* cfg, int* threads,
-                         int* waves_per_env);
+                         int* waves_per_env, int* lift);
 
 /* ---- serving (no reference counterpart) ----------------------------------
  * A continuous-batching server: a loop thread owns n_streams HIP streams and

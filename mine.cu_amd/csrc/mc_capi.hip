@@ -121,19 +121,25 @@ mc::Params make_params(const mc_config* c, const mc_state* s) {
 
 // Launch shape of the per-env work: NT threads per workgroup, wpe waves per
 // env, so one workgroup holds epw = NT / (64 wpe) envs.
+// lift: build the upward-ray early exit into the kernel (pays off when rays
+// climb through tall air columns; costs a scan and registers otherwise).
 struct Shape {
   int nt;
   int wpe;
+  int lift;
 };
 
 bool valid_shape(Shape s) {
   return (s.nt == 64 || s.nt == 256 || s.nt == 512) &&
-         (s.wpe == 1 || s.wpe == 2 || s.wpe == 4 || s.wpe == 8) && s.wpe * 64 <= s.nt;
+         (s.wpe == 1 || s.wpe == 2 || s.wpe == 4 || s.wpe == 8) && s.wpe * 64 <= s.nt &&
+         (s.lift == 0 || s.lift == 1);
 }
 
-// Default before any tuning: one wave per env up to 1024 pixels, four above.
-Shape default_shape(int W, int H) {
-  return ((int64_t)W * H <= 1024) ? Shape{64, 1} : Shape{256, 4};
+// Default before any tuning: one wave per env up to 1024 pixels, four above;
+// the upward-ray exit for worlds at least 24 voxels tall.
+Shape default_shape(int W, int H, int Y) {
+  return Shape{((int64_t)W * H <= 1024) ? 64 : 256, ((int64_t)W * H <= 1024) ? 1 : 4,
+               Y >= 24 ? 1 : 0};
 }
 
 int epw_of(Shape s) { return (s.nt / mc::kWave) / s.wpe; }
@@ -145,23 +151,25 @@ Shape fit_lds(Shape s, int X, int Y, int Z) {
   return s;
 }
 
-template <int NT, int KIND>
+template <int NT, int KIND, bool LIFT>
 int launch_kind(const mc::Params& p, int obs_mode, int wpe, hipStream_t st) {
   const int epw = (NT / mc::kWave) / wpe;
   const int grid = (p.B + epw - 1) / epw;
   const size_t lds = (size_t)epw * mc::world_stride_bytes(p.X, p.Y, p.Z);
-  hipLaunchKernelGGL((mc::k_env<NT, KIND>), dim3(grid), dim3(NT), lds, st, p, obs_mode, wpe);
+  hipLaunchKernelGGL((mc::k_env<NT, KIND, LIFT>), dim3(grid), dim3(NT), lds, st, p, obs_mode,
+                     wpe);
   return check_launch("env kernel launch");
 }
 
-template <int NT>
+template <int NT, bool LIFT>
 int launch_nt(int kind, const mc::Params& p, int obs_mode, int wpe, hipStream_t st) {
   switch (kind) {
-    case mc::kItemStep: return launch_kind<NT, mc::kItemStep>(p, obs_mode, wpe, st);
-    case mc::kItemResetRender: return launch_kind<NT, mc::kItemResetRender>(p, obs_mode, wpe, st);
+    case mc::kItemStep: return launch_kind<NT, mc::kItemStep, LIFT>(p, obs_mode, wpe, st);
+    case mc::kItemResetRender:
+      return launch_kind<NT, mc::kItemResetRender, LIFT>(p, obs_mode, wpe, st);
     case mc::kItemRenderCameras:
-      return launch_kind<NT, mc::kItemRenderCameras>(p, obs_mode, wpe, st);
-    default: return launch_kind<NT, mc::kItemRenderState>(p, obs_mode, wpe, st);
+      return launch_kind<NT, mc::kItemRenderCameras, LIFT>(p, obs_mode, wpe, st);
+    default: return launch_kind<NT, mc::kItemRenderState, LIFT>(p, obs_mode, wpe, st);
   }
 }
 
@@ -169,10 +177,13 @@ int launch_env(int kind, const mc::Params& p, int obs_mode, Shape s, hipStream_t
   s = fit_lds(s, p.X, p.Y, p.Z);
   if (epw_of(s) * mc::world_stride_bytes(p.X, p.Y, p.Z) > mc::kMaxLdsWorld)
     return fail(MC_E_UNSUPPORTED, "world does not fit the LDS budget");
-  switch (s.nt) {
-    case 64: return launch_nt<64>(kind, p, obs_mode, s.wpe, st);
-    case 256: return launch_nt<256>(kind, p, obs_mode, s.wpe, st);
-    default: return launch_nt<512>(kind, p, obs_mode, s.wpe, st);
+  switch (s.nt * 2 + s.lift) {
+    case 128: return launch_nt<64, false>(kind, p, obs_mode, s.wpe, st);
+    case 129: return launch_nt<64, true>(kind, p, obs_mode, s.wpe, st);
+    case 512: return launch_nt<256, false>(kind, p, obs_mode, s.wpe, st);
+    case 513: return launch_nt<256, true>(kind, p, obs_mode, s.wpe, st);
+    case 1024: return launch_nt<512, false>(kind, p, obs_mode, s.wpe, st);
+    default: return launch_nt<512, true>(kind, p, obs_mode, s.wpe, st);
   }
 }
 
@@ -270,7 +281,7 @@ int mc_render(const int8_t* voxels, const float* cameras, void* obs, int obs_mod
   p.voxels = const_cast<int8_t*>(voxels);
   p.cameras = const_cast<float*>(cameras);
   p.obs = obs;
-  return launch_env(mc::kItemRenderCameras, p, obs_mode, default_shape(W, H), S(stream));
+  return launch_env(mc::kItemRenderCameras, p, obs_mode, default_shape(W, H, Y), S(stream));
 }
 
 int mc_render_state(const mc_config* cfg, const mc_state* st, void* stream) {
@@ -280,7 +291,7 @@ int mc_render_state(const mc_config* cfg, const mc_state* st, void* stream) {
   if (r) return r;
   mc::Params p = make_params(cfg, st);
   return launch_env(mc::kItemRenderState, p, cfg->obs_mode,
-                    default_shape(cfg->width, cfg->height), S(stream));
+                    default_shape(cfg->width, cfg->height, cfg->world_y), S(stream));
 }
 
 int mc_step(const mc_config* cfg, const mc_state* st, const void* actions, int action_kind,
@@ -304,7 +315,7 @@ int mc_step(const mc_config* cfg, const mc_state* st, const void* actions, int a
   hipLaunchKernelGGL(mc::k_ring_setup, dim3(1), dim3(1024), 0, S(stream), p, 0);
   r = check_launch("episode pre-pass");
   if (r) return r;
-  return launch_env(mc::kItemStep, p, cfg->obs_mode, default_shape(cfg->width, cfg->height),
+  return launch_env(mc::kItemStep, p, cfg->obs_mode, default_shape(cfg->width, cfg->height, cfg->world_y),
                     S(stream));
 }
 
@@ -330,7 +341,7 @@ int mc_env_reset_state(const mc_config* cfg, const mc_state* st, const uint8_t* 
   if (r) return r;
   if (!workspace) return fail(MC_E_INVALID, "workspace is NULL");
   return reset_with_ring(cfg, st, mask, reinterpret_cast<uint32_t*>(workspace), 0,
-                         default_shape(cfg->width, cfg->height), stream);
+                         default_shape(cfg->width, cfg->height, cfg->world_y), stream);
 }
 
 int mc_env_create(const mc_config* cfg, const mc_state* st, mc_env** out) {
@@ -348,7 +359,7 @@ int mc_env_create(const mc_config* cfg, const mc_state* st, mc_env** out) {
   e->ring = nullptr;
   e->desc = nullptr;
   e->slot = 0;
-  e->shape = default_shape(cfg->width, cfg->height);
+  e->shape = default_shape(cfg->width, cfg->height, cfg->world_y);
   hipError_t he = hipMalloc(reinterpret_cast<void**>(&e->ring), MC_WORKSPACE_BYTES);
   if (he == hipSuccess) he = hipMemset(e->ring, 0, MC_WORKSPACE_BYTES);
   if (he == hipSuccess) {
@@ -367,25 +378,28 @@ int mc_env_create(const mc_config* cfg, const mc_state* st, mc_env** out) {
   return MC_OK;
 }
 
-int mc_env_set_shape(mc_env* env, int threads, int waves_per_env) {
+int mc_env_set_shape(mc_env* env, int threads, int waves_per_env, int lift) {
   if (!env) return fail(MC_E_INVALID, "env is NULL");
+  const Shape d = default_shape(env->cfg.width, env->cfg.height, env->cfg.world_y);
   if (threads == 0 && waves_per_env == 0) {
-    env->shape = default_shape(env->cfg.width, env->cfg.height);
-    return MC_OK;
+    env->shape = Shape{d.nt, d.wpe, lift < 0 ? d.lift : lift};
+  } else {
+    env->shape = Shape{threads, waves_per_env, lift < 0 ? env->shape.lift : lift};
   }
-  const Shape s{threads, waves_per_env};
-  if (!valid_shape(s))
-    return fail(MC_E_INVALID, "shape (%d threads, %d waves/env) not supported", threads,
-                waves_per_env);
-  env->shape = s;
+  if (!valid_shape(env->shape)) {
+    env->shape = d;
+    return fail(MC_E_INVALID, "shape (%d threads, %d waves/env, lift %d) not supported",
+                threads, waves_per_env, lift);
+  }
   return MC_OK;
 }
 
-int mc_env_get_shape(const mc_env* env, int* threads, int* waves_per_env) {
+int mc_env_get_shape(const mc_env* env, int* threads, int* waves_per_env, int* lift) {
   if (!env || !threads || !waves_per_env) return fail(MC_E_INVALID, "NULL argument");
   const Shape s = fit_lds(env->shape, env->cfg.world_x, env->cfg.world_y, env->cfg.world_z);
   *threads = s.nt;
   *waves_per_env = s.wpe;
+  if (lift) *lift = s.lift;
   return MC_OK;
 }
 
@@ -507,7 +521,7 @@ int launch_step_ref(int obs_mode, int8_t* voxels, float* positions, float* veloc
     p.do_place = do_place;
     p.speed_mult_in = speed_mult;
   }
-  return launch_env(mc::kItemStep, p, obs_mode, default_shape(W, H), S(stream));
+  return launch_env(mc::kItemStep, p, obs_mode, default_shape(W, H, ws), S(stream));
 }
 }  // namespace
 
@@ -577,7 +591,7 @@ BucketKey bucket_of(const mc_config& c) {
 }
 
 struct Occupancy {
-  int nt, lds, blocks;
+  int nt, lift, lds, blocks;
 };
 }  // namespace
 
@@ -597,24 +611,24 @@ struct mc_batch {
 };
 
 namespace {
-template <int NT>
+template <int NT, bool LIFT>
 int occupancy_blocks(mc_batch* b, int lds) {
   for (const auto& o : b->occ)
-    if (o.nt == NT && o.lds == lds) return o.blocks;
+    if (o.nt == NT && o.lift == (int)LIFT && o.lds == lds) return o.blocks;
   int nb = 0;
-  const hipError_t e =
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mc::k_multi<NT>, NT, (size_t)lds);
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &nb, mc::k_multi<NT, LIFT>, NT, (size_t)lds);
   if (e != hipSuccess || nb < 1) nb = 1;
-  b->occ.push_back({NT, lds, nb});
+  b->occ.push_back({NT, (int)LIFT, lds, nb});
   return nb;
 }
 
-template <int NT>
+template <int NT, bool LIFT>
 int launch_multi(mc_batch* b, const mc::MultiArgs& a, int lds, hipStream_t st) {
   int grid = a.total_items;
-  if (a.persistent) grid = std::min(grid, occupancy_blocks<NT>(b, lds) * b->cus);
+  if (a.persistent) grid = std::min(grid, occupancy_blocks<NT, LIFT>(b, lds) * b->cus);
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((mc::k_multi<NT>), dim3(grid), dim3(NT), (size_t)lds, st, a);
+  hipLaunchKernelGGL((mc::k_multi<NT, LIFT>), dim3(grid), dim3(NT), (size_t)lds, st, a);
   return check_launch("batched step launch");
 }
 }  // namespace
@@ -673,6 +687,7 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
     return fail(MC_E_INVALID, "group count %d outside [1, %d]", n, mc::kMaxGroups);
   // Launch width: 512 threads if some group wants 8 waves per env, else 256.
   int nt = 256;
+  bool lift = false;
   for (int i = 0; i < n; ++i) {
     if (!envs[i] || !actions[i]) return fail(MC_E_INVALID, "group %d: NULL env/actions", i);
     const int k = action_kinds[i];
@@ -683,6 +698,7 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
     for (int j = 0; j < i; ++j)
       if (envs[j] == envs[i]) return fail(MC_E_INVALID, "group %d listed twice", i);
     if (envs[i]->shape.wpe == 8) nt = 512;
+    lift = lift || envs[i]->shape.lift;
   }
   struct G {
     int idx, wpe, items;
@@ -692,7 +708,7 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
   int lds = 0;
   for (int i = 0; i < n; ++i) {
     const mc_config& c = envs[i]->cfg;
-    Shape s{nt, std::min(envs[i]->shape.wpe, nt / mc::kWave)};
+    Shape s{nt, std::min(envs[i]->shape.wpe, nt / mc::kWave), lift ? 1 : 0};
     s = fit_lds(s, c.world_x, c.world_y, c.world_z);
     const int stride = mc::world_stride_bytes(c.world_x, c.world_y, c.world_z);
     if (epw_of(s) * stride > mc::kMaxLdsWorld)
@@ -737,8 +753,10 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
   a.persistent = b->persistent;
   a.counters = b->counters + 2 * slot;
   a.entries = de;
-  const int r = (nt == 512) ? launch_multi<512>(b, a, lds, S(stream))
-                            : launch_multi<256>(b, a, lds, S(stream));
+  const int r = (nt == 512) ? (lift ? launch_multi<512, true>(b, a, lds, S(stream))
+                                   : launch_multi<512, false>(b, a, lds, S(stream)))
+                            : (lift ? launch_multi<256, true>(b, a, lds, S(stream))
+                                    : launch_multi<256, false>(b, a, lds, S(stream)));
   if (r) return r;
   e = hipEventRecord(b->ev[slot], S(stream));
   if (e != hipSuccess) return fail(MC_E_HIP, "batch event: %s", hipGetErrorString(e));
@@ -792,7 +810,8 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
     if (it == rep.end() || it->second->cfg.batch_size < envs[i]->cfg.batch_size) rep[k] = envs[i];
   }
   const hipStream_t st = S(stream);
-  const Shape cands[] = {{64, 1}, {256, 1}, {256, 2}, {256, 4}, {512, 8}};
+  const Shape cands[] = {{64, 1, 0},  {256, 1, 0}, {256, 2, 0}, {256, 4, 0}, {512, 8, 0},
+                         {64, 1, 1},  {256, 1, 1}, {256, 2, 1}, {256, 4, 1}, {512, 8, 1}};
   for (const auto& kv : rep) {
     if (b->tuned.count(kv.first)) continue;
     mc_config c = kv.second->cfg;
@@ -817,7 +836,7 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
     if (hipMemcpy(acts, ha.data(), ha.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
       return fail(MC_E_HIP, "autotune: action upload failed");
     int r = mc_regen_worlds(&c, s.voxels, nullptr, stream);
-    if (!r) r = reset_with_ring(&c, &s, nullptr, (uint32_t*)ring, 0, default_shape(c.width, c.height), stream);
+    if (!r) r = reset_with_ring(&c, &s, nullptr, (uint32_t*)ring, 0, default_shape(c.width, c.height, c.world_y), stream);
     mc::Params p = make_params(&c, &s);
     p.ring = (uint32_t*)ring;
     p.action_kind = MC_ACT_DISCRETE;
@@ -826,7 +845,7 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
     for (int t = 0; t < 8 && !r; ++t) {
       p.actions = (const int64_t*)acts + (size_t)t * B;
       p.slot = slot;
-      r = launch_env(mc::kItemStep, p, c.obs_mode, default_shape(c.width, c.height), st);
+      r = launch_env(mc::kItemStep, p, c.obs_mode, default_shape(c.width, c.height, c.world_y), st);
       slot = (slot + 1) % 3;
     }
     if (r) return r;
@@ -850,7 +869,7 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
       return fail(MC_E_HIP, "autotune: event create failed");
-    Shape best = default_shape(c.width, c.height);
+    Shape best = default_shape(c.width, c.height, c.world_y);
     float best_ms = 1e30f;
     for (const Shape& cand : cands) {
       const Shape f = fit_lds(cand, c.world_x, c.world_y, c.world_z);
@@ -888,12 +907,13 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
 }
 
 int mc_batch_tuned_shape(const mc_batch* b, const mc_config* cfg, int* threads,
-                         int* waves_per_env) {
+                         int* waves_per_env, int* lift) {
   if (!b || !cfg || !threads || !waves_per_env) return fail(MC_E_INVALID, "NULL argument");
   auto it = b->tuned.find(bucket_of(*cfg));
   if (it == b->tuned.end()) return fail(MC_E_INVALID, "shape not tuned");
   *threads = it->second.nt;
   *waves_per_env = it->second.wpe;
+  if (lift) *lift = it->second.lift;
   return MC_OK;
 }
 

@@ -401,7 +401,7 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, 
     // single loop exit, the advance is branch-free, and the next voxel's LDS
     // read is issued before the loop comes back to test it.
     int by = INT_MAX, bxh = INT_MAX, bxl = INT_MIN, bzh = INT_MAX, bzl = INT_MIN;
-    if (LIFT && step_y > 0) {
+    if (LIFT && f.lift && step_y > 0) {
       by = f.hy1;
       if (step_x > 0) bxh = f.hx1; else bxl = f.hx0;
       if (step_z > 0) bzh = f.hz1; else bzl = f.hz0;
@@ -476,7 +476,7 @@ __device__ __forceinline__ void fill_uv(const Params& p, float* su, float* sv, i
   for (int i = tid; i < p.H; i += nt) sv[i] = v_of(p, i);
 }
 
-template <int OBS>
+template <int OBS, bool LIFT>
 __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t* world,
                                            const float* pal, const Frame& f, int tid, int nt,
                                            const float* su, const float* sv) {
@@ -520,8 +520,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
         auto shade = [&](int j) -> uint32_t {
           const float u = ucol(px0 + j);
           float rgb[3];
-          if (f.lift) trace<true>(p, f, u, v, world, pal, rgb);
-          else trace<false>(p, f, u, v, world, pal, rgb);
+          trace<LIFT>(p, f, u, v, world, pal, rgb);
           return to_u8(rgb[0]) | (to_u8(rgb[1]) << 8) | (to_u8(rgb[2]) << 16);
         };
         const uint32_t c0 = shade(0), c1 = shade(1), c2 = shade(2), c3 = shade(3);
@@ -539,8 +538,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
         if (f.fill_block >= 0) {
           for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
         } else {
-          if (f.lift) trace<true>(p, f, u, v, world, pal, rgb);
-          else trace<false>(p, f, u, v, world, pal, rgb);
+          trace<LIFT>(p, f, u, v, world, pal, rgb);
         }
         for (int c = 0; c < 3; ++c) out[(int64_t)pi * 3 + c] = (uint8_t)to_u8(rgb[c]);
       }
@@ -564,8 +562,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
 #pragma unroll
             for (int c = 0; c < 3; ++c) vals[j * 3 + c] = pal[f.fill_block * 3 + c];
           } else {
-            if (f.lift) trace<true>(p, f, u, v, world, pal, &vals[j * 3]);
-            else trace<false>(p, f, u, v, world, pal, &vals[j * 3]);
+            trace<LIFT>(p, f, u, v, world, pal, &vals[j * 3]);
           }
         }
         gfloat4* o = (gfloat4*)(out + (int64_t)p0 * 3);
@@ -582,8 +579,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
         if (f.fill_block >= 0) {
           for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
         } else {
-          if (f.lift) trace<true>(p, f, u, v, world, pal, rgb);
-          else trace<false>(p, f, u, v, world, pal, rgb);
+          trace<LIFT>(p, f, u, v, world, pal, rgb);
         }
         for (int c = 0; c < 3; ++c) out[(int64_t)pi * 3 + c] = rgb[c];
       }
@@ -881,7 +877,7 @@ __host__ __device__ __forceinline__ int world_stride_bytes(int X, int Y, int Z) 
   return (X * Y * Z + 15) & ~15;
 }
 
-template <int NT, int KIND>
+template <int NT, int KIND, bool LIFT>
 __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, int wpe,
                                          int8_t* lds, const float* pal, Frame* frames,
                                          float* su, float* sv) {
@@ -917,13 +913,15 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     if (et == 0) frames[e] = f;
   }
   __syncthreads();
-  if (active && frames[e].lift)
-    solid_extent(p, world, &frames[e], (int)floorf(frames[e].ey), et, ent);
-  __syncthreads();
+  if (LIFT) {
+    if (active && frames[e].lift)
+      solid_extent(p, world, &frames[e], (int)floorf(frames[e].ey), et, ent);
+    __syncthreads();
+  }
   if (active) {
     const Frame f = frames[e];
-    if (obs_mode == 0) render_env<0>(p, b, world, pal, f, et, ent, su, sv);
-    else render_env<1>(p, b, world, pal, f, et, ent, su, sv);
+    if (obs_mode == 0) render_env<0, LIFT>(p, b, world, pal, f, et, ent, su, sv);
+    else render_env<1, LIFT>(p, b, world, pal, f, et, ent, su, sv);
     if (KIND == kItemStep && any_done) store_fresh_world(p, gworld, nullptr, nvox, et, ent);
   }
 }
@@ -933,7 +931,7 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
 // ---------------------------------------------------------------------------
 
 // One group, one item per workgroup: grid = ceil(B / epw).
-template <int NT, int KIND>
+template <int NT, int KIND, bool LIFT>
 __global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
@@ -941,7 +939,7 @@ __global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
   __shared__ float s_u[kMaxUV], s_v[kMaxUV];
   load_palette(s_pal, threadIdx.x);
   const int epw = (NT / kWave) / wpe;
-  run_item<NT, KIND>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_frames, s_u, s_v);
+  run_item<NT, KIND, LIFT>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_frames, s_u, s_v);
 }
 
 // Several env groups (uneven batch sizes, world sizes, resolutions) in one
@@ -972,7 +970,7 @@ struct MultiArgs {
   const MultiEntry* entries;   // [n_groups] device memory
 };
 
-template <int NT>
+template <int NT, bool LIFT>
 __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
@@ -994,7 +992,16 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
     int g = 0;
     while (g + 1 < a.n_groups && s_begin[g + 1] <= item) ++g;
     const MultiEntry& me = a.entries[g];
-    Params p = *me.desc;
+    // The descriptor is uniform over the workgroup: keep it in scalar
+    // registers (the vector file is needed by the ray loops).
+    Params p;
+    {
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(me.desc);
+      uint32_t* dst = reinterpret_cast<uint32_t*>(&p);
+#pragma unroll
+      for (int i = 0; i < (int)(sizeof(Params) / 4); ++i)
+        dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    }
     p.actions = me.actions;
     p.look = me.look;
     p.dones = me.dones;
@@ -1002,8 +1009,8 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
     p.slot = me.slot;
     const int wpe = me.wpe;
     const int epw = (NT / kWave) / wpe;
-    run_item<NT, kItemStep>(p, me.obs_mode, (item - s_begin[g]) * epw, wpe, s_lds, s_pal,
-                            s_frames, s_u, s_v);
+    run_item<NT, kItemStep, LIFT>(p, me.obs_mode, (item - s_begin[g]) * epw, wpe, s_lds, s_pal,
+                                  s_frames, s_u, s_v);
     if (!a.persistent) break;
   }
   if (a.persistent && threadIdx.x == 0) {

@@ -291,16 +291,19 @@ class MineEnv:
         return bt, _lib.MC_ACT_BUTTONS, lk
 
     @property
-    def shape(self) -> Tuple[int, int]:
-        """Launch shape (threads per workgroup, waves per env) of this group."""
-        t, w = ctypes.c_int(), ctypes.c_int()
-        _lib.check(_lib.load().mc_env_get_shape(self._handle, ctypes.byref(t), ctypes.byref(w)))
-        return t.value, w.value
+    def shape(self) -> Tuple[int, int, int]:
+        """Launch shape (threads per workgroup, waves per env, upward-ray exit
+        0/1) of this group."""
+        t, w, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().mc_env_get_shape(self._handle, ctypes.byref(t), ctypes.byref(w),
+                                                ctypes.byref(l)))
+        return t.value, w.value, l.value
 
-    def set_shape(self, threads: int = 0, waves_per_env: int = 0) -> None:
-        """Override the launch shape ((0, 0) restores the default).  Results
-        do not depend on it."""
-        _lib.check(_lib.load().mc_env_set_shape(self._handle, threads, waves_per_env), "set_shape")
+    def set_shape(self, threads: int = 0, waves_per_env: int = 0, lift: int = -1) -> None:
+        """Override the launch shape ((0, 0) restores the default size; lift
+        -1 keeps the current value).  Results do not depend on it."""
+        _lib.check(_lib.load().mc_env_set_shape(self._handle, threads, waves_per_env, lift),
+                   "set_shape")
 
     def _render(self) -> torch.Tensor:
         """minecu/__init__.py:446-466: cameras from state, then render."""

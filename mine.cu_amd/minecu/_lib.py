@@ -68,14 +68,14 @@ SIGNATURES = {
     "mc_env_step": ([_P, _P, _I, _P, _P, _P], _I),
     "mc_env_reset": ([_P, _P, _P], _I),
     "mc_env_render": ([_P, _P], _I),
-    "mc_env_set_shape": ([_P, _I, _I], _I),
-    "mc_env_get_shape": ([_P, _P, _P], _I),
+    "mc_env_set_shape": ([_P, _I, _I, _I], _I),
+    "mc_env_get_shape": ([_P, _P, _P, _P], _I),
     "mc_batch_create": ([_P], _I),
     "mc_batch_destroy": ([_P], _I),
     "mc_batch_set_persistent": ([_P, _I], _I),
     "mc_batch_step": ([_P, _P, _P, _P, _P, _P, _I, _P], _I),
     "mc_batch_autotune": ([_P, _P, _I, _I, _P], _I),
-    "mc_batch_tuned_shape": ([_P, _P, _P, _P], _I),
+    "mc_batch_tuned_shape": ([_P, _P, _P, _P, _P], _I),
     "mc_server_create": ([ctypes.c_int64, _I, _P], _I),
     "mc_server_destroy": ([_P], _I),
     "mc_server_last_error": ([_P], ctypes.c_char_p),

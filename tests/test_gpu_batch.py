@@ -93,9 +93,11 @@ def test_batch_matches_oracle():
     assert st.exact, st.summary()
 
 
-@pytest.mark.parametrize("shape", [(64, 1), (256, 1), (256, 2), (256, 4), (512, 8)])
-def test_launch_shape_does_not_change_results(shape):
-    kw = dict(batch_size=40, world_size=16, render_width=32, render_height=24, obs_dtype="uint8",
+@pytest.mark.parametrize("shape", [(64, 1, 0), (256, 1, 0), (256, 2, 1), (256, 4, 0),
+                                   (512, 8, 1), (64, 1, 1)])
+@pytest.mark.parametrize("ws", [16, 32])
+def test_launch_shape_does_not_change_results(shape, ws):
+    kw = dict(batch_size=40, world_size=ws, render_width=32, render_height=24, obs_dtype="uint8",
               episode_length=9)
     ref = _make(kw)
     e = _make(kw)
@@ -116,8 +118,8 @@ def test_autotune_then_batch_still_exact():
     batch = minecu.EnvBatch()
     shapes = batch.autotune(mixed, steps=6)
     assert len(shapes) == len(kws)
-    for t_, w_ in shapes:
-        assert t_ in (64, 256, 512) and w_ in (1, 2, 4, 8) and w_ * 64 <= t_
+    for t_, w_, l_ in shapes:
+        assert t_ in (64, 256, 512) and w_ in (1, 2, 4, 8) and w_ * 64 <= t_ and l_ in (0, 1)
     acts = [_actions(T, kw["batch_size"], 40 + i) for i, kw in enumerate(kws)]
     for t in range(T):
         batch.step(mixed, [a[t] for a in acts])
