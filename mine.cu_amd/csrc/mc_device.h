@@ -124,7 +124,29 @@ struct Frame {
   // `lift` is set; used to end upward rays early (see trace()).
   int hx0, hx1, hz0, hz1, hy1;
   int lift;
+  // inside: the eye voxel (ivx, ivy, ivz) lies in the world and every world
+  // dimension is at most kMaxInsideDim; then every ray starts in the world
+  // and trace() takes its in-world loop.  d*p / d*n: distance from the eye
+  // to the voxel's upper / lower face per axis, (float)(iv + 1) - e and
+  // (float)iv - e.
+  int inside;
+  int ivx, ivy, ivz;
+  float dxp, dxn, dyp, dyn, dzp, dzn;
 };
+
+// Largest world dimension of the in-world DDA loop (10-bit packed position).
+constexpr int kMaxInsideDim = 512;
+
+// Frame fields are uniform over the waves of one env: read them into scalar
+// registers.
+__device__ __forceinline__ Frame uniform_frame(const Frame* src) {
+  Frame f;
+  const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+  uint32_t* d = reinterpret_cast<uint32_t*>(&f);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(Frame) / 4); ++i) d[i] = __builtin_amdgcn_readfirstlane(s[i]);
+  return f;
+}
 
 // ---------------------------------------------------------------------------
 // World regeneration, closed form of generate_world_kernel + place_tree_kernel
@@ -284,6 +306,19 @@ __device__ __forceinline__ Frame make_frame(const Params& p, float x, float y, f
   }
   // The solid-extent scan reads whole 16-byte rows of z.
   f.lift = (f.fill_block < 0) && ((p.Z & 15) == 0);
+  // floor(e) lies in [0, n) exactly when 0 <= e < n (false for NaN).
+  f.inside = f.ex >= 0.0f && f.ex < (float)p.X && f.ey >= 0.0f && f.ey < (float)p.Y &&
+             f.ez >= 0.0f && f.ez < (float)p.Z && p.X <= kMaxInsideDim &&
+             p.Y <= kMaxInsideDim && p.Z <= kMaxInsideDim;
+  f.ivx = f.inside ? (int)floorf(f.ex) : 0;
+  f.ivy = f.inside ? (int)floorf(f.ey) : 0;
+  f.ivz = f.inside ? (int)floorf(f.ez) : 0;
+  f.dxp = (float)(f.ivx + 1) - f.ex;
+  f.dxn = (float)f.ivx - f.ex;
+  f.dyp = (float)(f.ivy + 1) - f.ey;
+  f.dyn = (float)f.ivy - f.ey;
+  f.dzp = (float)(f.ivz + 1) - f.ez;
+  f.dzn = (float)f.ivz - f.ez;
   return f;
 }
 
@@ -346,11 +381,21 @@ __device__ __forceinline__ void solid_extent(const Params& p, const int8_t* worl
 // One ray: AABB cull + DDA + face shade + fog (src/kernels.cu:51-79, 354-469).
 // Writes the shaded colour in [0,1] to rgb.
 //
+// With the eye voxel in the world (f.inside) the slab test always passes
+// (every slab interval contains t = 0), and once the DDA leaves the world
+// box it never returns: each voxel coordinate moves one way only.  The
+// reference's remaining iterations then see only out-of-bounds voxels and
+// end without a hit, so leaving the world ends the ray as sky.  That loop
+// tracks the position as three 10-bit fields of one word, each counting
+// voxels in the direction of travel and biased so that its top bit sets on
+// the step that leaves the world: one AND tests all three bounds.
+//
 // LIFT: for a ray with ray_y >= 0 the DDA's voxel y never decreases, so once
 // the current voxel lies beyond every solid voxel at heights >= floor(eye y)
 // (above hy1, or past [hx0, hx1] / [hz0, hz1] in the direction of travel),
 // every voxel it can still visit is air or outside the world: the reference's
-// result is sky, and the loop stops there.
+// result is sky.  Those bounds tighten the per-axis limits of the packed
+// position, so the same test ends the ray there.
 // ---------------------------------------------------------------------------
 template <bool LIFT>
 __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, float v,
@@ -368,72 +413,109 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, 
   const float inv_x = (ray_x != 0.0f) ? 1.0f / ray_x : 1e30f;
   const float inv_y = (ray_y != 0.0f) ? 1.0f / ray_y : 1e30f;
   const float inv_z = (ray_z != 0.0f) ? 1.0f / ray_z : 1e30f;
-  const float t1 = (0.0f - f.ex) * inv_x, t2 = ((float)p.X - f.ex) * inv_x;
-  const float t3 = (0.0f - f.ey) * inv_y, t4 = ((float)p.Y - f.ey) * inv_y;
-  const float t5 = (0.0f - f.ez) * inv_z, t6 = ((float)p.Z - f.ez) * inv_z;
-  const float t_entry = fmaxf(fmaxf(fminf(t1, t2), fminf(t3, t4)), fminf(t5, t6));
-  const float t_exit = fminf(fminf(fmaxf(t1, t2), fmaxf(t3, t4)), fmaxf(t5, t6));
   const float vd = p.view_distance;
+  const int max_steps = p.max_steps;
+  const int X = p.X, Y = p.Y, Z = p.Z, XZ = X * Z;
+  const int step_x = (ray_x >= 0) ? 1 : -1;
+  const int step_y = (ray_y >= 0) ? 1 : -1;
+  const int step_z = (ray_z >= 0) ? 1 : -1;
+  const float td_x = fabsf(inv_x), td_y = fabsf(inv_y), td_z = fabsf(inv_z);
+  const int dix = step_x * Z, diy = step_y * XZ, diz = step_z;
   int hit_block = -1;
   int hit_face = -1;
   float t = 0.0f;
-  int step_y = (ray_y >= 0) ? 1 : -1;
-  if (!(t_exit < 0 || t_entry > t_exit) && !(t_entry > vd)) {
-    int vx = (int)floorf(f.ex), vy = (int)floorf(f.ey), vz = (int)floorf(f.ez);
-    const int step_x = (ray_x >= 0) ? 1 : -1;
-    const int step_z = (ray_z >= 0) ? 1 : -1;
-    const float td_x = fabsf(inv_x), td_y = fabsf(inv_y), td_z = fabsf(inv_z);
-    float tm_x = (ray_x >= 0) ? ((float)(vx + 1) - f.ex) * inv_x : ((float)vx - f.ex) * inv_x;
-    float tm_y = (ray_y >= 0) ? ((float)(vy + 1) - f.ey) * inv_y : ((float)vy - f.ey) * inv_y;
-    float tm_z = (ray_z >= 0) ? ((float)(vz + 1) - f.ez) * inv_z : ((float)vz - f.ez) * inv_z;
-    const int X = p.X, Y = p.Y, Z = p.Z, XZ = X * Z;
-    const int max_steps = p.max_steps;
-    // Linear voxel index kept in step with (vx, vy, vz); read only in bounds.
-    const int dix = step_x * Z, diy = step_y * XZ, diz = step_z;
-    int idx = vy * XZ + vx * Z + vz;
-    bool inb = (unsigned)vx < (unsigned)X && (unsigned)vy < (unsigned)Y && (unsigned)vz < (unsigned)Z;
-    // Out-of-bounds positions read voxel 0 (always valid) and discard it, so
-    // the read needs no branch.
-    int blk = inb ? (int)world[inb ? idx : 0] : -1;
-    int face = -1;
-    // The reference's iteration is (limit check, lookup, advance).  Here the
-    // limit check of iteration k+1 closes iteration k, the lookup decides a
-    // single loop exit, the advance is branch-free, and the next voxel's LDS
-    // read is issued before the loop comes back to test it.
-    int by = INT_MAX, bxh = INT_MAX, bxl = INT_MIN, bzh = INT_MAX, bzl = INT_MIN;
+  if (f.inside) {
+    float tm_x = ((ray_x >= 0) ? f.dxp : f.dxn) * inv_x;
+    float tm_y = ((ray_y >= 0) ? f.dyp : f.dyn) * inv_y;
+    float tm_z = ((ray_z >= 0) ? f.dzp : f.dzn) * inv_z;
+    // Per-axis limits in travel coordinates (voxels counted from the world
+    // face the ray moves away from).
+    int lim_x = X, lim_y = Y, lim_z = Z;
     if (LIFT && f.lift && step_y > 0) {
-      by = f.hy1;
-      if (step_x > 0) bxh = f.hx1; else bxl = f.hx0;
-      if (step_z > 0) bzh = f.hz1; else bzl = f.hz0;
+      lim_y = max(0, min(Y, f.hy1 + 1));
+      lim_x = max(0, (step_x > 0) ? min(X, f.hx1 + 1) : min(X, X - f.hx0));
+      lim_z = max(0, (step_z > 0) ? min(Z, f.hz1 + 1) : min(Z, Z - f.hz0));
     }
-    bool go = !(0 >= max_steps || t >= vd);
+    const int tx = (step_x > 0) ? f.ivx : X - 1 - f.ivx;
+    const int ty = (step_y > 0) ? f.ivy : Y - 1 - f.ivy;
+    const int tz = (step_z > 0) ? f.ivz : Z - 1 - f.ivz;
+    constexpr uint32_t kGuard = (1u << 9) | (1u << 19) | (1u << 29);
+    uint32_t pos = (uint32_t)(tx + 512 - lim_x) | ((uint32_t)(ty + 512 - lim_y) << 10) |
+                   ((uint32_t)(tz + 512 - lim_z) << 20);
+    int idx = f.ivy * XZ + f.ivx * Z + f.ivz;
+    bool out = (pos & kGuard) != 0u;
+    int blk = (int)world[idx];
+    bool lim = (max_steps <= 0) | (t >= vd);
+    bool last_x = false, last_y = false;
     int step = 0;
-    while (go) {
-      // blk >= 0 implies in bounds (out-of-bounds blk is -1); a dead voxel is
-      // never solid, so a LIFT exit is a no-hit exit.
-      const bool dead = LIFT && ((vy > by) | (vx > bxh) | (vx < bxl) | (vz > bzh) | (vz < bzl));
-      if ((blk >= 0) | ((!inb) & (t > t_exit)) | dead) break;
-      const bool cx = tm_x < tm_y && tm_x < tm_z;
-      const bool cy = !cx && (tm_y < tm_z);
-      const bool cz = !cx && !cy;
-      t = cx ? tm_x : (cy ? tm_y : tm_z);
+    // The reference's iteration is (limit check, lookup, advance); here the
+    // loop test holds the limit check and the lookup, the advance is
+    // branch-free, and the next voxel's LDS read is issued before the test.
+    while (!(lim | out | (blk >= 0))) {
+      const bool cx = (tm_x < tm_y) & (tm_x < tm_z);
+      const bool cy = !cx & (tm_y < tm_z);
+      const bool cz = !cx & !cy;
+      t = fminf(fminf(tm_x, tm_y), tm_z);
       tm_x = cx ? tm_x + td_x : tm_x;
       tm_y = cy ? tm_y + td_y : tm_y;
       tm_z = cz ? tm_z + td_z : tm_z;
-      vx = cx ? vx + step_x : vx;
-      vy = cy ? vy + step_y : vy;
-      vz = cz ? vz + step_z : vz;
       idx += cx ? dix : (cy ? diy : diz);
-      face = cx ? 0 : (cy ? 1 : 2);
-      inb = (unsigned)vx < (unsigned)X && (unsigned)vy < (unsigned)Y && (unsigned)vz < (unsigned)Z;
-      const int raw = (int)world[inb ? idx : 0];
-      blk = inb ? raw : -1;
+      pos += cx ? 1u : (cy ? (1u << 10) : (1u << 20));
+      out = (pos & kGuard) != 0u;
+      blk = (int)world[out ? 0 : idx];
       ++step;
-      go = !(step >= max_steps || t >= vd);
+      lim = (step >= max_steps) | (t >= vd);
+      last_x = cx;
+      last_y = cy;
     }
-    if (go && inb && blk >= 0) {
+    if (!lim & !out & (blk >= 0)) {
       hit_block = blk;
-      hit_face = face;
+      hit_face = (step == 0) ? -1 : (last_x ? 0 : (last_y ? 1 : 2));
+    }
+  } else {
+    const float t1 = (0.0f - f.ex) * inv_x, t2 = ((float)X - f.ex) * inv_x;
+    const float t3 = (0.0f - f.ey) * inv_y, t4 = ((float)Y - f.ey) * inv_y;
+    const float t5 = (0.0f - f.ez) * inv_z, t6 = ((float)Z - f.ez) * inv_z;
+    const float t_entry = fmaxf(fmaxf(fminf(t1, t2), fminf(t3, t4)), fminf(t5, t6));
+    const float t_exit = fminf(fminf(fmaxf(t1, t2), fmaxf(t3, t4)), fmaxf(t5, t6));
+    if (!(t_exit < 0 || t_entry > t_exit) && !(t_entry > vd)) {
+      int vx = (int)floorf(f.ex), vy = (int)floorf(f.ey), vz = (int)floorf(f.ez);
+      float tm_x = (ray_x >= 0) ? ((float)(vx + 1) - f.ex) * inv_x : ((float)vx - f.ex) * inv_x;
+      float tm_y = (ray_y >= 0) ? ((float)(vy + 1) - f.ey) * inv_y : ((float)vy - f.ey) * inv_y;
+      float tm_z = (ray_z >= 0) ? ((float)(vz + 1) - f.ez) * inv_z : ((float)vz - f.ez) * inv_z;
+      // Linear voxel index kept in step with (vx, vy, vz); out-of-bounds
+      // positions read voxel 0 (always valid) and discard it.
+      int idx = vy * XZ + vx * Z + vz;
+      bool inb = (unsigned)vx < (unsigned)X && (unsigned)vy < (unsigned)Y && (unsigned)vz < (unsigned)Z;
+      int blk = inb ? (int)world[inb ? idx : 0] : -1;
+      int face = -1;
+      bool go = !(0 >= max_steps || t >= vd);
+      int step = 0;
+      while (go) {
+        // blk >= 0 implies in bounds (out-of-bounds blk is -1).
+        if ((blk >= 0) | ((!inb) & (t > t_exit))) break;
+        const bool cx = tm_x < tm_y && tm_x < tm_z;
+        const bool cy = !cx && (tm_y < tm_z);
+        const bool cz = !cx && !cy;
+        t = cx ? tm_x : (cy ? tm_y : tm_z);
+        tm_x = cx ? tm_x + td_x : tm_x;
+        tm_y = cy ? tm_y + td_y : tm_y;
+        tm_z = cz ? tm_z + td_z : tm_z;
+        vx = cx ? vx + step_x : vx;
+        vy = cy ? vy + step_y : vy;
+        vz = cz ? vz + step_z : vz;
+        idx += cx ? dix : (cy ? diy : diz);
+        face = cx ? 0 : (cy ? 1 : 2);
+        inb = (unsigned)vx < (unsigned)X && (unsigned)vy < (unsigned)Y && (unsigned)vz < (unsigned)Z;
+        const int raw = (int)world[inb ? idx : 0];
+        blk = inb ? raw : -1;
+        ++step;
+        go = !(step >= max_steps || t >= vd);
+      }
+      if (go && inb && blk >= 0) {
+        hit_block = blk;
+        hit_face = face;
+      }
     }
   }
   if (hit_block >= 0 && hit_block < 13) {
@@ -919,7 +1001,7 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     __syncthreads();
   }
   if (active) {
-    const Frame f = frames[e];
+    const Frame f = uniform_frame(&frames[e]);
     if (obs_mode == 0) render_env<0, LIFT>(p, b, world, pal, f, et, ent, su, sv);
     else render_env<1, LIFT>(p, b, world, pal, f, et, ent, su, sv);
     if (KIND == kItemStep && any_done) store_fresh_world(p, gworld, nullptr, nvox, et, ent);
