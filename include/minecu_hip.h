@@ -42,8 +42,16 @@ enum {
 
 /* Observation encodings (the reference's `obs_dtype`, minecu/__init__.py:167). */
 enum {
-  MC_OBS_U8 = 0,   /* shaded + fogged, truncated to uint8  (render_kernel_uint8, kernels.cu:300-470) */
-  MC_OBS_F32 = 1   /* shaded + fogged float32 in [0,1]     (render_kernel, kernels.cu:87-293) */
+  MC_OBS_U8 = 0,          /* shaded + fogged, truncated to uint8  (render_kernel_uint8, kernels.cu:300-470) */
+  MC_OBS_F32 = 1,         /* shaded + fogged float32 in [0,1]     (render_kernel, kernels.cu:87-293) */
+  /* Flat block colours, no fog, no view-distance limit (kernels.cu:479-495).  Steps render
+   * the flat image; the reset render and mc_render_state stay shaded uint8, as MineEnv's
+   * _render() does for every uint8 variant (minecu/__init__.py:446-466). */
+  MC_OBS_U8_MINIMAL = 2,  /* obs_dtype "uint8_minimal"  (render_kernel_uint8_minimal, kernels.cu:985-1119) */
+  MC_OBS_U8_PREBASIS = 3  /* "uint8_prebasis" / "_full" / "uint8_smem": image plane from the [B,14]
+                             camera basis (render_kernel_uint8_prebasis, kernels.cu:568-700; the
+                             smem kernel, kernels.cu:835-982, renders the same image); steps
+                             write st->basis when it is non-NULL */
 };
 
 /* Action encodings accepted by the step. */
@@ -87,6 +95,7 @@ typedef struct mc_state {
   float* cameras;
   void* obs;
   float* rewards;
+  float* basis;            /* f32 [B,14] camera basis (kernels.cu:503-510), or NULL */
 } mc_state;
 
 /* Minimum workspace (device bytes) for the stateless mc_step / mc_env_reset_state. */
@@ -202,6 +211,90 @@ int mc_launch_render_uint8(const int8_t* voxels, const float* cameras, uint8_t* 
 int mc_launch_render(const int8_t* voxels, const float* cameras, float* output, int batch_size,
                      int world_size, int width, int height, int max_steps, float view_distance,
                      float fov_degrees, void* stream);
+
+/* Replaces launch_step_uint8_minimal (src/kernels.cu:2716-2774; binding
+ * bindings.cpp:632-682): the fused step with the flat minimal render. */
+int mc_launch_step_uint8_minimal(int8_t* voxels, float* positions, float* velocities,
+                                 float* yaws, float* pitches, uint8_t* on_ground,
+                                 const int8_t* buttons, const float* look,
+                                 const uint8_t* do_reset, float* forward_in, float* strafe_in,
+                                 float* delta_yaw_in, float* delta_pitch_in, uint8_t* jump_in,
+                                 uint8_t* do_break, uint8_t* do_place, float* speed_mult,
+                                 float* cameras, uint8_t* obs_buffer, float* rewards,
+                                 int batch_size, int world_size, int width, int height,
+                                 int max_steps, float view_distance, float fov_degrees, float dt,
+                                 float gravity, float walk_speed, float sprint_mult,
+                                 float jump_vel, int8_t target_block, float reward_value,
+                                 float spawn_x, float spawn_y, float spawn_z, void* stream);
+
+/* Replaces launch_step_uint8_prebasis (src/kernels.cu:2993-3076; binding
+ * bindings.cpp:736-792): the fused step writing basis f32 [B,14] and the flat
+ * prebasis render; world extents per axis. */
+int mc_launch_step_uint8_prebasis(int8_t* voxels, float* positions, float* velocities,
+                                  float* yaws, float* pitches, uint8_t* on_ground,
+                                  const int8_t* buttons, const float* look,
+                                  const uint8_t* do_reset, float* forward_in, float* strafe_in,
+                                  float* delta_yaw_in, float* delta_pitch_in, uint8_t* jump_in,
+                                  uint8_t* do_break, uint8_t* do_place, float* speed_mult,
+                                  float* cameras, float* basis, uint8_t* obs_buffer,
+                                  float* rewards, int batch_size, int world_x, int world_y,
+                                  int world_z, int width, int height, int max_steps,
+                                  float view_distance, float fov_degrees, float dt, float gravity,
+                                  float walk_speed, float sprint_mult, float jump_vel,
+                                  int8_t target_block, float reward_value, float spawn_x,
+                                  float spawn_y, float spawn_z, void* stream);
+
+/* Replaces launch_step_uint8_prebasis_full (src/kernels.cu:3081-3168; binding
+ * bindings.cpp:794-875): as the prebasis step, plus the episode check
+ * (do_reset = step_count >= episode_length; episode_length <= 0: no limit) and
+ * the step_count update in the kernel; world regeneration stays with the
+ * caller, as in the reference. */
+int mc_launch_step_uint8_prebasis_full(
+    int8_t* voxels, float* positions, float* velocities, float* yaws, float* pitches,
+    uint8_t* on_ground, const int8_t* buttons, const float* look, uint8_t* do_reset,
+    int32_t* step_count, float* forward_in, float* strafe_in, float* delta_yaw_in,
+    float* delta_pitch_in, uint8_t* jump_in, uint8_t* do_break, uint8_t* do_place,
+    float* speed_mult, float* cameras, float* basis, uint8_t* obs_buffer, float* rewards,
+    int batch_size, int world_x, int world_y, int world_z, int width, int height, int max_steps,
+    float view_distance, float fov_degrees, float dt, float gravity, float walk_speed,
+    float sprint_mult, float jump_vel, int8_t target_block, float reward_value, float spawn_x,
+    float spawn_y, float spawn_z, int episode_length, void* stream);
+
+/* Replaces launch_step_uint8_smem with the argument list its binding declares
+ * (bindings.cpp:195-210, 930-981: cubic world_size); the image equals the
+ * prebasis step's. */
+int mc_launch_step_uint8_smem(int8_t* voxels, float* positions, float* velocities, float* yaws,
+                              float* pitches, uint8_t* on_ground, const int8_t* buttons,
+                              const float* look, const uint8_t* do_reset, float* forward_in,
+                              float* strafe_in, float* delta_yaw_in, float* delta_pitch_in,
+                              uint8_t* jump_in, uint8_t* do_break, uint8_t* do_place,
+                              float* speed_mult, float* cameras, float* basis,
+                              uint8_t* obs_buffer, float* rewards, int batch_size,
+                              int world_size, int width, int height, int max_steps,
+                              float view_distance, float fov_degrees, float dt, float gravity,
+                              float walk_speed, float sprint_mult, float jump_vel,
+                              int8_t target_block, float reward_value, float spawn_x,
+                              float spawn_y, float spawn_z, void* stream);
+
+/* Replaces launch_precompute_camera_basis (src/kernels.cu:2160-2170). */
+int mc_launch_precompute_camera_basis(const float* cameras, float* basis, int batch_size,
+                                      int width, int height, float fov_degrees, void* stream);
+
+/* Replaces launch_render_uint8_minimal (src/kernels.cu:2128-2142). */
+int mc_launch_render_uint8_minimal(const int8_t* voxels, const float* cameras, uint8_t* output,
+                                   int batch_size, int world_size, int width, int height,
+                                   int max_steps, float view_distance, float fov_degrees,
+                                   void* stream);
+
+/* Replaces launch_render_uint8_prebasis (src/kernels.cu:2172-2189; binding
+ * bindings.cpp:877-899) and launch_render_uint8_smem (kernels.cu:2191-2211):
+ * flat render from a stored basis f32 [B,14]. */
+int mc_launch_render_uint8_prebasis(const int8_t* voxels, const float* basis, uint8_t* output,
+                                    int batch_size, int world_size, int width, int height,
+                                    int max_steps, void* stream);
+int mc_launch_render_uint8_smem(const int8_t* voxels, const float* basis, uint8_t* output,
+                                int batch_size, int world_size, int width, int height,
+                                int max_steps, void* stream);
 
 /* ---- env handle (no reference counterpart: per-group state instead of the
  * reference's process-global StepGraphState, kernels.cu:2338-2382) ------- */

@@ -59,8 +59,8 @@ int validate_config(const mc_config* c) {
   if (r) return r;
   r = check_image(c->batch_size, c->width, c->height);
   if (r) return r;
-  if (c->obs_mode != MC_OBS_U8 && c->obs_mode != MC_OBS_F32)
-    return fail(MC_E_INVALID, "obs_mode must be MC_OBS_U8 or MC_OBS_F32 (got %d)", c->obs_mode);
+  if (c->obs_mode < MC_OBS_U8 || c->obs_mode > MC_OBS_U8_PREBASIS)
+    return fail(MC_E_INVALID, "obs_mode must be one of MC_OBS_* (got %d)", c->obs_mode);
   return MC_OK;
 }
 
@@ -85,6 +85,8 @@ mc::Params make_params(const mc_config* c, const mc_state* s) {
   p.view_distance = c->view_distance;
   p.half_fov = mc_half_fov_tan(c->fov_degrees);
   p.aspect = (float)c->width / (float)c->height;
+  p.flat_hf = mc_half_fov_tan_flat(c->fov_degrees);
+  p.flat_su = p.flat_hf * p.aspect;
   p.dt = c->dt;
   p.gravity = c->gravity;
   p.walk_speed = c->walk_speed;
@@ -115,6 +117,7 @@ mc::Params make_params(const mc_config* c, const mc_state* s) {
     p.cameras = s->cameras;
     p.obs = s->obs;
     p.rewards = s->rewards;
+    p.basis = s->basis;
   }
   return p;
 }
@@ -169,6 +172,8 @@ int launch_nt(int kind, const mc::Params& p, int obs_mode, int wpe, hipStream_t 
       return launch_kind<NT, mc::kItemResetRender, LIFT>(p, obs_mode, wpe, st);
     case mc::kItemRenderCameras:
       return launch_kind<NT, mc::kItemRenderCameras, LIFT>(p, obs_mode, wpe, st);
+    case mc::kItemRenderBasis:
+      return launch_kind<NT, mc::kItemRenderBasis, LIFT>(p, obs_mode, wpe, st);
     default: return launch_kind<NT, mc::kItemRenderState, LIFT>(p, obs_mode, wpe, st);
   }
 }
@@ -267,7 +272,7 @@ int mc_render(const int8_t* voxels, const float* cameras, void* obs, int obs_mod
   if (r) return r;
   r = check_image(B, W, H);
   if (r) return r;
-  if (obs_mode != MC_OBS_U8 && obs_mode != MC_OBS_F32)
+  if (obs_mode < MC_OBS_U8 || obs_mode > MC_OBS_U8_PREBASIS)
     return fail(MC_E_INVALID, "bad obs_mode %d", obs_mode);
   mc_config c;
   memset(&c, 0, sizeof c);
@@ -451,7 +456,19 @@ int mc_env_render(mc_env* env, void* stream) {
 }
 
 namespace {
-int launch_step_ref(int obs_mode, int8_t* voxels, float* positions, float* velocities,
+// Variant-specific arguments of the reference-signature steps: world extents,
+// the prebasis basis output, and the prebasis_full episode bookkeeping
+// (step_count non-NULL: do_reset is computed from step_count >= episode_length
+// and step_count advanced in the kernel, episode_length <= 0 meaning none).
+struct RefStep {
+  int X, Y, Z;
+  float* basis;
+  int32_t* step_count;
+  int episode_length;
+};
+
+int launch_step_ref(int obs_mode, const RefStep& v, int8_t* voxels, float* positions,
+                    float* velocities,
                     float* yaws, float* pitches, uint8_t* on_ground, const int8_t* buttons,
                     const float* look, const uint8_t* do_reset, float* forward_in,
                     float* strafe_in, float* delta_yaw_in, float* delta_pitch_in,
@@ -470,14 +487,17 @@ int launch_step_ref(int obs_mode, int8_t* voxels, float* positions, float* veloc
                         do_break && do_place && speed_mult;
   if (any_ctrl && !all_ctrl)
     return fail(MC_E_INVALID, "step: decode buffers must be all NULL or all non-NULL");
-  int r = check_dims(B, ws, ws, ws);
+  (void)ws;
+  int r = check_dims(B, v.X, v.Y, v.Z);
   if (r) return r;
   r = check_image(B, W, H);
   if (r) return r;
   mc_config c;
   memset(&c, 0, sizeof c);
   c.batch_size = B;
-  c.world_x = c.world_y = c.world_z = ws;
+  c.world_x = v.X;
+  c.world_y = v.Y;
+  c.world_z = v.Z;
   c.width = W;
   c.height = H;
   c.max_steps = max_steps;
@@ -509,8 +529,15 @@ int launch_step_ref(int obs_mode, int8_t* voxels, float* positions, float* veloc
   p.actions = buttons;
   p.action_kind = MC_ACT_BUTTONS;
   p.look = look;
-  p.done_mode = 1;
   p.ring = nullptr;
+  p.basis = v.basis;
+  if (v.step_count) {
+    p.done_mode = 0;
+    p.step_count = v.step_count;
+    p.episode_length = v.episode_length > 0 ? v.episode_length : -1;
+  } else {
+    p.done_mode = 1;
+  }
   if (all_ctrl) {
     p.forward_in = forward_in;
     p.strafe_in = strafe_in;
@@ -521,7 +548,7 @@ int launch_step_ref(int obs_mode, int8_t* voxels, float* positions, float* veloc
     p.do_place = do_place;
     p.speed_mult_in = speed_mult;
   }
-  return launch_env(mc::kItemStep, p, obs_mode, default_shape(W, H, ws), S(stream));
+  return launch_env(mc::kItemStep, p, obs_mode, default_shape(W, H, v.Y), S(stream));
 }
 }  // namespace
 
@@ -536,7 +563,8 @@ int mc_launch_step_uint8(int8_t* voxels, float* positions, float* velocities, fl
                          float walk_speed, float sprint_mult, float jump_vel, int8_t target_block,
                          float reward_value, float spawn_x, float spawn_y, float spawn_z,
                          void* stream) {
-  return launch_step_ref(MC_OBS_U8, voxels, positions, velocities, yaws, pitches, on_ground,
+  return launch_step_ref(MC_OBS_U8, RefStep{world_size, world_size, world_size, nullptr, nullptr, 0},
+                         voxels, positions, velocities, yaws, pitches, on_ground,
                          buttons, look, do_reset, forward_in, strafe_in, delta_yaw_in,
                          delta_pitch_in, jump_in, do_break, do_place, speed_mult, cameras,
                          obs_buffer, rewards, batch_size, world_size, width, height, max_steps,
@@ -554,7 +582,8 @@ int mc_launch_step(int8_t* voxels, float* positions, float* velocities, float* y
                    float gravity, float walk_speed, float sprint_mult, float jump_vel,
                    int8_t target_block, float reward_value, float spawn_x, float spawn_y,
                    float spawn_z, void* stream) {
-  return launch_step_ref(MC_OBS_F32, voxels, positions, velocities, yaws, pitches, on_ground,
+  return launch_step_ref(MC_OBS_F32, RefStep{world_size, world_size, world_size, nullptr, nullptr, 0},
+                         voxels, positions, velocities, yaws, pitches, on_ground,
                          buttons, look, do_reset, forward_in, strafe_in, delta_yaw_in,
                          delta_pitch_in, jump_in, do_break, do_place, speed_mult, cameras,
                          obs_buffer, rewards, batch_size, world_size, width, height, max_steps,
@@ -574,6 +603,145 @@ int mc_launch_render(const int8_t* voxels, const float* cameras, float* output, 
                      float fov_degrees, void* stream) {
   return mc_render(voxels, cameras, output, MC_OBS_F32, batch_size, world_size, world_size,
                    world_size, width, height, max_steps, view_distance, fov_degrees, stream);
+}
+
+#define MC_REF_STEP_ARGS                                                                     \
+  voxels, positions, velocities, yaws, pitches, on_ground, buttons, look, do_reset, forward_in, \
+      strafe_in, delta_yaw_in, delta_pitch_in, jump_in, do_break, do_place, speed_mult, cameras, \
+      obs_buffer, rewards, batch_size
+#define MC_REF_STEP_TAIL                                                                      \
+  width, height, max_steps, view_distance, fov_degrees, dt, gravity, walk_speed, sprint_mult,    \
+      jump_vel, target_block, reward_value, spawn_x, spawn_y, spawn_z, stream
+
+int mc_launch_step_uint8_minimal(int8_t* voxels, float* positions, float* velocities,
+                                 float* yaws, float* pitches, uint8_t* on_ground,
+                                 const int8_t* buttons, const float* look,
+                                 const uint8_t* do_reset, float* forward_in, float* strafe_in,
+                                 float* delta_yaw_in, float* delta_pitch_in, uint8_t* jump_in,
+                                 uint8_t* do_break, uint8_t* do_place, float* speed_mult,
+                                 float* cameras, uint8_t* obs_buffer, float* rewards,
+                                 int batch_size, int world_size, int width, int height,
+                                 int max_steps, float view_distance, float fov_degrees, float dt,
+                                 float gravity, float walk_speed, float sprint_mult,
+                                 float jump_vel, int8_t target_block, float reward_value,
+                                 float spawn_x, float spawn_y, float spawn_z, void* stream) {
+  return launch_step_ref(MC_OBS_U8_MINIMAL,
+                         RefStep{world_size, world_size, world_size, nullptr, nullptr, 0},
+                         MC_REF_STEP_ARGS, world_size, MC_REF_STEP_TAIL);
+}
+
+int mc_launch_step_uint8_prebasis(int8_t* voxels, float* positions, float* velocities,
+                                  float* yaws, float* pitches, uint8_t* on_ground,
+                                  const int8_t* buttons, const float* look,
+                                  const uint8_t* do_reset, float* forward_in, float* strafe_in,
+                                  float* delta_yaw_in, float* delta_pitch_in, uint8_t* jump_in,
+                                  uint8_t* do_break, uint8_t* do_place, float* speed_mult,
+                                  float* cameras, float* basis, uint8_t* obs_buffer,
+                                  float* rewards, int batch_size, int world_x, int world_y,
+                                  int world_z, int width, int height, int max_steps,
+                                  float view_distance, float fov_degrees, float dt, float gravity,
+                                  float walk_speed, float sprint_mult, float jump_vel,
+                                  int8_t target_block, float reward_value, float spawn_x,
+                                  float spawn_y, float spawn_z, void* stream) {
+  if (!basis) return fail(MC_E_INVALID, "step: NULL basis");
+  return launch_step_ref(MC_OBS_U8_PREBASIS,
+                         RefStep{world_x, world_y, world_z, basis, nullptr, 0},
+                         MC_REF_STEP_ARGS, world_y, MC_REF_STEP_TAIL);
+}
+
+int mc_launch_step_uint8_prebasis_full(
+    int8_t* voxels, float* positions, float* velocities, float* yaws, float* pitches,
+    uint8_t* on_ground, const int8_t* buttons, const float* look, uint8_t* do_reset,
+    int32_t* step_count, float* forward_in, float* strafe_in, float* delta_yaw_in,
+    float* delta_pitch_in, uint8_t* jump_in, uint8_t* do_break, uint8_t* do_place,
+    float* speed_mult, float* cameras, float* basis, uint8_t* obs_buffer, float* rewards,
+    int batch_size, int world_x, int world_y, int world_z, int width, int height, int max_steps,
+    float view_distance, float fov_degrees, float dt, float gravity, float walk_speed,
+    float sprint_mult, float jump_vel, int8_t target_block, float reward_value, float spawn_x,
+    float spawn_y, float spawn_z, int episode_length, void* stream) {
+  if (!basis || !step_count) return fail(MC_E_INVALID, "step: NULL basis or step_count");
+  return launch_step_ref(MC_OBS_U8_PREBASIS,
+                         RefStep{world_x, world_y, world_z, basis, step_count, episode_length},
+                         MC_REF_STEP_ARGS, world_y, MC_REF_STEP_TAIL);
+}
+
+int mc_launch_step_uint8_smem(int8_t* voxels, float* positions, float* velocities, float* yaws,
+                              float* pitches, uint8_t* on_ground, const int8_t* buttons,
+                              const float* look, const uint8_t* do_reset, float* forward_in,
+                              float* strafe_in, float* delta_yaw_in, float* delta_pitch_in,
+                              uint8_t* jump_in, uint8_t* do_break, uint8_t* do_place,
+                              float* speed_mult, float* cameras, float* basis,
+                              uint8_t* obs_buffer, float* rewards, int batch_size,
+                              int world_size, int width, int height, int max_steps,
+                              float view_distance, float fov_degrees, float dt, float gravity,
+                              float walk_speed, float sprint_mult, float jump_vel,
+                              int8_t target_block, float reward_value, float spawn_x,
+                              float spawn_y, float spawn_z, void* stream) {
+  if (!basis) return fail(MC_E_INVALID, "step: NULL basis");
+  return launch_step_ref(MC_OBS_U8_PREBASIS,
+                         RefStep{world_size, world_size, world_size, basis, nullptr, 0},
+                         MC_REF_STEP_ARGS, world_size, MC_REF_STEP_TAIL);
+}
+#undef MC_REF_STEP_ARGS
+#undef MC_REF_STEP_TAIL
+
+int mc_launch_precompute_camera_basis(const float* cameras, float* basis, int batch_size,
+                                      int width, int height, float fov_degrees, void* stream) {
+  if (!cameras || !basis) return fail(MC_E_INVALID, "camera basis: NULL pointer");
+  if (batch_size < 1) return fail(MC_E_INVALID, "batch_size must be >= 1 (got %d)", batch_size);
+  int r = check_image(batch_size, width, height);
+  if (r) return r;
+  mc_config c;
+  memset(&c, 0, sizeof c);
+  c.batch_size = batch_size;
+  c.width = width;
+  c.height = height;
+  c.fov_degrees = fov_degrees;
+  mc::Params p = make_params(&c, nullptr);
+  p.cameras = const_cast<float*>(cameras);
+  p.basis = basis;
+  hipLaunchKernelGGL(mc::k_camera_basis, dim3((batch_size + 255) / 256), dim3(256), 0,
+                     S(stream), p);
+  return check_launch("camera basis launch");
+}
+
+int mc_launch_render_uint8_minimal(const int8_t* voxels, const float* cameras, uint8_t* output,
+                                   int batch_size, int world_size, int width, int height,
+                                   int max_steps, float view_distance, float fov_degrees,
+                                   void* stream) {
+  return mc_render(voxels, cameras, output, MC_OBS_U8_MINIMAL, batch_size, world_size,
+                   world_size, world_size, width, height, max_steps, view_distance, fov_degrees,
+                   stream);
+}
+
+int mc_launch_render_uint8_prebasis(const int8_t* voxels, const float* basis, uint8_t* output,
+                                    int batch_size, int world_size, int width, int height,
+                                    int max_steps, void* stream) {
+  if (!voxels || !basis || !output) return fail(MC_E_INVALID, "render: NULL pointer");
+  int r = check_dims(batch_size, world_size, world_size, world_size);
+  if (r) return r;
+  r = check_image(batch_size, width, height);
+  if (r) return r;
+  mc_config c;
+  memset(&c, 0, sizeof c);
+  c.batch_size = batch_size;
+  c.world_x = c.world_y = c.world_z = world_size;
+  c.width = width;
+  c.height = height;
+  c.max_steps = max_steps;
+  mc::Params p = make_params(&c, nullptr);
+  p.voxels = const_cast<int8_t*>(voxels);
+  p.basis = const_cast<float*>(basis);
+  p.obs = output;
+  return launch_env(mc::kItemRenderBasis, p, MC_OBS_U8_PREBASIS,
+                    default_shape(width, height, world_size), S(stream));
+}
+
+int mc_launch_render_uint8_smem(const int8_t* voxels, const float* basis, uint8_t* output,
+                                int batch_size, int world_size, int width, int height,
+                                int max_steps, void* stream) {
+  return mc_launch_render_uint8_prebasis(voxels, basis, output, batch_size, world_size, width,
+                                         height, max_steps, stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -32,7 +32,7 @@ typedef MC_GLOBAL uint32_t guint32;
 typedef MC_GLOBAL v4f gfloat4;
 // Bytes of world staged in LDS per workgroup: 64 KiB less room for the
 // kernels' static LDS (palette, frames, u/v tables, work-item bookkeeping).
-constexpr int kMaxLdsWorld = 65536 - 3072;
+constexpr int kMaxLdsWorld = 65536 - 4096;
 
 // Block ids (src/kernels.cu:11-24).
 constexpr int8_t kAir = -1, kGrass = 0, kDirt = 1, kOakLog = 3, kLeaves = 4, kBrick = 8,
@@ -60,6 +60,29 @@ __device__ __forceinline__ float palette_entry(int i) {
 }
 constexpr int kPaletteLen = 42;
 constexpr int kSkyBase = 39;
+
+// Flat uint8 colours, 13 blocks then the sky (src/kernels.cu:479-495), packed
+// 0x00BBGGRR.
+__device__ __forceinline__ uint32_t flat_entry(int i) {
+  switch (i) {
+    case 0: return 76u | (166u << 8) | (51u << 16);     // grass
+    case 1: return 140u | (89u << 8) | (51u << 16);     // dirt
+    case 2: return 127u | (127u << 8) | (127u << 16);   // stone
+    case 3: return 140u | (102u << 8) | (64u << 16);    // oak log
+    case 4: return 51u | (140u << 8) | (38u << 16);     // leaves
+    case 5: return 229u | (217u << 8) | (153u << 16);   // sand
+    case 6: return 51u | (102u << 8) | (204u << 16);    // water
+    case 7: return 217u | (229u << 8) | (242u << 16);   // glass
+    case 8: return 178u | (89u << 8) | (76u << 16);     // brick
+    case 9: return 102u | (102u << 8) | (102u << 16);   // cobblestone
+    case 10: return 191u | (153u << 8) | (102u << 16);  // planks
+    case 11: return 242u | (242u << 8) | (250u << 16);  // snow
+    case 12: return 38u | (38u << 8) | (38u << 16);     // bedrock
+    default: return 135u | (206u << 8) | (235u << 16);  // sky
+  }
+}
+constexpr int kFlatLen = 14;
+constexpr int kFlatSky = 13;
 
 // Everything one launch needs; passed by value as the kernel argument.
 struct Params {
@@ -110,6 +133,12 @@ struct Params {
   uint8_t* do_break;
   uint8_t* do_place;
   float* speed_mult_in;
+  // Flat-colour variants: tan of the half field of view with the flat
+  // kernels' degree conversion, and that times the aspect (prebasis scale_u).
+  float flat_hf, flat_su;
+  // [B, 14] camera basis (src/kernels.cu:503-561; NULL: none).  Step items
+  // write it in the prebasis modes; render-basis items read it.
+  float* basis;
 };
 
 // Camera frame of one env, published by wave 0 through LDS.
@@ -132,6 +161,11 @@ struct Frame {
   int inside;
   int ivx, ivy, ivz;
   float dxp, dxn, dyp, dyn, dzp, dzn;
+  // Flat-colour render: >= 0 when the eye voxel is in the world and solid
+  // (every ray stops there on its first lookup); image-plane scales of a
+  // render from a stored camera basis.
+  int flat_fill;
+  float su, sv;
 };
 
 // Largest world dimension of the in-world DDA loop (10-bit packed position).
@@ -279,6 +313,42 @@ __device__ __forceinline__ void stage_world(int8_t* lds, const int8_t* gworld, i
 // first DDA iteration (t = 0, face -1) hits the eye voxel when it is solid; the
 // shaded colour is then col * 1 * (1 - 0) + sky * 0 == col for every pixel.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void finish_frame(const Params& p, Frame& f, const int8_t* world) {
+  f.fill_block = -1;
+  f.hx0 = f.hz0 = INT_MAX;
+  f.hx1 = f.hz1 = f.hy1 = INT_MIN;
+  f.lift = 0;
+  if (p.max_steps >= 1 && p.view_distance > 0.0f && f.ex > 0.0f && f.ex < (float)p.X &&
+      f.ey > 0.0f && f.ey < (float)p.Y && f.ez > 0.0f && f.ez < (float)p.Z) {
+    const int vx = (int)floorf(f.ex), vy = (int)floorf(f.ey), vz = (int)floorf(f.ez);
+    const int8_t blk = world[vy * p.X * p.Z + vx * p.Z + vz];
+    if (blk >= 0 && blk < 13) f.fill_block = blk;
+  }
+  // floor(e) lies in [0, n) exactly when 0 <= e < n (false for NaN).
+  f.inside = f.ex >= 0.0f && f.ex < (float)p.X && f.ey >= 0.0f && f.ey < (float)p.Y &&
+             f.ez >= 0.0f && f.ez < (float)p.Z && p.X <= kMaxInsideDim &&
+             p.Y <= kMaxInsideDim && p.Z <= kMaxInsideDim;
+  f.ivx = f.inside ? (int)floorf(f.ex) : 0;
+  f.ivy = f.inside ? (int)floorf(f.ey) : 0;
+  f.ivz = f.inside ? (int)floorf(f.ez) : 0;
+  f.dxp = (float)(f.ivx + 1) - f.ex;
+  f.dxn = (float)f.ivx - f.ex;
+  f.dyp = (float)(f.ivy + 1) - f.ey;
+  f.dyn = (float)f.ivy - f.ey;
+  f.dzp = (float)(f.ivz + 1) - f.ez;
+  f.dzn = (float)f.ivz - f.ez;
+  // The flat render looks the eye voxel up first, with no slab test and no
+  // view-distance limit (src/kernels.cu:1077-1083).
+  f.flat_fill = -1;
+  if (f.inside && p.max_steps >= 1) {
+    const int8_t blk = world[f.ivy * p.X * p.Z + f.ivx * p.Z + f.ivz];
+    if (blk >= 0) f.flat_fill = blk;
+  }
+  // The solid-extent scan reads whole 16-byte rows of z; an eye inside a
+  // solid needs no ray at all.
+  f.lift = (f.fill_block < 0) && (f.flat_fill < 0) && ((p.Z & 15) == 0);
+}
+
 __device__ __forceinline__ Frame make_frame(const Params& p, float x, float y, float z,
                                             float sin_yaw, float cos_yaw, float sin_pitch,
                                             float cos_pitch, const int8_t* world) {
@@ -294,32 +364,36 @@ __device__ __forceinline__ Frame make_frame(const Params& p, float x, float y, f
   f.ux = -sin_yaw * sin_pitch;
   f.uy = cos_pitch;
   f.uz = -cos_yaw * sin_pitch;
-  f.fill_block = -1;
-  f.hx0 = f.hz0 = INT_MAX;
-  f.hx1 = f.hz1 = f.hy1 = INT_MIN;
-  f.lift = 0;
-  if (p.max_steps >= 1 && p.view_distance > 0.0f && f.ex > 0.0f && f.ex < (float)p.X &&
-      f.ey > 0.0f && f.ey < (float)p.Y && f.ez > 0.0f && f.ez < (float)p.Z) {
-    const int vx = (int)floorf(f.ex), vy = (int)floorf(f.ey), vz = (int)floorf(f.ez);
-    const int8_t blk = world[vy * p.X * p.Z + vx * p.Z + vz];
-    if (blk >= 0 && blk < 13) f.fill_block = blk;
-  }
-  // The solid-extent scan reads whole 16-byte rows of z.
-  f.lift = (f.fill_block < 0) && ((p.Z & 15) == 0);
-  // floor(e) lies in [0, n) exactly when 0 <= e < n (false for NaN).
-  f.inside = f.ex >= 0.0f && f.ex < (float)p.X && f.ey >= 0.0f && f.ey < (float)p.Y &&
-             f.ez >= 0.0f && f.ez < (float)p.Z && p.X <= kMaxInsideDim &&
-             p.Y <= kMaxInsideDim && p.Z <= kMaxInsideDim;
-  f.ivx = f.inside ? (int)floorf(f.ex) : 0;
-  f.ivy = f.inside ? (int)floorf(f.ey) : 0;
-  f.ivz = f.inside ? (int)floorf(f.ez) : 0;
-  f.dxp = (float)(f.ivx + 1) - f.ex;
-  f.dxn = (float)f.ivx - f.ex;
-  f.dyp = (float)(f.ivy + 1) - f.ey;
-  f.dyn = (float)f.ivy - f.ey;
-  f.dzp = (float)(f.ivz + 1) - f.ez;
-  f.dzn = (float)f.ivz - f.ez;
+  f.su = p.flat_su;
+  f.sv = p.flat_hf;
+  finish_frame(p, f, world);
   return f;
+}
+
+// Frame of env b read back from its stored camera basis (the prebasis render
+// input, src/kernels.cu:593-621): eye, forward, right (x, z), up, scale_u,
+// scale_v.
+__device__ __forceinline__ Frame frame_from_basis(const Params& p, int b, const int8_t* world) {
+  const float* bs = p.basis + (int64_t)b * 14;
+  Frame f;
+  f.ex = bs[0]; f.ey = bs[1]; f.ez = bs[2];
+  f.fx = bs[3]; f.fy = bs[4]; f.fz = bs[5];
+  f.rx = bs[6]; f.rz = bs[7];
+  f.ux = bs[8]; f.uy = bs[9]; f.uz = bs[10];
+  f.su = bs[11]; f.sv = bs[12];
+  finish_frame(p, f, world);
+  return f;
+}
+
+// Camera basis row of env b (src/kernels.cu:539-560); element 13 is padding
+// and is not written.
+__device__ __forceinline__ void store_basis(const Params& p, int b, const Frame& f) {
+  float* bs = p.basis + (int64_t)b * 14;
+  bs[0] = f.ex; bs[1] = f.ey; bs[2] = f.ez;
+  bs[3] = f.fx; bs[4] = f.fy; bs[5] = f.fz;
+  bs[6] = f.rx; bs[7] = f.rz;
+  bs[8] = f.ux; bs[9] = f.uy; bs[10] = f.uz;
+  bs[11] = p.flat_su; bs[12] = p.flat_hf;
 }
 
 // Extent of the solid voxels at heights >= y0 (16-byte LDS chunks; Z % 16 ==
@@ -534,28 +608,221 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// One ray of the flat-colour variants (src/kernels.cu:1040-1110, 625-686):
+// no slab test, no view-distance limit, no shading; the DDA starts at the eye
+// voxel and looks up at most max_steps voxels, skipping out-of-bounds ones.
+// The reference normalises with rsqrtf; here the factor is 1 / sqrtf(len^2),
+// both correctly rounded.  Returns the block hit, or -1.
+//
+// Eye voxel in the world: leaving the world ends the ray (as in trace()).
+// Otherwise a ray outside the world that moves away from it on some axis
+// can never come back, which ends it too.
+// ---------------------------------------------------------------------------
+template <bool LIFT>
+__device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, float u, float v,
+                                          const int8_t* __restrict__ world) {
+  float ray_x = f.fx + u * f.rx + v * f.ux;
+  float ray_y = f.fy + v * f.uy;
+  float ray_z = f.fz + u * f.rz + v * f.uz;
+  const float rl = 1.0f / sqrtf(ray_x * ray_x + ray_y * ray_y + ray_z * ray_z);
+  ray_x *= rl;
+  ray_y *= rl;
+  ray_z *= rl;
+  const float inv_x = (ray_x != 0.0f) ? 1.0f / ray_x : 1e30f;
+  const float inv_y = (ray_y != 0.0f) ? 1.0f / ray_y : 1e30f;
+  const float inv_z = (ray_z != 0.0f) ? 1.0f / ray_z : 1e30f;
+  const int max_steps = p.max_steps;
+  const int X = p.X, Y = p.Y, Z = p.Z, XZ = X * Z;
+  const int step_x = (ray_x >= 0) ? 1 : -1;
+  const int step_y = (ray_y >= 0) ? 1 : -1;
+  const int step_z = (ray_z >= 0) ? 1 : -1;
+  const float td_x = fabsf(inv_x), td_y = fabsf(inv_y), td_z = fabsf(inv_z);
+  const int dix = step_x * Z, diy = step_y * XZ, diz = step_z;
+  if (f.inside) {
+    float tm_x = ((ray_x >= 0) ? f.dxp : f.dxn) * inv_x;
+    float tm_y = ((ray_y >= 0) ? f.dyp : f.dyn) * inv_y;
+    float tm_z = ((ray_z >= 0) ? f.dzp : f.dzn) * inv_z;
+    int lim_x = X, lim_y = Y, lim_z = Z;
+    if (LIFT && f.lift && step_y > 0) {
+      lim_y = max(0, min(Y, f.hy1 + 1));
+      lim_x = max(0, (step_x > 0) ? min(X, f.hx1 + 1) : min(X, X - f.hx0));
+      lim_z = max(0, (step_z > 0) ? min(Z, f.hz1 + 1) : min(Z, Z - f.hz0));
+    }
+    const int tx = (step_x > 0) ? f.ivx : X - 1 - f.ivx;
+    const int ty = (step_y > 0) ? f.ivy : Y - 1 - f.ivy;
+    const int tz = (step_z > 0) ? f.ivz : Z - 1 - f.ivz;
+    constexpr uint32_t kGuard = (1u << 9) | (1u << 19) | (1u << 29);
+    uint32_t pos = (uint32_t)(tx + 512 - lim_x) | ((uint32_t)(ty + 512 - lim_y) << 10) |
+                   ((uint32_t)(tz + 512 - lim_z) << 20);
+    int idx = f.ivy * XZ + f.ivx * Z + f.ivz;
+    bool out = (pos & kGuard) != 0u;
+    int blk = (int)world[idx];
+    int step = 0;
+    bool lim = max_steps <= 0;
+    while (!(lim | out | (blk >= 0))) {
+      const bool cx = (tm_x < tm_y) & (tm_x < tm_z);
+      const bool cy = !cx & (tm_y < tm_z);
+      const bool cz = !cx & !cy;
+      tm_x = cx ? tm_x + td_x : tm_x;
+      tm_y = cy ? tm_y + td_y : tm_y;
+      tm_z = cz ? tm_z + td_z : tm_z;
+      idx += cx ? dix : (cy ? diy : diz);
+      pos += cx ? 1u : (cy ? (1u << 10) : (1u << 20));
+      out = (pos & kGuard) != 0u;
+      blk = (int)world[out ? 0 : idx];
+      ++step;
+      lim = step >= max_steps;
+    }
+    return (!lim & !out & (blk >= 0)) ? blk : -1;
+  }
+  int vx = (int)floorf(f.ex), vy = (int)floorf(f.ey), vz = (int)floorf(f.ez);
+  float tm_x = (ray_x >= 0) ? ((float)(vx + 1) - f.ex) * inv_x : ((float)vx - f.ex) * inv_x;
+  float tm_y = (ray_y >= 0) ? ((float)(vy + 1) - f.ey) * inv_y : ((float)vy - f.ey) * inv_y;
+  float tm_z = (ray_z >= 0) ? ((float)(vz + 1) - f.ez) * inv_z : ((float)vz - f.ez) * inv_z;
+  for (int step = 0; step < max_steps; ++step) {
+    const bool inb = (unsigned)vx < (unsigned)X && (unsigned)vy < (unsigned)Y &&
+                     (unsigned)vz < (unsigned)Z;
+    if (inb) {
+      const int blk = (int)world[vy * XZ + vx * Z + vz];
+      if (blk >= 0) return blk;
+    }
+    const bool away = (vx < 0 && step_x < 0) || (vx >= X && step_x > 0) ||
+                      (vy < 0 && step_y < 0) || (vy >= Y && step_y > 0) ||
+                      (vz < 0 && step_z < 0) || (vz >= Z && step_z > 0);
+    if (away) break;
+    if (tm_x < tm_y && tm_x < tm_z) {
+      tm_x += td_x;
+      vx += step_x;
+    } else if (tm_y < tm_z) {
+      tm_y += td_y;
+      vy += step_y;
+    } else {
+      tm_z += td_z;
+      vz += step_z;
+    }
+  }
+  return -1;
+}
+
 __device__ __forceinline__ uint32_t to_u8(float c) { return (uint32_t)(c * 255.0f) & 0xffu; }
 
 // ---------------------------------------------------------------------------
 // Render every pixel of env b into obs (layout [B,H,W,3]).
 // OBS: 0 = uint8, 1 = float32.
 // ---------------------------------------------------------------------------
-// Image-plane coordinates of column px / row py (src/kernels.cu:351-352).
-__device__ __forceinline__ float u_of(const Params& p, int px) {
-  return (2.0f * (px + 0.5f) / p.W - 1.0f) * p.half_fov * p.aspect;
+// Image-plane coordinates of column px / row py, per variant:
+//   kUvShaded   (a * half_fov) * aspect, b * half_fov     (src/kernels.cu:351-352)
+//   kUvMinimal  (a * flat_hf) * aspect,  b * flat_hf      (src/kernels.cu:1040-1041)
+//   kUvBasis    a * scale_u,             b * scale_v      (src/kernels.cu:624-625)
+// with a = 2 (px + 0.5) / W - 1 and b = 1 - 2 (py + 0.5) / H.  kUvRaw keeps
+// a and b, for renders whose scales come per env from a stored basis.
+enum UvMode { kUvShaded = 0, kUvMinimal = 1, kUvBasis = 2, kUvRaw = 3 };
+__device__ __forceinline__ float u_of(const Params& p, int m, int px) {
+  const float a = 2.0f * (px + 0.5f) / p.W - 1.0f;
+  if (m == kUvShaded) return a * p.half_fov * p.aspect;
+  if (m == kUvMinimal) return a * p.flat_hf * p.aspect;
+  if (m == kUvBasis) return a * p.flat_su;
+  return a;
 }
-__device__ __forceinline__ float v_of(const Params& p, int py) {
-  return (1.0f - 2.0f * (py + 0.5f) / p.H) * p.half_fov;
+__device__ __forceinline__ float v_of(const Params& p, int m, int py) {
+  const float c = 1.0f - 2.0f * (py + 0.5f) / p.H;
+  if (m == kUvShaded) return c * p.half_fov;
+  if (m == kUvMinimal || m == kUvBasis) return c * p.flat_hf;
+  return c;
 }
 
 // Per-workgroup tables of u (per column) and v (per row), filled before the
 // first barrier of an item when the image fits them.
 constexpr int kMaxUV = 256;
 __device__ __forceinline__ bool uv_tabled(const Params& p) { return p.W <= kMaxUV && p.H <= kMaxUV; }
-__device__ __forceinline__ void fill_uv(const Params& p, float* su, float* sv, int tid, int nt) {
+__device__ __forceinline__ void fill_uv(const Params& p, int m, float* su, float* sv, int tid,
+                                        int nt) {
   if (!uv_tabled(p)) return;
-  for (int i = tid; i < p.W; i += nt) su[i] = u_of(p, i);
-  for (int i = tid; i < p.H; i += nt) sv[i] = v_of(p, i);
+  for (int i = tid; i < p.W; i += nt) su[i] = u_of(p, m, i);
+  for (int i = tid; i < p.H; i += nt) sv[i] = v_of(p, m, i);
+}
+
+// Constant-colour fill of env b's image with packed 0x00BBGGRR: three 16-byte
+// phases of the repeating RGB pattern (chunk c starts at byte 16c, i.e. at
+// channel phase c mod 3).  Needs 3 * W * H % 16 == 0.
+__device__ __forceinline__ void fill_rgb8(uint8_t* out, int npix, uint32_t rgb, int tid, int nt) {
+  const uint32_t r = rgb & 0xffu, g = (rgb >> 8) & 0xffu, bl = (rgb >> 16) & 0xffu;
+  const uint32_t w0 = r | (g << 8) | (bl << 16) | (r << 24);
+  const uint32_t w1 = g | (bl << 8) | (r << 16) | (g << 24);
+  const uint32_t w2 = bl | (r << 8) | (g << 16) | (bl << 24);
+  guint4* o4 = (guint4*)out;
+  const int d3 = nt % 3;
+  int ph = tid % 3;
+  for (int c = tid; c < (npix * 3) >> 4; c += nt, ph = (ph + d3 >= 3) ? ph + d3 - 3 : ph + d3) {
+    const uint32_t a = (ph == 0) ? w0 : (ph == 1) ? w1 : w2;
+    const uint32_t bq = (ph == 0) ? w1 : (ph == 1) ? w2 : w0;
+    const uint32_t cq = (ph == 0) ? w2 : (ph == 1) ? w0 : w1;
+    o4[c] = v4u{a, bq, cq, a};
+  }
+}
+
+// uint8 image of env b from a per-pixel colour function returning packed
+// 0x00BBGGRR: quads of 4 pixels of one row per lane with three dword
+// stores, or single pixels when W % 4 != 0.
+template <typename ColorFn>
+__device__ __forceinline__ void render_rgb8(const Params& p, int b, int uvm, int tid, int nt,
+                                            const float* su, const float* sv, ColorFn color) {
+  const int W = p.W, H = p.H, npix = W * H;
+  uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
+  const bool tab = uv_tabled(p);
+  if ((W & 3) == 0) {
+    // Quad q covers pixels 4q..4q+3 of one row; (row, quad column) advance
+    // incrementally instead of dividing per quad.
+    const int qpr = W >> 2, drow = nt / qpr, dcol = nt - drow * qpr;
+    int py = tid / qpr, qx = tid - py * qpr;
+    for (int q = tid; q < (npix >> 2); q += nt) {
+      const int px0 = qx << 2, p0 = py * W + px0;
+      const float v = tab ? sv[py] : v_of(p, uvm, py);
+      qx += dcol;
+      py += drow;
+      if (qx >= qpr) { qx -= qpr; ++py; }
+      auto c = [&](int j) -> uint32_t {
+        return color(tab ? su[px0 + j] : u_of(p, uvm, px0 + j), v);
+      };
+      const uint32_t c0 = c(0), c1 = c(1), c2 = c(2), c3 = c(3);
+      guint32* o = (guint32*)(out + (int64_t)p0 * 3);
+      o[0] = c0 | (c1 << 24);
+      o[1] = (c1 >> 8) | (c2 << 16);
+      o[2] = (c2 >> 16) | (c3 << 8);
+    }
+  } else {
+    for (int pi = tid; pi < npix; pi += nt) {
+      const int py = pi / W, px = pi - py * W;
+      const uint32_t c = color(tab ? su[px] : u_of(p, uvm, px), tab ? sv[py] : v_of(p, uvm, py));
+      out[(int64_t)pi * 3 + 0] = (uint8_t)c;
+      out[(int64_t)pi * 3 + 1] = (uint8_t)(c >> 8);
+      out[(int64_t)pi * 3 + 2] = (uint8_t)(c >> 16);
+    }
+  }
+}
+
+// Flat-colour uint8 render of env b (src/kernels.cu:985-1119 minimal,
+// 568-700 prebasis, 835-982 smem: same image as prebasis).  flat: the packed
+// colour table, 13 blocks then the sky; ids >= 13 show the sky.
+template <bool LIFT>
+__device__ __forceinline__ void render_env_flat(const Params& p, int b, const int8_t* world,
+                                                const uint32_t* flat, const Frame& f, int uvm,
+                                                int tid, int nt, const float* su,
+                                                const float* sv) {
+  const int npix = p.W * p.H;
+  uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
+  const bool per_env_scale = (uvm == kUvRaw);
+  auto color = [&](float u, float v) -> uint32_t {
+    if (per_env_scale) { u = u * f.su; v = v * f.sv; }
+    const int h = (f.flat_fill >= 0) ? f.flat_fill : trace_flat<LIFT>(p, f, u, v, world);
+    return flat[(h >= 0 && h < kFlatSky) ? h : kFlatSky];
+  };
+  if (f.flat_fill >= 0 && ((npix * 3) & 15) == 0) {
+    fill_rgb8(out, npix, flat[f.flat_fill < kFlatSky ? f.flat_fill : kFlatSky], tid, nt);
+    return;
+  }
+  render_rgb8(p, b, uvm, tid, nt, su, sv, color);
 }
 
 template <int OBS, bool LIFT>
@@ -564,67 +831,28 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
                                            const float* su, const float* sv) {
   const int W = p.W, H = p.H, npix = W * H;
   const bool tab = uv_tabled(p);
-  auto ucol = [&](int px) { return tab ? su[px] : u_of(p, px); };
-  auto vrow = [&](int py) { return tab ? sv[py] : v_of(p, py); };
+  auto ucol = [&](int px) { return tab ? su[px] : u_of(p, kUvShaded, px); };
+  auto vrow = [&](int py) { return tab ? sv[py] : v_of(p, kUvShaded, py); };
   if (OBS == 0) {
     uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
     if (f.fill_block >= 0 && ((npix * 3) & 15) == 0) {
-      // Constant colour: three 16-byte phases of the repeating RGB pattern.
-      const uint32_t r = to_u8(pal[f.fill_block * 3]), g = to_u8(pal[f.fill_block * 3 + 1]),
-                     bl = to_u8(pal[f.fill_block * 3 + 2]);
-      const uint32_t w0 = r | (g << 8) | (bl << 16) | (r << 24);
-      const uint32_t w1 = g | (bl << 8) | (r << 16) | (g << 24);
-      const uint32_t w2 = bl | (r << 8) | (g << 16) | (bl << 24);
-      // Chunk c starts at byte 16c, i.e. at channel phase (16c mod 3) = c mod 3.
-      guint4* o4 = (guint4*)out;
-      const int d3 = nt % 3;
-      int ph = tid % 3;
-      for (int c = tid; c < (npix * 3) >> 4; c += nt, ph = (ph + d3 >= 3) ? ph + d3 - 3 : ph + d3) {
-        const uint32_t a = (ph == 0) ? w0 : (ph == 1) ? w1 : w2;
-        const uint32_t bq = (ph == 0) ? w1 : (ph == 1) ? w2 : w0;
-        const uint32_t cq = (ph == 0) ? w2 : (ph == 1) ? w0 : w1;
-        o4[c] = v4u{a, bq, cq, a};
-      }
+      fill_rgb8(out, npix,
+                to_u8(pal[f.fill_block * 3]) | (to_u8(pal[f.fill_block * 3 + 1]) << 8) |
+                    (to_u8(pal[f.fill_block * 3 + 2]) << 16),
+                tid, nt);
       return;
     }
-    if ((W & 3) == 0) {
-      // Quad q covers pixels 4q..4q+3 of one row; (row, quad column) advance
-      // incrementally instead of dividing per quad.
-      const int qpr = W >> 2, drow = nt / qpr, dcol = nt - drow * qpr;
-      int py = tid / qpr, qx = tid - py * qpr;
-      for (int q = tid; q < (npix >> 2); q += nt) {
-        const int px0 = qx << 2, p0 = py * W + px0;
-        const float v = vrow(py);
-        qx += dcol;
-        py += drow;
-        if (qx >= qpr) { qx -= qpr; ++py; }
-        // Packed 0x00BBGGRR per pixel.
-        auto shade = [&](int j) -> uint32_t {
-          const float u = ucol(px0 + j);
-          float rgb[3];
-          trace<LIFT>(p, f, u, v, world, pal, rgb);
-          return to_u8(rgb[0]) | (to_u8(rgb[1]) << 8) | (to_u8(rgb[2]) << 16);
-        };
-        const uint32_t c0 = shade(0), c1 = shade(1), c2 = shade(2), c3 = shade(3);
-        guint32* o = (guint32*)(out + (int64_t)p0 * 3);
-        o[0] = c0 | (c1 << 24);
-        o[1] = (c1 >> 8) | (c2 << 16);
-        o[2] = (c2 >> 16) | (c3 << 8);
+    // Packed 0x00BBGGRR per pixel.
+    auto color = [&](float u, float v) -> uint32_t {
+      float rgb[3];
+      if (f.fill_block >= 0) {
+        for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
+      } else {
+        trace<LIFT>(p, f, u, v, world, pal, rgb);
       }
-    } else {
-      for (int pi = tid; pi < npix; pi += nt) {
-        const int py = pi / W, px = pi - py * W;
-        const float v = vrow(py);
-        const float u = ucol(px);
-        float rgb[3];
-        if (f.fill_block >= 0) {
-          for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
-        } else {
-          trace<LIFT>(p, f, u, v, world, pal, rgb);
-        }
-        for (int c = 0; c < 3; ++c) out[(int64_t)pi * 3 + c] = (uint8_t)to_u8(rgb[c]);
-      }
-    }
+      return to_u8(rgb[0]) | (to_u8(rgb[1]) << 8) | (to_u8(rgb[2]) << 16);
+    };
+    render_rgb8(p, b, kUvShaded, tid, nt, su, sv, color);
   } else {
     float* out = reinterpret_cast<float*>(p.obs) + (int64_t)b * npix * 3;
     if ((W & 3) == 0) {
@@ -876,8 +1104,9 @@ __device__ __forceinline__ Frame env_update(const Params& p, int b, int8_t* worl
   return make_frame(p, nx, ny, nz, sin_yaw, cos_yaw, sin_pitch, cos_pitch, world);
 }
 
-__device__ __forceinline__ void load_palette(float* pal, int tid) {
+__device__ __forceinline__ void load_palette(float* pal, uint32_t* flat, int tid) {
   if (tid < kPaletteLen) pal[tid] = palette_entry(tid);
+  if (tid < kFlatLen) flat[tid] = flat_entry(tid);
 }
 
 // Wave-0 logic of MineEnv.reset for env b (minecu/__init__.py:270-302):
@@ -953,7 +1182,20 @@ __device__ __forceinline__ Frame camera_update(const Params& p, int b, const int
 // The envs of an item share the workgroup barriers; each has its own LDS world
 // slot (world_stride bytes apart) and camera frame.
 // ---------------------------------------------------------------------------
-enum ItemKind { kItemStep = 0, kItemResetRender = 1, kItemRenderCameras = 2, kItemRenderState = 3 };
+// Work-item kinds: a step, the reset render, a render from the [B, 5]
+// cameras, a render from the agent state, a render from the [B, 14] basis.
+enum ItemKind {
+  kItemStep = 0,
+  kItemResetRender = 1,
+  kItemRenderCameras = 2,
+  kItemRenderState = 3,
+  kItemRenderBasis = 4
+};
+// Observation modes: shaded uint8, shaded float32, and the flat-colour uint8
+// variants with the minimal kernel's and the prebasis kernel's image plane.
+// Steps render the variant; the reset and state renders of the flat modes
+// are shaded uint8, as the reference's _render() is (minecu/__init__.py:446-466).
+enum ObsMode { kObsU8 = 0, kObsF32 = 1, kObsU8Minimal = 2, kObsU8Prebasis = 3 };
 
 __host__ __device__ __forceinline__ int world_stride_bytes(int X, int Y, int Z) {
   return (X * Y * Z + 15) & ~15;
@@ -961,8 +1203,8 @@ __host__ __device__ __forceinline__ int world_stride_bytes(int X, int Y, int Z) 
 
 template <int NT, int KIND, bool LIFT>
 __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, int wpe,
-                                         int8_t* lds, const float* pal, Frame* frames,
-                                         float* su, float* sv) {
+                                         int8_t* lds, const float* pal, const uint32_t* flat,
+                                         Frame* frames, float* su, float* sv) {
   const int tid = threadIdx.x, wave = tid / kWave;
   const int epw = (NT / kWave) / wpe;
   const int e = wave / wpe;                                 // env slot within the item
@@ -981,7 +1223,14 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   const bool regen = (KIND == kItemResetRender) && (p.ring[3] != 0u);
   EnvIn in;
   if (KIND == kItemStep && active && et < kWave) in = load_env_in(p, b);
-  fill_uv(p, su, sv, tid, NT);
+  // Flat images for steps and camera renders of the flat modes; shaded
+  // uint8 for their reset and state renders.
+  const bool flat_kind = (KIND == kItemStep || KIND == kItemRenderCameras);
+  const int uvm = (KIND == kItemRenderBasis)                         ? kUvRaw
+                  : (flat_kind && obs_mode == kObsU8Minimal)  ? kUvMinimal
+                  : (flat_kind && obs_mode == kObsU8Prebasis) ? kUvBasis
+                                                              : kUvShaded;
+  fill_uv(p, uvm, su, sv, tid, NT);
   if (active) {
     if (regen) store_fresh_world(p, gworld, world, nvox, et, ent);
     else stage_world(world, gworld, nvox, et, ent);
@@ -991,8 +1240,12 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     Frame f;
     if (KIND == kItemStep) f = env_update(p, b, world, gworld, any_done, in);
     else if (KIND == kItemResetRender) f = reset_update(p, b, world);
+    else if (KIND == kItemRenderBasis) f = frame_from_basis(p, b, world);
     else f = camera_update(p, b, world, KIND == kItemRenderState);
-    if (et == 0) frames[e] = f;
+    if (et == 0) {
+      frames[e] = f;
+      if (KIND == kItemStep && p.basis) store_basis(p, b, f);
+    }
   }
   __syncthreads();
   if (LIFT) {
@@ -1002,8 +1255,9 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   }
   if (active) {
     const Frame f = uniform_frame(&frames[e]);
-    if (obs_mode == 0) render_env<0, LIFT>(p, b, world, pal, f, et, ent, su, sv);
-    else render_env<1, LIFT>(p, b, world, pal, f, et, ent, su, sv);
+    if (uvm != kUvShaded) render_env_flat<LIFT>(p, b, world, flat, f, uvm, et, ent, su, sv);
+    else if (obs_mode == kObsF32) render_env<1, LIFT>(p, b, world, pal, f, et, ent, su, sv);
+    else render_env<0, LIFT>(p, b, world, pal, f, et, ent, su, sv);
     if (KIND == kItemStep && any_done) store_fresh_world(p, gworld, nullptr, nvox, et, ent);
   }
 }
@@ -1017,11 +1271,13 @@ template <int NT, int KIND, bool LIFT>
 __global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
+  __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
   __shared__ float s_u[kMaxUV], s_v[kMaxUV];
-  load_palette(s_pal, threadIdx.x);
+  load_palette(s_pal, s_flat, threadIdx.x);
   const int epw = (NT / kWave) / wpe;
-  run_item<NT, KIND, LIFT>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_frames, s_u, s_v);
+  run_item<NT, KIND, LIFT>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_flat, s_frames,
+                           s_u, s_v);
 }
 
 // Several env groups (uneven batch sizes, world sizes, resolutions) in one
@@ -1056,11 +1312,12 @@ template <int NT, bool LIFT>
 __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
+  __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
   __shared__ int s_begin[kMaxGroups];
   __shared__ int s_item;
   __shared__ float s_u[kMaxUV], s_v[kMaxUV];
-  load_palette(s_pal, threadIdx.x);
+  load_palette(s_pal, s_flat, threadIdx.x);
   if (threadIdx.x < a.n_groups) s_begin[threadIdx.x] = a.entries[threadIdx.x].item_begin;
   int item = blockIdx.x;
   for (;;) {
@@ -1092,7 +1349,7 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
     const int wpe = me.wpe;
     const int epw = (NT / kWave) / wpe;
     run_item<NT, kItemStep, LIFT>(p, me.obs_mode, (item - s_begin[g]) * epw, wpe, s_lds, s_pal,
-                                  s_frames, s_u, s_v);
+                                  s_flat, s_frames, s_u, s_v);
     if (!a.persistent) break;
   }
   if (a.persistent && threadIdx.x == 0) {
@@ -1196,6 +1453,30 @@ __global__ void k_reset(float* pos, float* vel, float* yaw, float* pitch, uint8_
   pos[i * 3 + 0] = sx; pos[i * 3 + 1] = sy; pos[i * 3 + 2] = sz;
   vel[i * 3 + 0] = 0.0f; vel[i * 3 + 1] = 0.0f; vel[i * 3 + 2] = 0.0f;
   yaw[i] = 0.0f; pitch[i] = 0.0f; og[i] = 0;
+}
+
+// Camera basis of every env from its [B, 5] camera (src/kernels.cu:511-561),
+// one thread per env.
+__global__ void k_camera_basis(Params p) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= p.B) return;
+  const float* cam = p.cameras + (int64_t)b * 5;
+  float sin_yaw, cos_yaw, sin_pitch, cos_pitch;
+  mc_sincosf(cam[3], &sin_yaw, &cos_yaw);
+  mc_sincosf(cam[4], &sin_pitch, &cos_pitch);
+  Frame f;
+  f.ex = cam[0];
+  f.ey = cam[1] + 1.6f;
+  f.ez = cam[2];
+  f.fx = sin_yaw * cos_pitch;
+  f.fy = sin_pitch;
+  f.fz = cos_yaw * cos_pitch;
+  f.rx = cos_yaw;
+  f.rz = -sin_yaw;
+  f.ux = -sin_yaw * sin_pitch;
+  f.uy = cos_pitch;
+  f.uz = -cos_yaw * sin_pitch;
+  store_basis(p, b, f);
 }
 
 __global__ void k_debug_sincos(const float* x, float* s, float* c, int n) {

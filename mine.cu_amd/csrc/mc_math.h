@@ -181,3 +181,10 @@ MC_HD float mc_half_fov_tan(float fov_degrees) {
   const float fov_rad = fov_degrees * 3.14159265f / 180.0f;
   return mc_tanf(fov_rad * 0.5f);
 }
+
+// The flat-colour render variants convert with a single constant instead
+// (src/kernels.cu:536-537, 1036-1038: fov * 0.01745329f, then * 0.5f).
+MC_HD float mc_half_fov_tan_flat(float fov_degrees) {
+  const float fov_rad = fov_degrees * 0.01745329f;
+  return mc_tanf(fov_rad * 0.5f);
+}

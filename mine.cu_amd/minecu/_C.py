@@ -5,9 +5,9 @@ The reference's bindings check device and dtype only (e.g.
 bindings.cpp:537-543); these also check shapes and contiguity, so a mismatch
 raises RuntimeError instead of reading out of bounds on the device.  Every
 launch goes to torch's current stream (the reference uses the legacy default
-stream).  Not implemented: the reference's experimental render variants
-(coalesced/fp16/fast/minimal/prebasis/smem bindings), which MineEnv's shaded
-modes do not use.
+stream).  Not implemented: the half-precision DDA step (step_uint8_fp16),
+the ground-accelerated step_uint8_fast and the [Y,X,Z,B]-layout
+render_uint8_coalesced, none of which MineEnv calls in the modes built here.
 """
 from __future__ import annotations
 
@@ -114,12 +114,20 @@ def _step(fn_name, obs_dtype, voxels, positions, velocities, yaws, pitches, on_g
           look, do_reset, forward_in, strafe_in, delta_yaw_in, delta_pitch_in, jump_in, do_break,
           do_place, speed_mult, cameras, obs_buffer, rewards, width, height, max_steps,
           view_distance, fov_degrees, dt, gravity, walk_speed, sprint_mult, jump_vel,
-          target_block, reward_value, spawn_x, spawn_y, spawn_z):
+          target_block, reward_value, spawn_x, spawn_y, spawn_z, basis=None, step_count=None,
+          episode_length=None, per_axis=False):
+    """Checks every tensor, then calls the C launcher.  basis / step_count /
+    episode_length: the prebasis variants' extra arguments; per_axis: the
+    launcher takes world_x, world_y, world_z (else the cubic world_size)."""
     _need(voxels, "voxels", torch.int8)
     B, Y, X, Z = voxels.shape
-    if not (X == Y == Z):
+    if not per_axis and not (X == Y == Z):
         raise RuntimeError("this step binding takes a cubic world (world_size = voxels.size(1))")
     dev = voxels.device
+    if basis is not None:
+        _need(basis, "basis", torch.float32, (B, 14), dev)
+    if step_count is not None:
+        _need(step_count, "step_count", torch.int32, (B,), dev)
     for t, n, d, s in ((positions, "positions", torch.float32, (B, 3)),
                        (velocities, "velocities", torch.float32, (B, 3)),
                        (yaws, "yaws", torch.float32, (B,)), (pitches, "pitches", torch.float32, (B,)),
@@ -138,13 +146,19 @@ def _step(fn_name, obs_dtype, voxels, positions, velocities, yaws, pitches, on_g
                        (rewards, "rewards", torch.float32, (B,))):
         _need(t, n, d, s, dev)
     fn = getattr(_lib.load(), fn_name)
-    _lib.check(fn(*(_ptr(t) for t in (voxels, positions, velocities, yaws, pitches, on_ground,
-                                      buttons, look, do_reset, forward_in, strafe_in,
-                                      delta_yaw_in, delta_pitch_in, jump_in, do_break, do_place,
-                                      speed_mult, cameras, obs_buffer, rewards)),
-                  B, Y, width, height, max_steps, view_distance, fov_degrees, dt, gravity,
-                  walk_speed, sprint_mult, jump_vel, int(target_block), reward_value, spawn_x,
-                  spawn_y, spawn_z, _stream(voxels)), fn_name)
+    ptrs = [voxels, positions, velocities, yaws, pitches, on_ground, buttons, look, do_reset]
+    if step_count is not None:
+        ptrs.append(step_count)
+    ptrs += [forward_in, strafe_in, delta_yaw_in, delta_pitch_in, jump_in, do_break, do_place,
+             speed_mult, cameras]
+    if basis is not None:
+        ptrs.append(basis)
+    ptrs += [obs_buffer, rewards]
+    dims = (B, X, Y, Z) if per_axis else (B, Y)
+    tail = [] if episode_length is None else [int(episode_length)]
+    _lib.check(fn(*(_ptr(t) for t in ptrs), *dims, width, height, max_steps, view_distance,
+                  fov_degrees, dt, gravity, walk_speed, sprint_mult, jump_vel, int(target_block),
+                  reward_value, spawn_x, spawn_y, spawn_z, *tail, _stream(voxels)), fn_name)
 
 
 def step_uint8(voxels, positions, velocities, yaws, pitches, on_ground, buttons, look, do_reset,
@@ -173,3 +187,100 @@ def step(voxels, positions, velocities, yaws, pitches, on_ground, buttons, look,
           delta_pitch_in, jump_in, do_break, do_place, speed_mult, cameras, obs_buffer, rewards,
           width, height, max_steps, view_distance, fov_degrees, dt, gravity, walk_speed,
           sprint_mult, jump_vel, target_block, reward_value, spawn_x, spawn_y, spawn_z)
+
+
+
+def step_uint8_minimal(voxels, positions, velocities, yaws, pitches, on_ground, buttons, look,
+                       do_reset, forward_in, strafe_in, delta_yaw_in, delta_pitch_in, jump_in,
+                       do_break, do_place, speed_mult, cameras, obs_buffer, rewards, width, height,
+                       max_steps=64, view_distance=32.0, fov_degrees=70.0, dt=0.05, gravity=-20.0,
+                       walk_speed=4.0, sprint_mult=1.5, jump_vel=8.0, target_block=-1,
+                       reward_value=1.0, spawn_x=16.0, spawn_y=9.0, spawn_z=16.0) -> None:
+    """bindings.cpp:632-682 -> launch_step_uint8_minimal (flat colours)."""
+    _step("mc_launch_step_uint8_minimal", torch.uint8, voxels, positions, velocities, yaws,
+          pitches, on_ground, buttons, look, do_reset, forward_in, strafe_in, delta_yaw_in,
+          delta_pitch_in, jump_in, do_break, do_place, speed_mult, cameras, obs_buffer, rewards,
+          width, height, max_steps, view_distance, fov_degrees, dt, gravity, walk_speed,
+          sprint_mult, jump_vel, target_block, reward_value, spawn_x, spawn_y, spawn_z)
+
+
+def step_uint8_prebasis(voxels, positions, velocities, yaws, pitches, on_ground, buttons, look,
+                        do_reset, forward_in, strafe_in, delta_yaw_in, delta_pitch_in, jump_in,
+                        do_break, do_place, speed_mult, cameras, basis, obs_buffer, rewards, width,
+                        height, max_steps=64, view_distance=32.0, fov_degrees=70.0, dt=0.05,
+                        gravity=-20.0, walk_speed=4.0, sprint_mult=1.5, jump_vel=8.0,
+                        target_block=-1, reward_value=1.0, spawn_x=16.0, spawn_y=9.0,
+                        spawn_z=16.0) -> None:
+    """bindings.cpp:736-792 -> launch_step_uint8_prebasis (writes basis [B,14])."""
+    _step("mc_launch_step_uint8_prebasis", torch.uint8, voxels, positions, velocities, yaws,
+          pitches, on_ground, buttons, look, do_reset, forward_in, strafe_in, delta_yaw_in,
+          delta_pitch_in, jump_in, do_break, do_place, speed_mult, cameras, obs_buffer, rewards,
+          width, height, max_steps, view_distance, fov_degrees, dt, gravity, walk_speed,
+          sprint_mult, jump_vel, target_block, reward_value, spawn_x, spawn_y, spawn_z,
+          basis=basis, per_axis=True)
+
+
+def step_uint8_prebasis_full(voxels, positions, velocities, yaws, pitches, on_ground, buttons,
+                             look, do_reset, step_count, forward_in, strafe_in, delta_yaw_in,
+                             delta_pitch_in, jump_in, do_break, do_place, speed_mult, cameras,
+                             basis, obs_buffer, rewards, width, height, max_steps=64,
+                             view_distance=32.0, fov_degrees=70.0, dt=0.05, gravity=-20.0,
+                             walk_speed=4.0, sprint_mult=1.5, jump_vel=8.0, target_block=-1,
+                             reward_value=1.0, spawn_x=16.0, spawn_y=9.0, spawn_z=16.0,
+                             episode_length=0) -> None:
+    """bindings.cpp:794-875 -> launch_step_uint8_prebasis_full: do_reset =
+    step_count >= episode_length (0: never) and the step_count update happen
+    in the kernel; world regeneration stays with the caller."""
+    _step("mc_launch_step_uint8_prebasis_full", torch.uint8, voxels, positions, velocities, yaws,
+          pitches, on_ground, buttons, look, do_reset, forward_in, strafe_in, delta_yaw_in,
+          delta_pitch_in, jump_in, do_break, do_place, speed_mult, cameras, obs_buffer, rewards,
+          width, height, max_steps, view_distance, fov_degrees, dt, gravity, walk_speed,
+          sprint_mult, jump_vel, target_block, reward_value, spawn_x, spawn_y, spawn_z,
+          basis=basis, step_count=step_count, episode_length=episode_length, per_axis=True)
+
+
+def step_uint8_smem(voxels, positions, velocities, yaws, pitches, on_ground, buttons, look,
+                    do_reset, forward_in, strafe_in, delta_yaw_in, delta_pitch_in, jump_in,
+                    do_break, do_place, speed_mult, cameras, basis, obs_buffer, rewards, width,
+                    height, max_steps=64, view_distance=32.0, fov_degrees=70.0, dt=0.05,
+                    gravity=-20.0, walk_speed=4.0, sprint_mult=1.5, jump_vel=8.0,
+                    target_block=-1, reward_value=1.0, spawn_x=16.0, spawn_y=9.0,
+                    spawn_z=16.0) -> None:
+    """bindings.cpp:930-981 -> launch_step_uint8_smem (cubic world; the image
+    equals the prebasis step's)."""
+    _step("mc_launch_step_uint8_smem", torch.uint8, voxels, positions, velocities, yaws,
+          pitches, on_ground, buttons, look, do_reset, forward_in, strafe_in, delta_yaw_in,
+          delta_pitch_in, jump_in, do_break, do_place, speed_mult, cameras, obs_buffer, rewards,
+          width, height, max_steps, view_distance, fov_degrees, dt, gravity, walk_speed,
+          sprint_mult, jump_vel, target_block, reward_value, spawn_x, spawn_y, spawn_z,
+          basis=basis)
+
+
+def precompute_camera_basis(cameras, basis, width, height, fov_degrees=70.0) -> None:
+    """bindings.cpp:857-875 -> launch_precompute_camera_basis."""
+    _need(cameras, "cameras", torch.float32)
+    if cameras.dim() != 2 or cameras.shape[1] != 5:
+        raise RuntimeError(f"cameras must have shape [B, 5] (got {tuple(cameras.shape)})")
+    B = cameras.shape[0]
+    _need(basis, "basis", torch.float32, (B, 14), cameras.device)
+    _lib.check(_lib.load().mc_launch_precompute_camera_basis(
+        _ptr(cameras), _ptr(basis), B, width, height, fov_degrees, _stream(cameras)),
+        "precompute_camera_basis")
+
+
+def render_uint8_prebasis(voxels, basis, output, max_steps=64) -> None:
+    """bindings.cpp:877-899 -> launch_render_uint8_prebasis (cubic world_size =
+    voxels.size(1); flat colours from a stored basis)."""
+    _need(voxels, "voxels", torch.int8)
+    B, Y, X, Z = voxels.shape
+    if not (X == Y == Z):
+        raise RuntimeError("render_uint8_prebasis takes a cubic world (world_size = voxels.size(1))")
+    dev = voxels.device
+    _need(basis, "basis", torch.float32, (B, 14), dev)
+    _need(output, "output", torch.uint8, None, dev)
+    if output.dim() != 4 or output.shape[0] != B or output.shape[3] != 3:
+        raise RuntimeError(f"output must have shape [B, H, W, 3] (got {tuple(output.shape)})")
+    H, W = output.shape[1], output.shape[2]
+    _lib.check(_lib.load().mc_launch_render_uint8_prebasis(
+        _ptr(voxels), _ptr(basis), _ptr(output), B, Y, W, H, max_steps, _stream(voxels)),
+        "render_uint8_prebasis")

@@ -57,8 +57,16 @@ class BlockType:
 
 
 # obs_dtype strings the reference accepts (minecu/__init__.py:167) and the
-# encoding each maps to here.  The flat-colour variants are not built yet.
-_OBS_MODES = {"uint8": _lib.MC_OBS_U8, "float32": _lib.MC_OBS_F32}
+# encoding each maps to here.  "uint8_smem" renders the same image as
+# "uint8_prebasis" (src/kernels.cu:835-982 only moves the world to shared
+# memory, which every mode here does); "uint8_prebasis_full" adds its episode
+# convention (0 = no limit).
+_OBS_MODES = {"uint8": _lib.MC_OBS_U8, "float32": _lib.MC_OBS_F32,
+              "uint8_minimal": _lib.MC_OBS_U8_MINIMAL,
+              "uint8_prebasis": _lib.MC_OBS_U8_PREBASIS,
+              "uint8_prebasis_full": _lib.MC_OBS_U8_PREBASIS,
+              "uint8_smem": _lib.MC_OBS_U8_PREBASIS}
+_BASIS_OBS = ("uint8_prebasis", "uint8_prebasis_full", "uint8_smem")
 _REFERENCE_OBS = ("float32", "uint8", "uint8_minimal", "uint8_fp16", "uint8_prebasis",
                   "uint8_prebasis_full", "uint8_smem")
 
@@ -143,8 +151,7 @@ class MineEnv:
             "obs_dtype must be one of " + ", ".join(repr(o) for o in _REFERENCE_OBS)
         if obs_dtype not in _OBS_MODES:
             raise NotImplementedError(
-                f"obs_dtype={obs_dtype!r} (flat-colour render variant) is not implemented; "
-                "use 'uint8' or 'float32'")
+                f"obs_dtype={obs_dtype!r} (half-precision DDA variant) is not implemented")
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -206,7 +213,9 @@ class MineEnv:
         obs_t = torch.uint8 if self.obs_dtype != "float32" else torch.float32
         self.obs_buffer = torch.zeros((B, H, W, 3), dtype=obs_t, device=dev)
         self.cameras = torch.zeros((B, 5), dtype=torch.float32, device=dev)
-        self.basis = None
+        # Camera basis of the prebasis variants (minecu/__init__.py:237-241).
+        self.basis = (torch.zeros((B, 14), dtype=torch.float32, device=dev)
+                      if self.obs_dtype in _BASIS_OBS else None)
         self.rewards = torch.zeros((B,), dtype=torch.float32, device=dev)
         self.step_count = torch.zeros((B,), dtype=torch.int32, device=dev)
         self.do_reset = torch.zeros((B,), dtype=torch.bool, device=dev)
@@ -228,6 +237,10 @@ class MineEnv:
         c.seed = self.seed & 0xFFFFFFFF
         c.tree_x, c.tree_z, c.house_x, c.house_z = self.tree_x, self.tree_z, self.house_x, self.house_z
         c.episode_length = -1 if self.episode_length is None else int(self.episode_length)
+        if self.obs_dtype == "uint8_prebasis_full" and c.episode_length <= 0:
+            # Its C++ episode check reads 0 -- what it passes for None -- as no
+            # limit (minecu/__init__.py:349, src/kernels.cu:1954-1968).
+            c.episode_length = -1
         c.look_delta = self.look_delta
         c.obs_mode = _OBS_MODES[self.obs_dtype]
         s = _lib.McState()
@@ -238,6 +251,7 @@ class MineEnv:
                         ("cameras", self.cameras), ("obs", self.obs_buffer),
                         ("rewards", self.rewards)):
             setattr(s, name, t.data_ptr())
+        s.basis = self.basis.data_ptr() if self.basis is not None else None
         return c, s
 
     def _stream(self) -> int:
@@ -334,13 +348,16 @@ class MineEnv:
     _STATE_KEYS = ("voxels", "positions", "velocities", "yaws", "pitches", "on_ground",
                    "step_count", "do_reset", "cameras", "rewards", "obs_buffer")
 
+    def _state_keys(self):
+        return self._STATE_KEYS + (("basis",) if self.basis is not None else ())
+
     def state_dict(self) -> Dict[str, torch.Tensor]:
         """Copies of every state tensor; a step is a pure function of this
         state and the actions, so load_state_dict() resumes bit-exactly."""
-        return {k: getattr(self, k).clone() for k in self._STATE_KEYS}
+        return {k: getattr(self, k).clone() for k in self._state_keys()}
 
     def load_state_dict(self, sd: Dict[str, torch.Tensor]) -> None:
-        for k in self._STATE_KEYS:
+        for k in self._state_keys():
             dst = getattr(self, k)
             src = sd[k]
             if tuple(src.shape) != tuple(dst.shape) or src.dtype != dst.dtype:

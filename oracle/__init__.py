@@ -65,6 +65,10 @@ def lib():
         L.orc_raycast_interact.argtypes = [P, P, P, P, P, P, P, i, i, i, i, i, f]
         L.orc_update_cameras.argtypes = [P, P, P, P, i]
         L.orc_render.argtypes = [P, P, P, i, i, i, i, i, i, i, i, f, f, P, i]
+        L.orc_half_fov_tan_flat.argtypes = [f]
+        L.orc_half_fov_tan_flat.restype = f
+        L.orc_camera_basis.argtypes = [P, P, i, i, i, f]
+        L.orc_render_flat.argtypes = [P, P, P, i, i, i, i, i, i, i, i, f, P, i]
         _lib = L
     return _lib
 
@@ -118,6 +122,35 @@ def render(voxels, cameras, width, height, max_steps=64, view_distance=32.0,
     return (out, hits) if want_hits else out
 
 
+def half_fov_tan_flat(fov_degrees: float) -> float:
+    return float(lib().orc_half_fov_tan_flat(fov_degrees))
+
+
+def camera_basis(cameras, width, height, fov_degrees=70.0):
+    """[B,14] camera basis (precompute_camera_basis_kernel, src/kernels.cu:511-561);
+    column 13 stays zero."""
+    cameras = np.ascontiguousarray(cameras, dtype=np.float32)
+    B = cameras.shape[0]
+    basis = np.zeros((B, 14), np.float32)
+    lib().orc_camera_basis(_p(cameras), _p(basis), B, width, height, fov_degrees)
+    return basis
+
+
+def render_flat(voxels, cams, width, height, mode, max_steps=64, fov_degrees=70.0,
+                want_hits=False, nthreads=None):
+    """Flat-colour uint8 render.  mode "minimal": cams are [B,5] cameras
+    (src/kernels.cu:985-1119); mode "prebasis": cams are a [B,14] basis
+    (src/kernels.cu:568-700)."""
+    B, Y, X, Z = voxels.shape
+    cams = np.ascontiguousarray(cams, dtype=np.float32)
+    out = np.empty((B, height, width, 3), dtype=np.uint8)
+    hits = np.empty((B, height, width), dtype=np.int32) if want_hits else None
+    lib().orc_render_flat(_p(voxels), _p(cams), _p(out), 0 if mode == "minimal" else 1, B, X, Y,
+                          Z, width, height, max_steps, fov_degrees, _p(hits),
+                          nthreads or default_threads())
+    return (out, hits) if want_hits else out
+
+
 def discrete_to_buttons(actions: np.ndarray, look_delta: float):
     actions = np.ascontiguousarray(actions, dtype=np.int64)
     B = actions.shape[0]
@@ -131,9 +164,17 @@ def discrete_to_buttons(actions: np.ndarray, look_delta: float):
 # MineEnv orchestration restated (minecu/__init__.py)
 # ---------------------------------------------------------------------------
 
+# obs_dtype -> flat render mode of the step (None: shaded).
+_FLAT = {"uint8": None, "float32": None, "uint8_minimal": "minimal",
+         "uint8_prebasis": "prebasis", "uint8_prebasis_full": "prebasis",
+         "uint8_smem": "prebasis"}
+
+
 class OracleEnv:
-    """numpy restatement of ``MineEnv`` for the shaded ``uint8``/``float32``
-    observation modes (minecu/__init__.py:141-466)."""
+    """numpy restatement of ``MineEnv`` (minecu/__init__.py:141-466) for the
+    shaded ``uint8``/``float32`` modes and the flat-colour uint8 variants:
+    their steps render flat colours (the prebasis variants from the [B,14]
+    ``basis`` they update), their reset/get_obs renders stay shaded uint8."""
 
     def __init__(self, batch_size=4096, world_size=32, world_x=None, world_y=None, world_z=None,
                  render_width=64, render_height=64, max_steps=64, view_distance=32.0,
@@ -141,7 +182,7 @@ class OracleEnv:
                  jump_vel=8.0, ground_height=8, target_block=OAKLOG, reward_value=1.0,
                  episode_length=None, seed=42, obs_dtype="uint8", look_delta=0.1,
                  nthreads=None):
-        assert obs_dtype in ("uint8", "float32")
+        assert obs_dtype in _FLAT, obs_dtype
         self.batch_size = B = batch_size
         self.world_x = world_x if world_x is not None else world_size
         self.world_y = world_y if world_y is not None else world_size
@@ -168,7 +209,9 @@ class OracleEnv:
         self.pitches = np.zeros(B, np.float32)
         self.on_ground = np.ones(B, np.uint8)
         self.obs_buffer = np.zeros((B, render_height, render_width, 3),
-                                   np.uint8 if obs_dtype == "uint8" else np.float32)
+                                   np.float32 if obs_dtype == "float32" else np.uint8)
+        self.flat = _FLAT[obs_dtype]
+        self.basis = np.zeros((B, 14), np.float32) if self.flat == "prebasis" else None
         self.cameras = np.zeros((B, 5), np.float32)
         self.rewards = np.zeros(B, np.float32)
         self.step_count = np.zeros(B, np.int32)
@@ -190,7 +233,7 @@ class OracleEnv:
                              self.batch_size)
         r = render(self.voxels, self.cameras, self.render_width, self.render_height,
                    self.max_steps, self.view_distance, self.fov_degrees,
-                   out_u8=self.obs_dtype == "uint8", want_hits=want_hits, nthreads=self.nthreads)
+                   out_u8=self.obs_dtype != "float32", want_hits=want_hits, nthreads=self.nthreads)
         if want_hits:
             self.obs_buffer[...], self.hits = r
         else:
@@ -218,7 +261,12 @@ class OracleEnv:
             buttons, look = discrete_to_buttons(np.asarray(buttons), self.look_delta)
         buttons = np.ascontiguousarray(buttons, dtype=np.int8)
         look = np.ascontiguousarray(look, dtype=np.float32)
-        if self.episode_length is not None:
+        if self.obs_dtype == "uint8_prebasis_full":
+            # episode_check_kernel: 0 (the reference's stand-in for None) is no
+            # limit (src/kernels.cu:1954-1968, minecu/__init__.py:349).
+            L_ = self.episode_length or 0
+            self.do_reset = ((self.step_count >= L_) & (L_ > 0)).astype(np.uint8)
+        elif self.episode_length is not None:
             self.do_reset = (self.step_count >= self.episode_length).astype(np.uint8)
         else:
             self.do_reset[:] = 0
@@ -238,7 +286,23 @@ class OracleEnv:
         L.orc_reset(_p(self.positions), _p(self.velocities), _p(self.yaws), _p(self.pitches),
                     _p(self.on_ground), _p(self.do_reset), B, self.spawn_x, self.spawn_y,
                     self.spawn_z)
-        self._render(want_hits)
+        if self.flat is None:
+            self._render(want_hits)
+        else:
+            L.orc_update_cameras(_p(self.cameras), _p(self.positions), _p(self.yaws),
+                                 _p(self.pitches), B)
+            if self.flat == "prebasis":
+                L.orc_camera_basis(_p(self.cameras), _p(self.basis), B, self.render_width,
+                                   self.render_height, self.fov_degrees)
+                cams = self.basis
+            else:
+                cams = self.cameras
+            r = render_flat(self.voxels, cams, self.render_width, self.render_height, self.flat,
+                            self.max_steps, self.fov_degrees, want_hits, self.nthreads)
+            if want_hits:
+                self.obs_buffer[...], self.hits = r
+            else:
+                self.obs_buffer[...] = r
         self.step_count += 1
         self.step_count[self.do_reset.astype(bool)] = 1
         if self.do_reset.any():
@@ -247,6 +311,9 @@ class OracleEnv:
 
     def state(self):
         """Copy of every state array, keyed like MineEnv's attributes."""
-        return {k: getattr(self, k).copy() for k in
-                ("voxels", "positions", "velocities", "yaws", "pitches", "on_ground",
-                 "step_count", "cameras", "rewards", "obs_buffer")}
+        d = {k: getattr(self, k).copy() for k in
+             ("voxels", "positions", "velocities", "yaws", "pitches", "on_ground",
+              "step_count", "cameras", "rewards", "obs_buffer")}
+        if self.basis is not None:
+            d["basis"] = self.basis.copy()
+        return d

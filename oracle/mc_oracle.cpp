@@ -155,7 +155,9 @@ PixelResult render_pixel(const int8_t* vox, int X, int Y, int Z, const float* ca
       t = t_max_z; t_max_z += t_delta_z; voxel_z += step_z; hit_face = 2;
     }
   }
-  if (hit_block >= 0) {
+  // Ids outside the 13-entry table (never produced by the environment) show
+  // the sky; the reference would index past BLOCK_COLORS.
+  if (hit_block >= 0 && hit_block < 13) {
     float r = BLOCK_COLORS[hit_block][0];
     float g = BLOCK_COLORS[hit_block][1];
     float b = BLOCK_COLORS[hit_block][2];
@@ -169,10 +171,101 @@ PixelResult render_pixel(const int8_t* vox, int X, int Y, int Z, const float* ca
     res.rgb[1] = g * shade * (1.0f - fog) + SKY_COLOR[1] * fog;
     res.rgb[2] = b * shade * (1.0f - fog) + SKY_COLOR[2] * fog;
   } else {
-    res.hit_index = -1;
+    if (hit_block < 0) res.hit_index = -1;
     for (int c = 0; c < 3; ++c) res.rgb[c] = SKY_COLOR[c];
   }
   return res;
+}
+
+// Flat uint8 colour table and sky (src/kernels.cu:479-495).
+const uint8_t BLOCK_COLORS_U8[13][3] = {
+    {76, 166, 51},   {140, 89, 51},   {127, 127, 127}, {140, 102, 64}, {51, 140, 38},
+    {229, 217, 153}, {51, 102, 204},  {217, 229, 242}, {178, 89, 76},  {102, 102, 102},
+    {191, 153, 102}, {242, 242, 250}, {38, 38, 38},
+};
+const uint8_t SKY_COLOR_U8[3] = {135, 206, 235};
+
+// Camera basis of one camera (precompute_camera_basis_kernel,
+// src/kernels.cu:511-561): [0-2] eye, [3-5] forward, [6-7] right x/z,
+// [8-10] up, [11] half_fov * aspect, [12] half_fov; [13] is not written.
+void camera_basis_row(const float* cam, int width, int height, float half_fov_flat,
+                      float* basis) {
+  float sin_yaw, cos_yaw, sin_pitch, cos_pitch;
+  mc_sincosf(cam[3], &sin_yaw, &cos_yaw);
+  mc_sincosf(cam[4], &sin_pitch, &cos_pitch);
+  float aspect = (float)width / (float)height;
+  basis[0] = cam[0];
+  basis[1] = cam[1] + 1.6f;
+  basis[2] = cam[2];
+  basis[3] = sin_yaw * cos_pitch;
+  basis[4] = sin_pitch;
+  basis[5] = cos_yaw * cos_pitch;
+  basis[6] = cos_yaw;
+  basis[7] = -sin_yaw;
+  basis[8] = -sin_yaw * sin_pitch;
+  basis[9] = cos_pitch;
+  basis[10] = -cos_yaw * sin_pitch;
+  basis[11] = half_fov_flat * aspect;
+  basis[12] = half_fov_flat;
+}
+
+// One pixel of the flat-colour kernels (render_kernel_uint8_minimal,
+// src/kernels.cu:985-1119; render_kernel_uint8_prebasis, 568-700): ray from
+// the basis and image-plane coordinates (u, v), normalised with the
+// reciprocal square root (restated as 1 / sqrtf, correctly rounded), DDA from
+// the eye voxel for at most max_steps lookups, out-of-bounds voxels skipped,
+// no view-distance limit.  Returns the block hit (-1: none); *hit_index gets
+// its in-world index or -1.
+int flat_pixel(const int8_t* vox, int X, int Y, int Z, const float* bs, float u, float v,
+               int max_steps, int32_t* hit_index) {
+  float eye_x = bs[0], eye_y = bs[1], eye_z = bs[2];
+  float ray_x = bs[3] + u * bs[6] + v * bs[8];
+  float ray_y = bs[4] + v * bs[9];
+  float ray_z = bs[5] + u * bs[7] + v * bs[10];
+  float ray_len = 1.0f / sqrtf(ray_x * ray_x + ray_y * ray_y + ray_z * ray_z);
+  ray_x *= ray_len;
+  ray_y *= ray_len;
+  ray_z *= ray_len;
+  int voxel_x = (int)floorf(eye_x);
+  int voxel_y = (int)floorf(eye_y);
+  int voxel_z = (int)floorf(eye_z);
+  int step_x = (ray_x >= 0) ? 1 : -1;
+  int step_y = (ray_y >= 0) ? 1 : -1;
+  int step_z = (ray_z >= 0) ? 1 : -1;
+  float inv_ray_x = (ray_x != 0.0f) ? 1.0f / ray_x : 1e30f;
+  float inv_ray_y = (ray_y != 0.0f) ? 1.0f / ray_y : 1e30f;
+  float inv_ray_z = (ray_z != 0.0f) ? 1.0f / ray_z : 1e30f;
+  float t_delta_x = fabsf(inv_ray_x);
+  float t_delta_y = fabsf(inv_ray_y);
+  float t_delta_z = fabsf(inv_ray_z);
+  float t_max_x = (ray_x >= 0) ? ((float)(voxel_x + 1) - eye_x) * inv_ray_x
+                               : ((float)voxel_x - eye_x) * inv_ray_x;
+  float t_max_y = (ray_y >= 0) ? ((float)(voxel_y + 1) - eye_y) * inv_ray_y
+                               : ((float)voxel_y - eye_y) * inv_ray_y;
+  float t_max_z = (ray_z >= 0) ? ((float)(voxel_z + 1) - eye_z) * inv_ray_z
+                               : ((float)voxel_z - eye_z) * inv_ray_z;
+  *hit_index = -1;
+  // The reference unrolls by 4 with a limit check in front of every body
+  // (kernels.cu:1088-1110); this loop visits the same voxels.
+  for (int step = 0; step < max_steps; ++step) {
+    if (voxel_x >= 0 && voxel_x < X && voxel_y >= 0 && voxel_y < Y && voxel_z >= 0 &&
+        voxel_z < Z) {
+      int idx = voxel_y * X * Z + voxel_x * Z + voxel_z;
+      int8_t block = vox[idx];
+      if (block >= 0) {
+        *hit_index = idx;
+        return block;
+      }
+    }
+    if (t_max_x < t_max_y && t_max_x < t_max_z) {
+      t_max_x += t_delta_x; voxel_x += step_x;
+    } else if (t_max_y < t_max_z) {
+      t_max_y += t_delta_y; voxel_y += step_y;
+    } else {
+      t_max_z += t_delta_z; voxel_z += step_z;
+    }
+  }
+  return -1;
 }
 
 template <class F>
@@ -471,6 +564,50 @@ void orc_render(const int8_t* voxels, const float* cameras, void* out, int out_u
           for (int c = 0; c < 3; ++c) of[o + c] = r.rgb[c];
         }
         if (hit_index) hit_index[b * npix + (int64_t)py * width + px] = r.hit_index;
+      }
+    }
+  });
+}
+
+float orc_half_fov_tan_flat(float fov_degrees) { return mc_half_fov_tan_flat(fov_degrees); }
+
+// precompute_camera_basis_kernel (src/kernels.cu:511-561) over B cameras.
+void orc_camera_basis(const float* cameras, float* basis, int B, int width, int height,
+                      float fov_degrees) {
+  const float hf = mc_half_fov_tan_flat(fov_degrees);
+  for (int b = 0; b < B; ++b) camera_basis_row(cameras + b * 5, width, height, hf, basis + b * 14);
+}
+
+// Flat-colour render.  mode 0: render_kernel_uint8_minimal from cameras [B,5]
+// (src/kernels.cu:985-1119, u = (a * half_fov) * aspect); mode 1:
+// render_kernel_uint8_prebasis from basis [B,14] (src/kernels.cu:568-700,
+// u = a * scale_u); the smem kernel (kernels.cu:835-982) computes the same
+// image as mode 1.  Ids >= 13 show the sky (the reference would index past
+// BLOCK_COLORS_U8).
+void orc_render_flat(const int8_t* voxels, const float* cams, uint8_t* out, int mode, int B,
+                     int X, int Y, int Z, int width, int height, int max_steps,
+                     float fov_degrees, int32_t* hit_index, int nthreads) {
+  const float hf = mc_half_fov_tan_flat(fov_degrees);
+  const float aspect = (float)width / (float)height;
+  const int64_t wxyz = (int64_t)X * Y * Z;
+  const int64_t npix = (int64_t)width * height;
+  parallel_for(B, nthreads, [&](int b) {
+    const int8_t* vox = voxels + b * wxyz;
+    float bs[14];
+    if (mode == 0) camera_basis_row(cams + b * 5, width, height, hf, bs);
+    else for (int k = 0; k < 13; ++k) bs[k] = cams[b * 14 + k];
+    for (int py = 0; py < height; ++py) {
+      for (int px = 0; px < width; ++px) {
+        float a = 2.0f * (px + 0.5f) / width - 1.0f;
+        float c = 1.0f - 2.0f * (py + 0.5f) / height;
+        float u = (mode == 0) ? a * hf * aspect : a * bs[11];
+        float v = (mode == 0) ? c * hf : c * bs[12];
+        int32_t hi;
+        int blk = flat_pixel(vox, X, Y, Z, bs, u, v, max_steps, &hi);
+        const uint8_t* col = (blk >= 0 && blk < 13) ? BLOCK_COLORS_U8[blk] : SKY_COLOR_U8;
+        int64_t o = (b * npix + (int64_t)py * width + px) * 3;
+        for (int k = 0; k < 3; ++k) out[o + k] = col[k];
+        if (hit_index) hit_index[b * npix + (int64_t)py * width + px] = hi;
       }
     }
   });

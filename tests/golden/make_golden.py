@@ -7,7 +7,7 @@ observation, the final voxels and the final per-pixel hit indices.  The
 fixtures freeze the oracle: tests/test_golden.py re-runs it (CPU) and the HIP
 env (GPU) against them.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 """
 import hashlib
 import os
@@ -29,6 +29,11 @@ CASES = {
                           obs_dtype="uint8", episode_length=12), 30, 1, True),
     "ws16_32px_f32": (dict(batch_size=4, world_size=16, render_width=32, render_height=32,
                            obs_dtype="float32", episode_length=None), 20, 2, True),
+    # flat-colour variants (reset render shaded, step renders flat)
+    "ws16_32x24_minimal": (dict(batch_size=6, world_size=16, render_width=32, render_height=24,
+                                obs_dtype="uint8_minimal", episode_length=10), 24, 3, False),
+    "ws32_48x32_prebasis": (dict(batch_size=6, world_size=32, render_width=48, render_height=32,
+                                 obs_dtype="uint8_prebasis", episode_length=11), 24, 4, True),
 }
 
 
@@ -60,11 +65,15 @@ def run_case(kw, T, seed, bh):
     out = {k: np.stack(v) for k, v in rec.items()}
     out.update(actions=acts, obs_first=obs0, obs_last=env.obs_buffer.copy(),
                voxels_last=env.voxels.copy(), hits_last=env.hits.copy())
+    if env.basis is not None:
+        out["basis_last"] = env.basis.copy()
     return out
 
 
-def main():
+def main(names=None):
     for name, (kw, T, seed, bh) in CASES.items():
+        if names and name not in names:
+            continue
         out = run_case(kw, T, seed, bh)
         path = os.path.join(HERE, f"{name}.npz")
         np.savez_compressed(path, **out)
@@ -72,4 +81,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -29,9 +29,12 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
     _l, _ = _lib()
-    # 28 four-byte fields, 11 pointers
+    # 28 four-byte fields, 12 pointers
     assert ctypes.sizeof(_l.McConfig) == 28 * 4
-    assert ctypes.sizeof(_l.McState) == 11 * 8
+    assert ctypes.sizeof(_l.McState) == 12 * 8
+    text = open(HEADER).read()
+    state = text[text.index("typedef struct mc_state"):text.index("} mc_state;")]
+    assert len(re.findall(r"\*\s*\w+;", state)) == 12
 
 
 def _err(L):
@@ -58,6 +61,16 @@ def test_null_and_range_errors():
     assert "non-NULL" in _err(L) and not out.value
     assert L.mc_env_step(None, v, 0, v, None, None) == _l.MC_E_INVALID
     assert L.mc_debug_sincosf(None, None, None, 4, None) == _l.MC_E_INVALID
+    # flat variants: obs modes 2/3 are valid, 4 is not; NULL basis is refused
+    assert L.mc_render(v, v, v, 4, 4, 8, 8, 8, 8, 8, 64, 32.0, 70.0, None) == _l.MC_E_INVALID
+    assert L.mc_launch_precompute_camera_basis(None, v, 4, 8, 8, 70.0, None) == _l.MC_E_INVALID
+    assert L.mc_launch_render_uint8_prebasis(v, None, v, 4, 8, 8, 8, 64, None) == _l.MC_E_INVALID
+    args = [v] * 21
+    args[18] = None  # basis
+    assert L.mc_launch_step_uint8_prebasis(*args, 4, 8, 8, 8, 8, 8, 64, 32.0, 70.0, 0.05, -20.0,
+                                           4.0, 1.5, 8.0, 3, 1.0, 4.0, 10.0, 4.0,
+                                           None) == _l.MC_E_INVALID
+    assert "basis" in _err(L)
 
 
 def test_reference_step_rejects_partial_decode_buffers():

@@ -47,3 +47,5 @@ def test_hip_env_matches_golden(name):
         assert np.array_equal(sha(obs.cpu().numpy()), g["obs_sha"][t]), f"{name} step {t} obs"
     assert np.array_equal(env.obs_buffer.cpu().numpy(), g["obs_last"])
     assert np.array_equal(env.voxels.cpu().numpy(), g["voxels_last"])
+    if "basis_last" in g:
+        assert np.array_equal(env.basis.cpu().numpy().view(np.uint8), g["basis_last"].view(np.uint8))
