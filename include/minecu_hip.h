@@ -48,10 +48,12 @@ enum {
    * the flat image; the reset render and mc_render_state stay shaded uint8, as MineEnv's
    * _render() does for every uint8 variant (minecu/__init__.py:446-466). */
   MC_OBS_U8_MINIMAL = 2,  /* obs_dtype "uint8_minimal"  (render_kernel_uint8_minimal, kernels.cu:985-1119) */
-  MC_OBS_U8_PREBASIS = 3  /* "uint8_prebasis" / "_full" / "uint8_smem": image plane from the [B,14]
+  MC_OBS_U8_PREBASIS = 3, /* "uint8_prebasis" / "_full" / "uint8_smem": image plane from the [B,14]
                              camera basis (render_kernel_uint8_prebasis, kernels.cu:568-700; the
                              smem kernel, kernels.cu:835-982, renders the same image); steps
                              write st->basis when it is non-NULL */
+  MC_OBS_U8_FP16 = 4      /* "uint8_fp16": the minimal image plane with a binary16 DDA
+                             (render_kernel_uint8_fp16, kernels.cu:1127-1262) */
 };
 
 /* Action encodings accepted by the step. */
@@ -227,6 +229,20 @@ int mc_launch_step_uint8_minimal(int8_t* voxels, float* positions, float* veloci
                                  float jump_vel, int8_t target_block, float reward_value,
                                  float spawn_x, float spawn_y, float spawn_z, void* stream);
 
+/* Replaces launch_step_uint8_fp16 (src/kernels.cu:2776-2835; binding
+ * bindings.cpp:684-734): the fused step with the binary16-DDA flat render. */
+int mc_launch_step_uint8_fp16(int8_t* voxels, float* positions, float* velocities, float* yaws,
+                              float* pitches, uint8_t* on_ground, const int8_t* buttons,
+                              const float* look, const uint8_t* do_reset, float* forward_in,
+                              float* strafe_in, float* delta_yaw_in, float* delta_pitch_in,
+                              uint8_t* jump_in, uint8_t* do_break, uint8_t* do_place,
+                              float* speed_mult, float* cameras, uint8_t* obs_buffer,
+                              float* rewards, int batch_size, int world_size, int width,
+                              int height, int max_steps, float view_distance, float fov_degrees,
+                              float dt, float gravity, float walk_speed, float sprint_mult,
+                              float jump_vel, int8_t target_block, float reward_value,
+                              float spawn_x, float spawn_y, float spawn_z, void* stream);
+
 /* Replaces launch_step_uint8_prebasis (src/kernels.cu:2993-3076; binding
  * bindings.cpp:736-792): the fused step writing basis f32 [B,14] and the flat
  * prebasis render; world extents per axis. */
@@ -285,6 +301,12 @@ int mc_launch_render_uint8_minimal(const int8_t* voxels, const float* cameras, u
                                    int batch_size, int world_size, int width, int height,
                                    int max_steps, float view_distance, float fov_degrees,
                                    void* stream);
+
+/* Replaces launch_render_uint8_fp16 (src/kernels.cu:2144-2158). */
+int mc_launch_render_uint8_fp16(const int8_t* voxels, const float* cameras, uint8_t* output,
+                                int batch_size, int world_size, int width, int height,
+                                int max_steps, float view_distance, float fov_degrees,
+                                void* stream);
 
 /* Replaces launch_render_uint8_prebasis (src/kernels.cu:2172-2189; binding
  * bindings.cpp:877-899) and launch_render_uint8_smem (kernels.cu:2191-2211):
@@ -401,6 +423,10 @@ int mc_server_autotune(mc_server* s, int steps, void* stream);
 
 /* Device evaluation of the shared sin/cos (mc_math.h) on n inputs. */
 int mc_debug_sincosf(const float* x, float* s, float* c, int n, void* stream);
+/* Device binary16 arithmetic of the half-precision DDA on n float pairs:
+ * ha = half(a), sum = half(a) + half(b) (binary16 bits), lt = half(a) < half(b). */
+int mc_debug_f16(const float* a, const float* b, uint16_t* ha, uint16_t* sum, uint8_t* lt, int n,
+                 void* stream);
 
 #ifdef __cplusplus
 }

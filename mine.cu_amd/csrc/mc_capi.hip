@@ -59,7 +59,7 @@ int validate_config(const mc_config* c) {
   if (r) return r;
   r = check_image(c->batch_size, c->width, c->height);
   if (r) return r;
-  if (c->obs_mode < MC_OBS_U8 || c->obs_mode > MC_OBS_U8_PREBASIS)
+  if (c->obs_mode < MC_OBS_U8 || c->obs_mode > MC_OBS_U8_FP16)
     return fail(MC_E_INVALID, "obs_mode must be one of MC_OBS_* (got %d)", c->obs_mode);
   return MC_OK;
 }
@@ -272,7 +272,7 @@ int mc_render(const int8_t* voxels, const float* cameras, void* obs, int obs_mod
   if (r) return r;
   r = check_image(B, W, H);
   if (r) return r;
-  if (obs_mode < MC_OBS_U8 || obs_mode > MC_OBS_U8_PREBASIS)
+  if (obs_mode < MC_OBS_U8 || obs_mode > MC_OBS_U8_FP16)
     return fail(MC_E_INVALID, "bad obs_mode %d", obs_mode);
   mc_config c;
   memset(&c, 0, sizeof c);
@@ -630,6 +630,22 @@ int mc_launch_step_uint8_minimal(int8_t* voxels, float* positions, float* veloci
                          MC_REF_STEP_ARGS, world_size, MC_REF_STEP_TAIL);
 }
 
+int mc_launch_step_uint8_fp16(int8_t* voxels, float* positions, float* velocities, float* yaws,
+                              float* pitches, uint8_t* on_ground, const int8_t* buttons,
+                              const float* look, const uint8_t* do_reset, float* forward_in,
+                              float* strafe_in, float* delta_yaw_in, float* delta_pitch_in,
+                              uint8_t* jump_in, uint8_t* do_break, uint8_t* do_place,
+                              float* speed_mult, float* cameras, uint8_t* obs_buffer,
+                              float* rewards, int batch_size, int world_size, int width,
+                              int height, int max_steps, float view_distance, float fov_degrees,
+                              float dt, float gravity, float walk_speed, float sprint_mult,
+                              float jump_vel, int8_t target_block, float reward_value,
+                              float spawn_x, float spawn_y, float spawn_z, void* stream) {
+  return launch_step_ref(MC_OBS_U8_FP16,
+                         RefStep{world_size, world_size, world_size, nullptr, nullptr, 0},
+                         MC_REF_STEP_ARGS, world_size, MC_REF_STEP_TAIL);
+}
+
 int mc_launch_step_uint8_prebasis(int8_t* voxels, float* positions, float* velocities,
                                   float* yaws, float* pitches, uint8_t* on_ground,
                                   const int8_t* buttons, const float* look,
@@ -712,6 +728,14 @@ int mc_launch_render_uint8_minimal(const int8_t* voxels, const float* cameras, u
   return mc_render(voxels, cameras, output, MC_OBS_U8_MINIMAL, batch_size, world_size,
                    world_size, world_size, width, height, max_steps, view_distance, fov_degrees,
                    stream);
+}
+
+int mc_launch_render_uint8_fp16(const int8_t* voxels, const float* cameras, uint8_t* output,
+                                int batch_size, int world_size, int width, int height,
+                                int max_steps, float view_distance, float fov_degrees,
+                                void* stream) {
+  return mc_render(voxels, cameras, output, MC_OBS_U8_FP16, batch_size, world_size, world_size,
+                   world_size, width, height, max_steps, view_distance, fov_degrees, stream);
 }
 
 int mc_launch_render_uint8_prebasis(const int8_t* voxels, const float* basis, uint8_t* output,
@@ -1091,6 +1115,15 @@ int mc_debug_sincosf(const float* x, float* s, float* c, int n, void* stream) {
   hipLaunchKernelGGL(mc::k_debug_sincos, dim3((n + 255) / 256), dim3(256), 0, S(stream), x, s, c,
                      n);
   return check_launch("debug_sincosf");
+}
+
+int mc_debug_f16(const float* a, const float* b, uint16_t* ha, uint16_t* sum, uint8_t* lt, int n,
+                 void* stream) {
+  if (!a || !b || !ha || !sum || !lt || n < 0) return fail(MC_E_INVALID, "debug_f16: bad args");
+  if (n == 0) return MC_OK;
+  hipLaunchKernelGGL(mc::k_debug_f16, dim3((n + 255) / 256), dim3(256), 0, S(stream), a, b, ha,
+                     sum, lt, n);
+  return check_launch("debug_f16");
 }
 
 }  // extern "C"

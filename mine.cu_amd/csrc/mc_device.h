@@ -615,13 +615,18 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, 
 // The reference normalises with rsqrtf; here the factor is 1 / sqrtf(len^2),
 // both correctly rounded.  Returns the block hit, or -1.
 //
+// T = _Float16: the half-precision variant (src/kernels.cu:1127-1262): the
+// same fp32 ray, a 65504 stand-in for 1/0, and t_max / t_delta rounded to
+// binary16 (round to nearest even) then stepped and compared in binary16.
+//
 // Eye voxel in the world: leaving the world ends the ray (as in trace()).
 // Otherwise a ray outside the world that moves away from it on some axis
 // can never come back, which ends it too.
 // ---------------------------------------------------------------------------
-template <bool LIFT>
+template <bool LIFT, typename T>
 __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, float u, float v,
                                           const int8_t* __restrict__ world) {
+  constexpr float kInf = (sizeof(T) == 2) ? 65504.0f : 1e30f;
   float ray_x = f.fx + u * f.rx + v * f.ux;
   float ray_y = f.fy + v * f.uy;
   float ray_z = f.fz + u * f.rz + v * f.uz;
@@ -629,20 +634,20 @@ __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, float
   ray_x *= rl;
   ray_y *= rl;
   ray_z *= rl;
-  const float inv_x = (ray_x != 0.0f) ? 1.0f / ray_x : 1e30f;
-  const float inv_y = (ray_y != 0.0f) ? 1.0f / ray_y : 1e30f;
-  const float inv_z = (ray_z != 0.0f) ? 1.0f / ray_z : 1e30f;
+  const float inv_x = (ray_x != 0.0f) ? 1.0f / ray_x : kInf;
+  const float inv_y = (ray_y != 0.0f) ? 1.0f / ray_y : kInf;
+  const float inv_z = (ray_z != 0.0f) ? 1.0f / ray_z : kInf;
   const int max_steps = p.max_steps;
   const int X = p.X, Y = p.Y, Z = p.Z, XZ = X * Z;
   const int step_x = (ray_x >= 0) ? 1 : -1;
   const int step_y = (ray_y >= 0) ? 1 : -1;
   const int step_z = (ray_z >= 0) ? 1 : -1;
-  const float td_x = fabsf(inv_x), td_y = fabsf(inv_y), td_z = fabsf(inv_z);
+  const T td_x = (T)fabsf(inv_x), td_y = (T)fabsf(inv_y), td_z = (T)fabsf(inv_z);
   const int dix = step_x * Z, diy = step_y * XZ, diz = step_z;
   if (f.inside) {
-    float tm_x = ((ray_x >= 0) ? f.dxp : f.dxn) * inv_x;
-    float tm_y = ((ray_y >= 0) ? f.dyp : f.dyn) * inv_y;
-    float tm_z = ((ray_z >= 0) ? f.dzp : f.dzn) * inv_z;
+    T tm_x = (T)(((ray_x >= 0) ? f.dxp : f.dxn) * inv_x);
+    T tm_y = (T)(((ray_y >= 0) ? f.dyp : f.dyn) * inv_y);
+    T tm_z = (T)(((ray_z >= 0) ? f.dzp : f.dzn) * inv_z);
     int lim_x = X, lim_y = Y, lim_z = Z;
     if (LIFT && f.lift && step_y > 0) {
       lim_y = max(0, min(Y, f.hy1 + 1));
@@ -677,9 +682,9 @@ __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, float
     return (!lim & !out & (blk >= 0)) ? blk : -1;
   }
   int vx = (int)floorf(f.ex), vy = (int)floorf(f.ey), vz = (int)floorf(f.ez);
-  float tm_x = (ray_x >= 0) ? ((float)(vx + 1) - f.ex) * inv_x : ((float)vx - f.ex) * inv_x;
-  float tm_y = (ray_y >= 0) ? ((float)(vy + 1) - f.ey) * inv_y : ((float)vy - f.ey) * inv_y;
-  float tm_z = (ray_z >= 0) ? ((float)(vz + 1) - f.ez) * inv_z : ((float)vz - f.ez) * inv_z;
+  T tm_x = (T)((ray_x >= 0) ? ((float)(vx + 1) - f.ex) * inv_x : ((float)vx - f.ex) * inv_x);
+  T tm_y = (T)((ray_y >= 0) ? ((float)(vy + 1) - f.ey) * inv_y : ((float)vy - f.ey) * inv_y);
+  T tm_z = (T)((ray_z >= 0) ? ((float)(vz + 1) - f.ez) * inv_z : ((float)vz - f.ez) * inv_z);
   for (int step = 0; step < max_steps; ++step) {
     const bool inb = (unsigned)vx < (unsigned)X && (unsigned)vy < (unsigned)Y &&
                      (unsigned)vz < (unsigned)Z;
@@ -803,9 +808,10 @@ __device__ __forceinline__ void render_rgb8(const Params& p, int b, int uvm, int
 }
 
 // Flat-colour uint8 render of env b (src/kernels.cu:985-1119 minimal,
-// 568-700 prebasis, 835-982 smem: same image as prebasis).  flat: the packed
-// colour table, 13 blocks then the sky; ids >= 13 show the sky.
-template <bool LIFT>
+// 568-700 prebasis, 835-982 smem: same image as prebasis, 1127-1262 with a
+// binary16 DDA when T = _Float16).  flat: the packed colour table, 13 blocks
+// then the sky; ids >= 13 show the sky.
+template <bool LIFT, typename T>
 __device__ __forceinline__ void render_env_flat(const Params& p, int b, const int8_t* world,
                                                 const uint32_t* flat, const Frame& f, int uvm,
                                                 int tid, int nt, const float* su,
@@ -815,7 +821,7 @@ __device__ __forceinline__ void render_env_flat(const Params& p, int b, const in
   const bool per_env_scale = (uvm == kUvRaw);
   auto color = [&](float u, float v) -> uint32_t {
     if (per_env_scale) { u = u * f.su; v = v * f.sv; }
-    const int h = (f.flat_fill >= 0) ? f.flat_fill : trace_flat<LIFT>(p, f, u, v, world);
+    const int h = (f.flat_fill >= 0) ? f.flat_fill : trace_flat<LIFT, T>(p, f, u, v, world);
     return flat[(h >= 0 && h < kFlatSky) ? h : kFlatSky];
   };
   if (f.flat_fill >= 0 && ((npix * 3) & 15) == 0) {
@@ -1195,7 +1201,7 @@ enum ItemKind {
 // variants with the minimal kernel's and the prebasis kernel's image plane.
 // Steps render the variant; the reset and state renders of the flat modes
 // are shaded uint8, as the reference's _render() is (minecu/__init__.py:446-466).
-enum ObsMode { kObsU8 = 0, kObsF32 = 1, kObsU8Minimal = 2, kObsU8Prebasis = 3 };
+enum ObsMode { kObsU8 = 0, kObsF32 = 1, kObsU8Minimal = 2, kObsU8Prebasis = 3, kObsU8Fp16 = 4 };
 
 __host__ __device__ __forceinline__ int world_stride_bytes(int X, int Y, int Z) {
   return (X * Y * Z + 15) & ~15;
@@ -1226,8 +1232,9 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   // Flat images for steps and camera renders of the flat modes; shaded
   // uint8 for their reset and state renders.
   const bool flat_kind = (KIND == kItemStep || KIND == kItemRenderCameras);
-  const int uvm = (KIND == kItemRenderBasis)                         ? kUvRaw
-                  : (flat_kind && obs_mode == kObsU8Minimal)  ? kUvMinimal
+  const int uvm = (KIND == kItemRenderBasis) ? kUvRaw
+                  : (flat_kind && (obs_mode == kObsU8Minimal || obs_mode == kObsU8Fp16))
+                      ? kUvMinimal
                   : (flat_kind && obs_mode == kObsU8Prebasis) ? kUvBasis
                                                               : kUvShaded;
   fill_uv(p, uvm, su, sv, tid, NT);
@@ -1255,7 +1262,10 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   }
   if (active) {
     const Frame f = uniform_frame(&frames[e]);
-    if (uvm != kUvShaded) render_env_flat<LIFT>(p, b, world, flat, f, uvm, et, ent, su, sv);
+    if (uvm != kUvShaded && obs_mode == kObsU8Fp16)
+      render_env_flat<LIFT, _Float16>(p, b, world, flat, f, uvm, et, ent, su, sv);
+    else if (uvm != kUvShaded)
+      render_env_flat<LIFT, float>(p, b, world, flat, f, uvm, et, ent, su, sv);
     else if (obs_mode == kObsF32) render_env<1, LIFT>(p, b, world, pal, f, et, ent, su, sv);
     else render_env<0, LIFT>(p, b, world, pal, f, et, ent, su, sv);
     if (KIND == kItemStep && any_done) store_fresh_world(p, gworld, nullptr, nvox, et, ent);
@@ -1477,6 +1487,19 @@ __global__ void k_camera_basis(Params p) {
   f.uy = cos_pitch;
   f.uz = -cos_yaw * sin_pitch;
   store_basis(p, b, f);
+}
+
+// binary16 conversion, sum and order of n float pairs (test hook for the
+// half-precision DDA arithmetic).
+__global__ void k_debug_f16(const float* a, const float* b, uint16_t* ha, uint16_t* sum,
+                            uint8_t* lt, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const _Float16 x = (_Float16)a[i], y = (_Float16)b[i];
+  const _Float16 z = x + y;
+  ha[i] = __builtin_bit_cast(uint16_t, x);
+  sum[i] = __builtin_bit_cast(uint16_t, z);
+  lt[i] = (x < y) ? 1 : 0;
 }
 
 __global__ void k_debug_sincos(const float* x, float* s, float* c, int n) {

@@ -5,9 +5,8 @@ The reference's bindings check device and dtype only (e.g.
 bindings.cpp:537-543); these also check shapes and contiguity, so a mismatch
 raises RuntimeError instead of reading out of bounds on the device.  Every
 launch goes to torch's current stream (the reference uses the legacy default
-stream).  Not implemented: the half-precision DDA step (step_uint8_fp16),
-the ground-accelerated step_uint8_fast and the [Y,X,Z,B]-layout
-render_uint8_coalesced, none of which MineEnv calls in the modes built here.
+stream).  Not implemented: the ground-accelerated step_uint8_fast and the
+[Y,X,Z,B]-layout render_uint8_coalesced, neither of which MineEnv calls.
 """
 from __future__ import annotations
 
@@ -198,6 +197,20 @@ def step_uint8_minimal(voxels, positions, velocities, yaws, pitches, on_ground, 
                        reward_value=1.0, spawn_x=16.0, spawn_y=9.0, spawn_z=16.0) -> None:
     """bindings.cpp:632-682 -> launch_step_uint8_minimal (flat colours)."""
     _step("mc_launch_step_uint8_minimal", torch.uint8, voxels, positions, velocities, yaws,
+          pitches, on_ground, buttons, look, do_reset, forward_in, strafe_in, delta_yaw_in,
+          delta_pitch_in, jump_in, do_break, do_place, speed_mult, cameras, obs_buffer, rewards,
+          width, height, max_steps, view_distance, fov_degrees, dt, gravity, walk_speed,
+          sprint_mult, jump_vel, target_block, reward_value, spawn_x, spawn_y, spawn_z)
+
+
+def step_uint8_fp16(voxels, positions, velocities, yaws, pitches, on_ground, buttons, look,
+                    do_reset, forward_in, strafe_in, delta_yaw_in, delta_pitch_in, jump_in,
+                    do_break, do_place, speed_mult, cameras, obs_buffer, rewards, width, height,
+                    max_steps=64, view_distance=32.0, fov_degrees=70.0, dt=0.05, gravity=-20.0,
+                    walk_speed=4.0, sprint_mult=1.5, jump_vel=8.0, target_block=-1,
+                    reward_value=1.0, spawn_x=16.0, spawn_y=9.0, spawn_z=16.0) -> None:
+    """bindings.cpp:684-734 -> launch_step_uint8_fp16 (flat colours, binary16 DDA)."""
+    _step("mc_launch_step_uint8_fp16", torch.uint8, voxels, positions, velocities, yaws,
           pitches, on_ground, buttons, look, do_reset, forward_in, strafe_in, delta_yaw_in,
           delta_pitch_in, jump_in, do_break, do_place, speed_mult, cameras, obs_buffer, rewards,
           width, height, max_steps, view_distance, fov_degrees, dt, gravity, walk_speed,

@@ -65,7 +65,8 @@ _OBS_MODES = {"uint8": _lib.MC_OBS_U8, "float32": _lib.MC_OBS_F32,
               "uint8_minimal": _lib.MC_OBS_U8_MINIMAL,
               "uint8_prebasis": _lib.MC_OBS_U8_PREBASIS,
               "uint8_prebasis_full": _lib.MC_OBS_U8_PREBASIS,
-              "uint8_smem": _lib.MC_OBS_U8_PREBASIS}
+              "uint8_smem": _lib.MC_OBS_U8_PREBASIS,
+              "uint8_fp16": _lib.MC_OBS_U8_FP16}
 _BASIS_OBS = ("uint8_prebasis", "uint8_prebasis_full", "uint8_smem")
 _REFERENCE_OBS = ("float32", "uint8", "uint8_minimal", "uint8_fp16", "uint8_prebasis",
                   "uint8_prebasis_full", "uint8_smem")
@@ -149,9 +150,6 @@ class MineEnv:
         assert "cuda" in str(device), "MineEnv requires CUDA device"
         assert obs_dtype in _REFERENCE_OBS, \
             "obs_dtype must be one of " + ", ".join(repr(o) for o in _REFERENCE_OBS)
-        if obs_dtype not in _OBS_MODES:
-            raise NotImplementedError(
-                f"obs_dtype={obs_dtype!r} (half-precision DDA variant) is not implemented")
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())

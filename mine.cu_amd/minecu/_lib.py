@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libminecu_hip.so")
 
 MC_OK, MC_E_INVALID, MC_E_HIP, MC_E_UNSUPPORTED = 0, -1, -2, -3
-MC_OBS_U8, MC_OBS_F32, MC_OBS_U8_MINIMAL, MC_OBS_U8_PREBASIS = 0, 1, 2, 3
+MC_OBS_U8, MC_OBS_F32, MC_OBS_U8_MINIMAL, MC_OBS_U8_PREBASIS, MC_OBS_U8_FP16 = 0, 1, 2, 3, 4
 MC_ACT_BUTTONS, MC_ACT_DISCRETE = 0, 1
 MC_WORKSPACE_BYTES = 256
 
@@ -63,6 +63,8 @@ SIGNATURES = {
     "mc_launch_render_uint8": ([_P, _P, _P, _I, _I, _I, _I, _I, _F, _F, _P], _I),
     "mc_launch_render": ([_P, _P, _P, _I, _I, _I, _I, _I, _F, _F, _P], _I),
     "mc_launch_step_uint8_minimal": ([_P] * 20 + [_I] * 5 + [_F] * 7 + [ctypes.c_int8, _F, _F, _F, _F, _P], _I),
+    "mc_launch_step_uint8_fp16": ([_P] * 20 + [_I] * 5 + [_F] * 7 + [ctypes.c_int8, _F, _F, _F, _F, _P], _I),
+    "mc_launch_render_uint8_fp16": ([_P, _P, _P, _I, _I, _I, _I, _I, _F, _F, _P], _I),
     "mc_launch_step_uint8_prebasis": ([_P] * 21 + [_I] * 7 + [_F] * 7 + [ctypes.c_int8, _F, _F, _F, _F, _P], _I),
     "mc_launch_step_uint8_prebasis_full": ([_P] * 22 + [_I] * 7 + [_F] * 7 + [ctypes.c_int8, _F, _F, _F, _F, _I, _P], _I),
     "mc_launch_step_uint8_smem": ([_P] * 21 + [_I] * 5 + [_F] * 7 + [ctypes.c_int8, _F, _F, _F, _F, _P], _I),
@@ -98,6 +100,7 @@ SIGNATURES = {
     "mc_server_stats": ([_P, _P, _P, _P, _P], _I),
     "mc_server_autotune": ([_P, _I, _P], _I),
     "mc_debug_sincosf": ([_P, _P, _P, _I, _P], _I),
+    "mc_debug_f16": ([_P, _P, _P, _P, _P, _I, _P], _I),
 }
 
 _lib = None

@@ -69,6 +69,7 @@ def lib():
         L.orc_half_fov_tan_flat.restype = f
         L.orc_camera_basis.argtypes = [P, P, i, i, i, f]
         L.orc_render_flat.argtypes = [P, P, P, i, i, i, i, i, i, i, i, f, P, i]
+        L.orc_f16.argtypes = [P, P, P, P, P, i]
         _lib = L
     return _lib
 
@@ -140,15 +141,31 @@ def render_flat(voxels, cams, width, height, mode, max_steps=64, fov_degrees=70.
                 want_hits=False, nthreads=None):
     """Flat-colour uint8 render.  mode "minimal": cams are [B,5] cameras
     (src/kernels.cu:985-1119); mode "prebasis": cams are a [B,14] basis
-    (src/kernels.cu:568-700)."""
+    (src/kernels.cu:568-700); mode "fp16": cameras, binary16 DDA
+    (src/kernels.cu:1127-1262)."""
     B, Y, X, Z = voxels.shape
     cams = np.ascontiguousarray(cams, dtype=np.float32)
     out = np.empty((B, height, width, 3), dtype=np.uint8)
     hits = np.empty((B, height, width), dtype=np.int32) if want_hits else None
-    lib().orc_render_flat(_p(voxels), _p(cams), _p(out), 0 if mode == "minimal" else 1, B, X, Y,
+    lib().orc_render_flat(_p(voxels), _p(cams), _p(out), _FLAT_MODE_ID[mode], B, X, Y,
                           Z, width, height, max_steps, fov_degrees, _p(hits),
                           nthreads or default_threads())
     return (out, hits) if want_hits else out
+
+
+_FLAT_MODE_ID = {"minimal": 0, "prebasis": 1, "fp16": 2}
+
+
+def f16_ops(a, b):
+    """Software binary16 (round to nearest even): (bits of half(a), bits of
+    half(a) + half(b), half(a) < half(b)) for float arrays a, b."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    ha = np.empty(a.shape, np.uint16)
+    s = np.empty(a.shape, np.uint16)
+    lt = np.empty(a.shape, np.uint8)
+    lib().orc_f16(_p(a), _p(b), _p(ha), _p(s), _p(lt), a.size)
+    return ha, s, lt
 
 
 def discrete_to_buttons(actions: np.ndarray, look_delta: float):
@@ -167,7 +184,7 @@ def discrete_to_buttons(actions: np.ndarray, look_delta: float):
 # obs_dtype -> flat render mode of the step (None: shaded).
 _FLAT = {"uint8": None, "float32": None, "uint8_minimal": "minimal",
          "uint8_prebasis": "prebasis", "uint8_prebasis_full": "prebasis",
-         "uint8_smem": "prebasis"}
+         "uint8_smem": "prebasis", "uint8_fp16": "fp16"}
 
 
 class OracleEnv:

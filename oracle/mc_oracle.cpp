@@ -18,6 +18,8 @@
 // this restatement takes X, Y, Z separately (as the prebasis path does,
 // src/bindings.cpp:759-761).  For cubic worlds the two are identical.
 #include <math.h>
+
+#include <cmath>
 #include <stdint.h>
 #include <string.h>
 
@@ -209,6 +211,47 @@ void camera_basis_row(const float* cam, int width, int height, float half_fov_fl
   basis[12] = half_fov_flat;
 }
 
+// IEEE binary16 in software (gcc has no _Float16 on this host), round to
+// nearest even, subnormals kept, overflow to infinity -- the semantics of
+// CUDA's __float2half / __hadd / __hlt used by render_kernel_uint8_fp16
+// (src/kernels.cu:1190-1225).  A binary16 sum is exact in binary64 (at most
+// 40 exponent steps + 11 significant bits), so add = one rounding of the
+// exact sum.
+uint16_t f16_from_double(double x) {
+  const uint16_t sign = std::signbit(x) ? 0x8000 : 0;
+  const double ax = fabs(x);
+  if (std::isnan(x)) return 0x7e00;
+  if (ax >= 65520.0) return sign | 0x7c00;          // rounds past 65504: infinity
+  if (ax < 6.103515625e-05) {                       // below 2^-14: subnormal (or zero)
+    const double q = nearbyint(ax * 16777216.0);    // units of 2^-24, ties to even
+    return sign | (uint16_t)q;                      // q == 1024 is the smallest normal
+  }
+  int e;
+  const double m = frexp(ax, &e);                   // ax = m 2^e, m in [0.5, 1)
+  double q = nearbyint(m * 2048.0);                 // 11 significant bits
+  if (q == 2048.0) { q = 1024.0; ++e; }
+  const int biased = e - 1 + 15;
+  if (biased >= 31) return sign | 0x7c00;
+  return sign | (uint16_t)((biased << 10) | ((int)q - 1024));
+}
+
+double f16_to_double(uint16_t h) {
+  const int e = (h >> 10) & 0x1f, f = h & 0x3ff;
+  double v;
+  if (e == 0) v = ldexp((double)f, -24);
+  else if (e == 31) v = f ? NAN : INFINITY;
+  else v = ldexp((double)(f | 0x400), e - 25);
+  return (h & 0x8000) ? -v : v;
+}
+
+struct Half {
+  uint16_t bits;
+  static Half from_float(float f) { return Half{f16_from_double((double)f)}; }
+  double value() const { return f16_to_double(bits); }
+  Half operator+(Half o) const { return Half{f16_from_double(value() + o.value())}; }
+  bool operator<(Half o) const { return value() < o.value(); }
+};
+
 // One pixel of the flat-colour kernels (render_kernel_uint8_minimal,
 // src/kernels.cu:985-1119; render_kernel_uint8_prebasis, 568-700): ray from
 // the basis and image-plane coordinates (u, v), normalised with the
@@ -216,8 +259,20 @@ void camera_basis_row(const float* cam, int width, int height, float half_fov_fl
 // the eye voxel for at most max_steps lookups, out-of-bounds voxels skipped,
 // no view-distance limit.  Returns the block hit (-1: none); *hit_index gets
 // its in-world index or -1.
+struct F32 {
+  float x;
+  static F32 from_float(float f) { return F32{f}; }
+  F32 operator+(F32 o) const { return F32{x + o.x}; }
+  bool operator<(F32 o) const { return x < o.x; }
+};
+
+// T = F32: the binary32 kernels; T = Half: render_kernel_uint8_fp16
+// (src/kernels.cu:1127-1262), whose 1/0 stand-in is 65504 and whose t_max /
+// t_delta are binary16.
+template <class T>
 int flat_pixel(const int8_t* vox, int X, int Y, int Z, const float* bs, float u, float v,
                int max_steps, int32_t* hit_index) {
+  const float kBig = sizeof(T) == 2 ? 65504.0f : 1e30f;
   float eye_x = bs[0], eye_y = bs[1], eye_z = bs[2];
   float ray_x = bs[3] + u * bs[6] + v * bs[8];
   float ray_y = bs[4] + v * bs[9];
@@ -232,18 +287,18 @@ int flat_pixel(const int8_t* vox, int X, int Y, int Z, const float* bs, float u,
   int step_x = (ray_x >= 0) ? 1 : -1;
   int step_y = (ray_y >= 0) ? 1 : -1;
   int step_z = (ray_z >= 0) ? 1 : -1;
-  float inv_ray_x = (ray_x != 0.0f) ? 1.0f / ray_x : 1e30f;
-  float inv_ray_y = (ray_y != 0.0f) ? 1.0f / ray_y : 1e30f;
-  float inv_ray_z = (ray_z != 0.0f) ? 1.0f / ray_z : 1e30f;
-  float t_delta_x = fabsf(inv_ray_x);
-  float t_delta_y = fabsf(inv_ray_y);
-  float t_delta_z = fabsf(inv_ray_z);
-  float t_max_x = (ray_x >= 0) ? ((float)(voxel_x + 1) - eye_x) * inv_ray_x
-                               : ((float)voxel_x - eye_x) * inv_ray_x;
-  float t_max_y = (ray_y >= 0) ? ((float)(voxel_y + 1) - eye_y) * inv_ray_y
-                               : ((float)voxel_y - eye_y) * inv_ray_y;
-  float t_max_z = (ray_z >= 0) ? ((float)(voxel_z + 1) - eye_z) * inv_ray_z
-                               : ((float)voxel_z - eye_z) * inv_ray_z;
+  float inv_ray_x = (ray_x != 0.0f) ? 1.0f / ray_x : kBig;
+  float inv_ray_y = (ray_y != 0.0f) ? 1.0f / ray_y : kBig;
+  float inv_ray_z = (ray_z != 0.0f) ? 1.0f / ray_z : kBig;
+  T t_delta_x = T::from_float(fabsf(inv_ray_x));
+  T t_delta_y = T::from_float(fabsf(inv_ray_y));
+  T t_delta_z = T::from_float(fabsf(inv_ray_z));
+  T t_max_x = T::from_float((ray_x >= 0) ? ((float)(voxel_x + 1) - eye_x) * inv_ray_x
+                                         : ((float)voxel_x - eye_x) * inv_ray_x);
+  T t_max_y = T::from_float((ray_y >= 0) ? ((float)(voxel_y + 1) - eye_y) * inv_ray_y
+                                         : ((float)voxel_y - eye_y) * inv_ray_y);
+  T t_max_z = T::from_float((ray_z >= 0) ? ((float)(voxel_z + 1) - eye_z) * inv_ray_z
+                                         : ((float)voxel_z - eye_z) * inv_ray_z);
   *hit_index = -1;
   // The reference unrolls by 4 with a limit check in front of every body
   // (kernels.cu:1088-1110); this loop visits the same voxels.
@@ -258,11 +313,11 @@ int flat_pixel(const int8_t* vox, int X, int Y, int Z, const float* bs, float u,
       }
     }
     if (t_max_x < t_max_y && t_max_x < t_max_z) {
-      t_max_x += t_delta_x; voxel_x += step_x;
+      t_max_x = t_max_x + t_delta_x; voxel_x += step_x;
     } else if (t_max_y < t_max_z) {
-      t_max_y += t_delta_y; voxel_y += step_y;
+      t_max_y = t_max_y + t_delta_y; voxel_y += step_y;
     } else {
-      t_max_z += t_delta_z; voxel_z += step_z;
+      t_max_z = t_max_z + t_delta_z; voxel_z += step_z;
     }
   }
   return -1;
@@ -582,8 +637,9 @@ void orc_camera_basis(const float* cameras, float* basis, int B, int width, int 
 // (src/kernels.cu:985-1119, u = (a * half_fov) * aspect); mode 1:
 // render_kernel_uint8_prebasis from basis [B,14] (src/kernels.cu:568-700,
 // u = a * scale_u); the smem kernel (kernels.cu:835-982) computes the same
-// image as mode 1.  Ids >= 13 show the sky (the reference would index past
-// BLOCK_COLORS_U8).
+// image as mode 1; mode 2: render_kernel_uint8_fp16 from cameras (mode 0's
+// image plane, binary16 DDA, src/kernels.cu:1127-1262).  Ids >= 13 show the
+// sky (the reference would index past BLOCK_COLORS_U8).
 void orc_render_flat(const int8_t* voxels, const float* cams, uint8_t* out, int mode, int B,
                      int X, int Y, int Z, int width, int height, int max_steps,
                      float fov_degrees, int32_t* hit_index, int nthreads) {
@@ -594,16 +650,17 @@ void orc_render_flat(const int8_t* voxels, const float* cams, uint8_t* out, int 
   parallel_for(B, nthreads, [&](int b) {
     const int8_t* vox = voxels + b * wxyz;
     float bs[14];
-    if (mode == 0) camera_basis_row(cams + b * 5, width, height, hf, bs);
+    if (mode != 1) camera_basis_row(cams + b * 5, width, height, hf, bs);
     else for (int k = 0; k < 13; ++k) bs[k] = cams[b * 14 + k];
     for (int py = 0; py < height; ++py) {
       for (int px = 0; px < width; ++px) {
         float a = 2.0f * (px + 0.5f) / width - 1.0f;
         float c = 1.0f - 2.0f * (py + 0.5f) / height;
-        float u = (mode == 0) ? a * hf * aspect : a * bs[11];
-        float v = (mode == 0) ? c * hf : c * bs[12];
+        float u = (mode != 1) ? a * hf * aspect : a * bs[11];
+        float v = (mode != 1) ? c * hf : c * bs[12];
         int32_t hi;
-        int blk = flat_pixel(vox, X, Y, Z, bs, u, v, max_steps, &hi);
+        int blk = (mode == 2) ? flat_pixel<Half>(vox, X, Y, Z, bs, u, v, max_steps, &hi)
+                              : flat_pixel<F32>(vox, X, Y, Z, bs, u, v, max_steps, &hi);
         const uint8_t* col = (blk >= 0 && blk < 13) ? BLOCK_COLORS_U8[blk] : SKY_COLOR_U8;
         int64_t o = (b * npix + (int64_t)py * width + px) * 3;
         for (int k = 0; k < 3; ++k) out[o + k] = col[k];
@@ -611,6 +668,17 @@ void orc_render_flat(const int8_t* voxels, const float* cams, uint8_t* out, int 
       }
     }
   });
+}
+
+// binary16 conversion, sum and order of n float pairs (the software binary16
+// used by mode 2 of orc_render_flat).
+void orc_f16(const float* a, const float* b, uint16_t* ha, uint16_t* sum, uint8_t* lt, int n) {
+  for (int i = 0; i < n; ++i) {
+    const Half x = Half::from_float(a[i]), y = Half::from_float(b[i]);
+    ha[i] = x.bits;
+    sum[i] = (x + y).bits;
+    lt[i] = (x < y) ? 1 : 0;
+  }
 }
 
 // Largest sample length a caller may pass (documentation aid for ctypes users).

@@ -34,6 +34,8 @@ CASES = {
                                 obs_dtype="uint8_minimal", episode_length=10), 24, 3, False),
     "ws32_48x32_prebasis": (dict(batch_size=6, world_size=32, render_width=48, render_height=32,
                                  obs_dtype="uint8_prebasis", episode_length=11), 24, 4, True),
+    "ws32_32px_fp16": (dict(batch_size=6, world_size=32, render_width=32, render_height=32,
+                            obs_dtype="uint8_fp16", episode_length=9), 20, 5, True),
 }
 
 

@@ -61,8 +61,8 @@ def test_null_and_range_errors():
     assert "non-NULL" in _err(L) and not out.value
     assert L.mc_env_step(None, v, 0, v, None, None) == _l.MC_E_INVALID
     assert L.mc_debug_sincosf(None, None, None, 4, None) == _l.MC_E_INVALID
-    # flat variants: obs modes 2/3 are valid, 4 is not; NULL basis is refused
-    assert L.mc_render(v, v, v, 4, 4, 8, 8, 8, 8, 8, 64, 32.0, 70.0, None) == _l.MC_E_INVALID
+    # flat variants: obs modes 2..4 are valid, 5 is not; NULL basis is refused
+    assert L.mc_render(v, v, v, 5, 4, 8, 8, 8, 8, 8, 64, 32.0, 70.0, None) == _l.MC_E_INVALID
     assert L.mc_launch_precompute_camera_basis(None, v, 4, 8, 8, 70.0, None) == _l.MC_E_INVALID
     assert L.mc_launch_render_uint8_prebasis(v, None, v, 4, 8, 8, 8, 64, None) == _l.MC_E_INVALID
     args = [v] * 21
@@ -96,8 +96,7 @@ def test_mineenv_argument_errors():
         minecu.MineEnv(batch_size=4, device="cpu")
     with pytest.raises(AssertionError):
         minecu.MineEnv(batch_size=4, obs_dtype="int16")
-    with pytest.raises(NotImplementedError):
-        minecu.MineEnv(batch_size=4, obs_dtype="uint8_fp16")
+
 
 
 def test_config_presets_match_reference():

@@ -1,6 +1,8 @@
 """GPU parity of the flat-colour observation variants (obs_dtype
-uint8_minimal / uint8_prebasis / uint8_prebasis_full / uint8_smem) and of
-the reference-named bindings that run them, against the CPU oracle."""
+uint8_minimal / uint8_prebasis / uint8_prebasis_full / uint8_smem /
+uint8_fp16) and of the reference-named bindings that run them, against the
+CPU oracle; the device binary16 arithmetic against the oracle's software
+binary16."""
 import ctypes
 
 import numpy as np
@@ -12,7 +14,8 @@ from parity_util import Stats, compare_env, compare_exact, compare_obs, run_pair
 
 pytestmark = pytest.mark.gpu
 
-FLAT_MODES = ("uint8_minimal", "uint8_prebasis", "uint8_prebasis_full", "uint8_smem")
+FLAT_MODES = ("uint8_minimal", "uint8_prebasis", "uint8_prebasis_full", "uint8_smem",
+              "uint8_fp16")
 
 
 def _envs(**kw):
@@ -57,7 +60,7 @@ def test_flat_env_matches_oracle(mode, ws, W, H):
     assert st.exact, st.summary()
 
 
-@pytest.mark.parametrize("mode", ("uint8_minimal", "uint8_prebasis"))
+@pytest.mark.parametrize("mode", ("uint8_minimal", "uint8_prebasis", "uint8_fp16"))
 def test_flat_eye_outside_world_and_odd_image(mode):
     """9^3 world: the spawn eye (y 11.6) is above the world, so rays start
     outside it; width 17 takes the per-pixel store path."""
@@ -102,7 +105,7 @@ def _ref_inputs(B, ws, seed):
 
 
 @pytest.mark.parametrize("fn", ["step_uint8_minimal", "step_uint8_prebasis",
-                                "step_uint8_prebasis_full", "step_uint8_smem"])
+                                "step_uint8_prebasis_full", "step_uint8_smem", "step_uint8_fp16"])
 def test_reference_flat_step_bindings(fn):
     """The reference-named _C step bindings against the oracle's kernel
     sequence: decode -> physics -> raycast -> reset -> (episode update) ->
@@ -125,8 +128,8 @@ def test_reference_flat_step_bindings(fn):
     kw = dict(target_block=3, spawn_x=8.0, spawn_y=10.0, spawn_z=8.0)
     head = (g["voxels"], g["positions"], g["velocities"], g["yaws"], g["pitches"],
             g["on_ground"], t(buttons), t(look), dr)
-    if fn == "step_uint8_minimal":
-        _C.step_uint8_minimal(*head, *dec, cams, obs, rew, W, H, **kw)
+    if fn in ("step_uint8_minimal", "step_uint8_fp16"):
+        getattr(_C, fn)(*head, *dec, cams, obs, rew, W, H, **kw)
     elif fn == "step_uint8_prebasis_full":
         _C.step_uint8_prebasis_full(*head, scd, *dec, cams, basis, obs, rew, W, H,
                                     episode_length=EL, **kw)
@@ -148,8 +151,8 @@ def test_reference_flat_step_bindings(fn):
     L.orc_reset(P(pos), P(vel), P(yaw), P(pitch), P(og), P(do_reset), B, 8.0, 10.0, 8.0)
     c = np.zeros((B, 5), np.float32)
     L.orc_update_cameras(P(c), P(pos), P(yaw), P(pitch), B)
-    if fn == "step_uint8_minimal":
-        ref_obs = oracle.render_flat(vox, c, W, H, "minimal")
+    if fn in ("step_uint8_minimal", "step_uint8_fp16"):
+        ref_obs = oracle.render_flat(vox, c, W, H, "minimal" if fn == "step_uint8_minimal" else "fp16")
     else:
         ref_basis = oracle.camera_basis(c, W, H)
         ref_obs = oracle.render_flat(vox, ref_basis, W, H, "prebasis")
@@ -160,7 +163,7 @@ def test_reference_flat_step_bindings(fn):
         compare_exact(k, g[k].cpu().numpy().view(np.int32), ref.view(np.int32), where)
     compare_exact("rewards", rew.cpu().numpy(), r, where)
     compare_exact("cameras", cams.cpu().numpy().view(np.int32), c.view(np.int32), where)
-    if fn != "step_uint8_minimal":
+    if fn not in ("step_uint8_minimal", "step_uint8_fp16"):
         compare_exact("basis", basis.cpu().numpy().view(np.int32), ref_basis.view(np.int32), where)
     if fn == "step_uint8_prebasis_full":
         compare_exact("do_reset", dr.cpu().numpy().astype(np.uint8), do_reset, where)
@@ -198,7 +201,33 @@ def test_basis_and_flat_renders():
     _lib.check(L.mc_launch_render_uint8_minimal(p(gv), p(gc), p(out), B, ws, W, H, 64, 32.0, 70.0,
                                                 torch.cuda.current_stream().cuda_stream))
     compare_obs(out.cpu().numpy(), oracle.render_flat(vox, cams, W, H, "minimal"), st, "minimal render")
+    out.zero_()
+    _lib.check(L.mc_launch_render_uint8_fp16(p(gv), p(gc), p(out), B, ws, W, H, 64, 32.0, 70.0,
+                                             torch.cuda.current_stream().cuda_stream))
+    compare_obs(out.cpu().numpy(), oracle.render_flat(vox, cams, W, H, "fp16"), st, "fp16 render")
     assert st.exact, st.summary()
+
+
+def test_device_binary16_matches_software_binary16():
+    """The GPU's native binary16 (conversion, sum, order) equals the
+    oracle's software binary16 on 4M float pairs spanning the binary16
+    range, including ties, subnormals and the overflow boundary."""
+    from minecu import _lib
+    from test_oracle_flat import _f16_cases
+    a, b = _f16_cases(1 << 22, 11)
+    b = np.abs(b)  # the DDA adds non-negative deltas
+    a = np.abs(a)
+    ha, s, lt = oracle.f16_ops(a, b)
+    ga, gb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    gha = torch.empty(a.size, dtype=torch.int16, device="cuda")
+    gs = torch.empty_like(gha)
+    glt = torch.empty(a.size, dtype=torch.uint8, device="cuda")
+    p = lambda x: ctypes.c_void_p(x.data_ptr())
+    _lib.check(_lib.load().mc_debug_f16(p(ga), p(gb), p(gha), p(gs), p(glt), a.size,
+                                        torch.cuda.current_stream().cuda_stream))
+    assert np.array_equal(gha.cpu().numpy().view(np.uint16), ha)
+    assert np.array_equal(gs.cpu().numpy().view(np.uint16), s)
+    assert np.array_equal(glt.cpu().numpy(), lt)
 
 
 def test_flat_groups_in_a_batch_equal_alone():

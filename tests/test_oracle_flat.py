@@ -1,8 +1,9 @@
 """Known answers for the oracle's flat-colour variants (obs_dtype
-uint8_minimal / uint8_prebasis / uint8_prebasis_full / uint8_smem,
-src/kernels.cu:479-1119): colour table, no shading, no fog and no
-view-distance limit, the max_steps limit, the camera basis, an independent
-float64 traversal, and the MineEnv orchestration of these modes."""
+uint8_minimal / uint8_prebasis / uint8_prebasis_full / uint8_smem /
+uint8_fp16, src/kernels.cu:479-1262): colour table, no shading, no fog and
+no view-distance limit, the max_steps limit, the camera basis, an
+independent float64 traversal, the software binary16 arithmetic against
+numpy, and the MineEnv orchestration of these modes."""
 import math
 
 import numpy as np
@@ -167,7 +168,8 @@ def test_env_flat_step_but_shaded_reset():
     uint8 variant and the step with the variant's flat kernel
     (minecu/__init__.py:304-466): sky is (135,206,234) after reset and
     (135,206,235) after a step."""
-    for dt in ("uint8_minimal", "uint8_prebasis", "uint8_prebasis_full", "uint8_smem"):
+    for dt in ("uint8_minimal", "uint8_prebasis", "uint8_prebasis_full", "uint8_smem",
+               "uint8_fp16"):
         env = oracle.OracleEnv(batch_size=2, world_size=16, render_width=16, render_height=16,
                                obs_dtype=dt)
         o0 = env.reset().copy()
@@ -175,7 +177,7 @@ def test_env_flat_step_but_shaded_reset():
         assert not ((o0.reshape(-1, 3) == SKY).all(axis=1)).any()
         o1, _, _ = env.step(np.zeros(2, np.int64))
         assert ((o1.reshape(-1, 3) == SKY).all(axis=1)).any()
-        if dt != "uint8_minimal":
+        if dt not in ("uint8_minimal", "uint8_fp16"):
             assert np.array_equal(env.basis, oracle.camera_basis(env.cameras, 16, 16))
 
 
@@ -194,3 +196,59 @@ def test_prebasis_full_episode_convention():
                            episode_length=0, obs_dtype="uint8_prebasis")
     env.reset()
     assert all(env.step(np.zeros(1, np.int64))[2][0] for _ in range(4))
+
+
+def _f16_cases(n, seed):
+    """float32 pairs over the binary16 range: subnormals, ties, the overflow
+    boundary 65504/65520, zeros, values far beyond the range."""
+    rng = np.random.default_rng(seed)
+    mags = np.exp2(rng.uniform(-27, 18, n)).astype(np.float32)
+    a = mags * rng.choice([-1, 1], n).astype(np.float32)
+    b = np.exp2(rng.uniform(-27, 18, n)).astype(np.float32)
+    # exact binary16 midpoints (ties) and boundaries
+    h = rng.integers(0, 0x7bff, n // 4).astype(np.uint16).view(np.float16).astype(np.float32)
+    nxt = (rng.integers(0, 0x7bff, n // 4).astype(np.uint16) + 1).view(np.float16).astype(np.float32)
+    a[: n // 4] = (h.astype(np.float64) + nxt.astype(np.float64)).astype(np.float32) / 2
+    special = np.float32([0.0, -0.0, 65504.0, 65519.0, 65520.0, 65536.0, 1e6, 5.96e-8, 2.98e-8,
+                          6.1035e-5, 1e30, 65504.0])
+    a[-len(special):] = special
+    b[-len(special):] = special[::-1]
+    return a, b
+
+
+def test_software_binary16_matches_numpy():
+    """Conversion, sum (exact in binary64, rounded once) and order against
+    numpy's binary16, whose float32/float64 -> float16 conversions round to
+    nearest even (IEEE 754)."""
+    a, b = _f16_cases(200000, 7)
+    ha, s, lt = oracle.f16_ops(a, b)
+    with np.errstate(over="ignore"):
+        na, nb = a.astype(np.float16), b.astype(np.float16)
+    assert np.array_equal(ha, na.view(np.uint16))
+    with np.errstate(invalid="ignore", over="ignore"):
+        exact = na.astype(np.float64) + nb.astype(np.float64)
+        ref = exact.astype(np.float16).view(np.uint16)
+    ok = ~np.isnan(exact)          # inf + -inf: any NaN (never formed by the DDA)
+    assert ok.mean() > 0.99
+    assert np.array_equal(s[ok], ref[ok])
+    assert np.array_equal(lt.astype(bool), na < nb)
+    # every finite binary16 value converts to itself
+    allh = np.arange(0, 0x7c00, dtype=np.uint16)
+    back, _, _ = oracle.f16_ops(allh.view(np.float16).astype(np.float32), np.zeros(allh.size, np.float32))
+    assert np.array_equal(back, allh)
+
+
+def test_fp16_variant_differs_only_by_precision():
+    """The binary16 DDA sees the same blocks as the binary32 one on most
+    pixels of a random scene, but not on all (its t values are coarser)."""
+    rng = np.random.default_rng(9)
+    B, n, W, H = 6, 16, 32, 32
+    vox = np.where(rng.random((B, n, n, n)) < 0.06, rng.integers(0, 13, (B, n, n, n)), AIR).astype(np.int8)
+    cams = np.zeros((B, 5), np.float32)
+    cams[:, :3] = rng.uniform(1, n - 1, (B, 3))
+    cams[:, 3] = rng.uniform(-3, 3, B)
+    cams[:, 4] = rng.uniform(-1.4, 1.4, B)
+    a = oracle.render_flat(vox, cams, W, H, "minimal")
+    h = oracle.render_flat(vox, cams, W, H, "fp16")
+    diff = (a != h).any(axis=-1).mean()
+    assert 0 < diff < 0.05, diff
