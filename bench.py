@@ -92,13 +92,14 @@ def algorithmic_bytes_per_env_step(w, obs_bytes):
     return w["W"] * w["H"] * 3 * obs_bytes + vox + 71 + regen
 
 
-def cpu_baseline(w, obs, seconds_budget=12.0):
+def cpu_baseline(w, obs, seconds_budget=15.0):
     """Time the scalar oracle on a bounded sample of the same workload."""
     import numpy as np
     import oracle
     threads = oracle.default_threads()
     B = min(w["batch"], 8192 if w["world"] <= 16 else 1024)
-    steps = w["episode_length"] or 60
+    # whole episodes (regeneration steps included) until the time budget
+    steps = 10 * (w["episode_length"] or 60)
     kw = dict(batch_size=B, world_size=w["world"], render_width=w["W"], render_height=w["H"],
               obs_dtype=obs, episode_length=w["episode_length"], nthreads=threads)
     env = oracle.OracleEnv(**kw)

@@ -423,6 +423,13 @@ int mc_server_autotune(mc_server* s, int steps, void* stream);
 
 /* Device evaluation of the shared sin/cos (mc_math.h) on n inputs. */
 int mc_debug_sincosf(const float* x, float* s, float* c, int n, void* stream);
+/* Scan of the ray-setup arithmetic (the cores of correctly rounded 1/x, sqrt
+ * and n/d used on the ray fast path) against the full IEEE operations on the
+ * device.  mode 0: reciprocals of float bit patterns [begin, begin+count) with
+ * |x| in [2^-90, 2^90]; mode 1: square roots of patterns in [2^-60, 2^60];
+ * mode 2: count hashed quotients.  out (device, 3 x u32, zeroed by the caller):
+ * += inputs checked, += mismatches, one mismatching input. */
+int mc_debug_fast_math(int mode, uint32_t begin, uint32_t count, uint32_t* out, void* stream);
 /* Device binary16 arithmetic of the half-precision DDA on n float pairs:
  * ha = half(a), sum = half(a) + half(b) (binary16 bits), lt = half(a) < half(b). */
 int mc_debug_f16(const float* a, const float* b, uint16_t* ha, uint16_t* sum, uint8_t* lt, int n,

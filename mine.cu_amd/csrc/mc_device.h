@@ -136,6 +136,13 @@ struct Params {
   // Flat-colour variants: tan of the half field of view with the flat
   // kernels' degree conversion, and that times the aspect (prebasis scale_u).
   float flat_hf, flat_su;
+  // view_distance is a power of two: 1 / view_distance is exact and t * inv_vd
+  // equals t / view_distance for every t.
+  int vd_pow2;
+  float inv_vd;
+  // The world's diagonal is well inside view_distance: an in-world voxel's
+  // entry distance t (at most the diagonal, plus rounding) never reaches it.
+  int vd_far;
   // [B, 14] camera basis (src/kernels.cu:503-561; NULL: none).  Step items
   // write it in the prebasis modes; render-basis items read it.
   float* basis;
@@ -452,6 +459,88 @@ __device__ __forceinline__ void solid_extent(const Params& p, const int8_t* worl
 }
 
 // ---------------------------------------------------------------------------
+// Correctly rounded sqrt, division and reciprocal for camera rays.
+//
+// The compiler's IEEE sequences (-fhip-fp32-correctly-rounded-divide-sqrt)
+// wrap a core -- v_sqrt plus a one-ulp residual test; v_rcp, one Newton step
+// and two FMA remainder corrections -- in operand scaling and special-case
+// fix-ups (v_div_scale / v_div_fmas / v_div_fixup, v_cmp_class).  For normal
+// operands whose quotient stays normal and whose exponents differ by less
+// than 96 the scaling and fix-ups are identities, so the core alone returns
+// the same bits.  ray_dir() checks that range per ray (len^2 in
+// [2^-60, 2^60], every component 0 or at least 2^-60 in magnitude) and falls
+// back to the full IEEE operations otherwise.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float sqrt_core(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __int_as_float(__float_as_int(s) - 1);
+  const float sp = __int_as_float(__float_as_int(s) + 1);
+  const float rm = fmaf(-sm, s, x);
+  const float rp = fmaf(-sp, s, x);
+  float r = (rm <= 0.0f) ? sm : s;
+  r = (rp > 0.0f) ? sp : r;
+  return r;
+}
+// Reciprocal estimate of d refined by one Newton step (shared by every
+// quotient with denominator d).
+__device__ __forceinline__ float recip_core(float d) {
+  const float r = __builtin_amdgcn_rcpf(d);
+  const float e = fmaf(-d, r, 1.0f);
+  return fmaf(e, r, r);
+}
+__device__ __forceinline__ float div_core(float n, float d, float r) {
+  float q = n * r;
+  float rem = fmaf(-d, q, n);
+  q = fmaf(rem, r, q);
+  rem = fmaf(-d, q, n);
+  return fmaf(rem, r, q);
+}
+
+// Unit ray direction and per-axis reciprocals (big where a component is 0).
+// RSQRT: the flat kernels' normalisation, a product with 1 / sqrt(len^2);
+// otherwise the shaded kernels' division by sqrt(len^2).  A zero component
+// may come out as -0 instead of +0 on the fast path; its only uses are
+// comparisons with 0 and the reciprocal's zero test, where the two are equal.
+template <bool RSQRT>
+__device__ __forceinline__ void ray_dir(float& x, float& y, float& z, float& ix, float& iy,
+                                        float& iz, float big) {
+  const float len2 = x * x + y * y + z * z;
+  auto ok = [](float c) { return c == 0.0f || fabsf(c) >= 0x1p-60f; };
+  if (len2 >= 0x1p-60f && len2 <= 0x1p60f && ok(x) && ok(y) && ok(z)) {
+    const float len = sqrt_core(len2);
+    const float rl = recip_core(len);
+    if (RSQRT) {
+      const float k = div_core(1.0f, len, rl);
+      x *= k;
+      y *= k;
+      z *= k;
+    } else {
+      x = div_core(x, len, rl);
+      y = div_core(y, len, rl);
+      z = div_core(z, len, rl);
+    }
+    ix = (x != 0.0f) ? div_core(1.0f, x, recip_core(x)) : big;
+    iy = (y != 0.0f) ? div_core(1.0f, y, recip_core(y)) : big;
+    iz = (z != 0.0f) ? div_core(1.0f, z, recip_core(z)) : big;
+  } else {
+    const float len = sqrtf(len2);
+    if (RSQRT) {
+      const float k = 1.0f / len;
+      x *= k;
+      y *= k;
+      z *= k;
+    } else {
+      x /= len;
+      y /= len;
+      z /= len;
+    }
+    ix = (x != 0.0f) ? 1.0f / x : big;
+    iy = (y != 0.0f) ? 1.0f / y : big;
+    iz = (z != 0.0f) ? 1.0f / z : big;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // One ray: AABB cull + DDA + face shade + fog (src/kernels.cu:51-79, 354-469).
 // Writes the shaded colour in [0,1] to rgb.
 //
@@ -478,15 +567,11 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, 
   float ray_x = f.fx + u * f.rx + v * f.ux;
   float ray_y = f.fy + v * f.uy;
   float ray_z = f.fz + u * f.rz + v * f.uz;
-  const float ray_len = sqrtf(ray_x * ray_x + ray_y * ray_y + ray_z * ray_z);
-  ray_x /= ray_len;
-  ray_y /= ray_len;
-  ray_z /= ray_len;
-  // One reciprocal per axis serves both the slab test and the DDA: the
-  // reference computes the same expression twice (kernels.cu:57-59, 395-397).
-  const float inv_x = (ray_x != 0.0f) ? 1.0f / ray_x : 1e30f;
-  const float inv_y = (ray_y != 0.0f) ? 1.0f / ray_y : 1e30f;
-  const float inv_z = (ray_z != 0.0f) ? 1.0f / ray_z : 1e30f;
+  // Normalise by sqrt(len^2); one reciprocal per axis serves both the slab
+  // test and the DDA: the reference computes the same expression twice
+  // (kernels.cu:57-59, 395-397).
+  float inv_x, inv_y, inv_z;
+  ray_dir<false>(ray_x, ray_y, ray_z, inv_x, inv_y, inv_z, 1e30f);
   const float vd = p.view_distance;
   const int max_steps = p.max_steps;
   const int X = p.X, Y = p.Y, Z = p.Z, XZ = X * Z;
@@ -516,9 +601,10 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, 
     constexpr uint32_t kGuard = (1u << 9) | (1u << 19) | (1u << 29);
     uint32_t pos = (uint32_t)(tx + 512 - lim_x) | ((uint32_t)(ty + 512 - lim_y) << 10) |
                    ((uint32_t)(tz + 512 - lim_z) << 20);
-    int idx = f.ivy * XZ + f.ivx * Z + f.ivz;
+    // LDS address of the current voxel.
+    const int8_t* vp = world + (f.ivy * XZ + f.ivx * Z + f.ivz);
     bool out = (pos & kGuard) != 0u;
-    int blk = (int)world[idx];
+    int blk = (int)*vp;
     bool lim = (max_steps <= 0) | (t >= vd);
     bool last_x = false, last_y = false;
     int step = 0;
@@ -533,12 +619,12 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, 
       tm_x = cx ? tm_x + td_x : tm_x;
       tm_y = cy ? tm_y + td_y : tm_y;
       tm_z = cz ? tm_z + td_z : tm_z;
-      idx += cx ? dix : (cy ? diy : diz);
+      vp += cx ? dix : (cy ? diy : diz);
       pos += cx ? 1u : (cy ? (1u << 10) : (1u << 20));
       out = (pos & kGuard) != 0u;
-      blk = (int)world[out ? 0 : idx];
+      blk = (int)*vp;  // one step past the edge at most: a margin byte
       ++step;
-      lim = (step >= max_steps) | (t >= vd);
+      lim = (step >= max_steps) | (!p.vd_far & (t >= vd));
       last_x = cx;
       last_y = cy;
     }
@@ -597,7 +683,8 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, 
     if (hit_face == 0) shade = 0.8f;
     else if (hit_face == 2) shade = 0.9f;
     else if (hit_face == 1 && step_y < 0) shade = 0.6f;
-    float fog = fminf(t / vd, 1.0f);
+    // t / vd, or t * (1 / vd) when vd is a power of two (the same value).
+    float fog = fminf(p.vd_pow2 ? t * p.inv_vd : t / vd, 1.0f);
     fog = fog * fog;
     const float keep = 1.0f - fog;
 #pragma unroll
@@ -630,13 +717,8 @@ __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, float
   float ray_x = f.fx + u * f.rx + v * f.ux;
   float ray_y = f.fy + v * f.uy;
   float ray_z = f.fz + u * f.rz + v * f.uz;
-  const float rl = 1.0f / sqrtf(ray_x * ray_x + ray_y * ray_y + ray_z * ray_z);
-  ray_x *= rl;
-  ray_y *= rl;
-  ray_z *= rl;
-  const float inv_x = (ray_x != 0.0f) ? 1.0f / ray_x : kInf;
-  const float inv_y = (ray_y != 0.0f) ? 1.0f / ray_y : kInf;
-  const float inv_z = (ray_z != 0.0f) ? 1.0f / ray_z : kInf;
+  float inv_x, inv_y, inv_z;
+  ray_dir<true>(ray_x, ray_y, ray_z, inv_x, inv_y, inv_z, kInf);
   const int max_steps = p.max_steps;
   const int X = p.X, Y = p.Y, Z = p.Z, XZ = X * Z;
   const int step_x = (ray_x >= 0) ? 1 : -1;
@@ -660,9 +742,10 @@ __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, float
     constexpr uint32_t kGuard = (1u << 9) | (1u << 19) | (1u << 29);
     uint32_t pos = (uint32_t)(tx + 512 - lim_x) | ((uint32_t)(ty + 512 - lim_y) << 10) |
                    ((uint32_t)(tz + 512 - lim_z) << 20);
-    int idx = f.ivy * XZ + f.ivx * Z + f.ivz;
+    // LDS address of the current voxel.
+    const int8_t* vp = world + (f.ivy * XZ + f.ivx * Z + f.ivz);
     bool out = (pos & kGuard) != 0u;
-    int blk = (int)world[idx];
+    int blk = (int)*vp;
     int step = 0;
     bool lim = max_steps <= 0;
     while (!(lim | out | (blk >= 0))) {
@@ -672,10 +755,10 @@ __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, float
       tm_x = cx ? tm_x + td_x : tm_x;
       tm_y = cy ? tm_y + td_y : tm_y;
       tm_z = cz ? tm_z + td_z : tm_z;
-      idx += cx ? dix : (cy ? diy : diz);
+      vp += cx ? dix : (cy ? diy : diz);
       pos += cx ? 1u : (cy ? (1u << 10) : (1u << 20));
       out = (pos & kGuard) != 0u;
-      blk = (int)world[out ? 0 : idx];
+      blk = (int)*vp;  // one step past the edge at most: a margin byte
       ++step;
       lim = step >= max_steps;
     }
@@ -739,7 +822,7 @@ __device__ __forceinline__ float v_of(const Params& p, int m, int py) {
 
 // Per-workgroup tables of u (per column) and v (per row), filled before the
 // first barrier of an item when the image fits them.
-constexpr int kMaxUV = 256;
+constexpr int kMaxUV = 128;
 __device__ __forceinline__ bool uv_tabled(const Params& p) { return p.W <= kMaxUV && p.H <= kMaxUV; }
 __device__ __forceinline__ void fill_uv(const Params& p, int m, float* su, float* sv, int tid,
                                         int nt) {
@@ -767,44 +850,52 @@ __device__ __forceinline__ void fill_rgb8(uint8_t* out, int npix, uint32_t rgb, 
   }
 }
 
+// Pixels are traced in 8x8 tiles, one pixel per lane of a wave per tile, and
+// a wave's loop over a tile ends when its longest ray ends: neighbouring
+// pixels have rays of similar length, so square tiles waste fewer lane-steps
+// than rows (measured on falling-agent frames at 16^3 / 32x32: 76% of the
+// lane-steps useful with 8x8 tiles, 65% with 4-pixel quads spread over 8 rows).
+// Tile t of an env goes to its wave t mod waves-per-env.
+
+// Store packed 0x00BBGGRR at pixel p of a uint8 RGB image: a short and a byte,
+// the short at the even byte offset.
+__device__ __forceinline__ void store_rgb8(uint8_t* out, int64_t p, uint32_t c) {
+  const int64_t o = p * 3;
+  typedef MC_GLOBAL uint16_t gu16;
+  typedef MC_GLOBAL uint8_t gu8;
+  if ((p & 1) == 0) {
+    *(gu16*)(out + o) = (uint16_t)c;
+    *(gu8*)(out + o + 2) = (uint8_t)(c >> 16);
+  } else {
+    *(gu8*)(out + o) = (uint8_t)c;
+    *(gu16*)(out + o + 1) = (uint16_t)(c >> 8);
+  }
+}
+
+// Tile loop over env b's image: fn(px, py) for every pixel, one per lane.
+template <typename PixelFn>
+__device__ __forceinline__ void for_tiles(const Params& p, int tid, int nt, PixelFn fn) {
+  const int lane = tid & (kWave - 1), wv = tid >> 6, nwv = nt >> 6;
+  const int tx = lane & 7, ty = lane >> 3;
+  const int tcols = (p.W + 7) >> 3, ntiles = tcols * ((p.H + 7) >> 3);
+  for (int t = wv; t < ntiles; t += nwv) {
+    const int tr = t / tcols, tc = t - tr * tcols;
+    const int py = tr * 8 + ty, px = tc * 8 + tx;
+    if (py < p.H && px < p.W) fn(px, py);
+  }
+}
+
 // uint8 image of env b from a per-pixel colour function returning packed
-// 0x00BBGGRR: quads of 4 pixels of one row per lane with three dword
-// stores, or single pixels when W % 4 != 0.
+// 0x00BBGGRR.
 template <typename ColorFn>
 __device__ __forceinline__ void render_rgb8(const Params& p, int b, int uvm, int tid, int nt,
                                             const float* su, const float* sv, ColorFn color) {
-  const int W = p.W, H = p.H, npix = W * H;
-  uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
+  uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * p.W * p.H * 3;
   const bool tab = uv_tabled(p);
-  if ((W & 3) == 0) {
-    // Quad q covers pixels 4q..4q+3 of one row; (row, quad column) advance
-    // incrementally instead of dividing per quad.
-    const int qpr = W >> 2, drow = nt / qpr, dcol = nt - drow * qpr;
-    int py = tid / qpr, qx = tid - py * qpr;
-    for (int q = tid; q < (npix >> 2); q += nt) {
-      const int px0 = qx << 2, p0 = py * W + px0;
-      const float v = tab ? sv[py] : v_of(p, uvm, py);
-      qx += dcol;
-      py += drow;
-      if (qx >= qpr) { qx -= qpr; ++py; }
-      auto c = [&](int j) -> uint32_t {
-        return color(tab ? su[px0 + j] : u_of(p, uvm, px0 + j), v);
-      };
-      const uint32_t c0 = c(0), c1 = c(1), c2 = c(2), c3 = c(3);
-      guint32* o = (guint32*)(out + (int64_t)p0 * 3);
-      o[0] = c0 | (c1 << 24);
-      o[1] = (c1 >> 8) | (c2 << 16);
-      o[2] = (c2 >> 16) | (c3 << 8);
-    }
-  } else {
-    for (int pi = tid; pi < npix; pi += nt) {
-      const int py = pi / W, px = pi - py * W;
-      const uint32_t c = color(tab ? su[px] : u_of(p, uvm, px), tab ? sv[py] : v_of(p, uvm, py));
-      out[(int64_t)pi * 3 + 0] = (uint8_t)c;
-      out[(int64_t)pi * 3 + 1] = (uint8_t)(c >> 8);
-      out[(int64_t)pi * 3 + 2] = (uint8_t)(c >> 16);
-    }
-  }
+  for_tiles(p, tid, nt, [&](int px, int py) {
+    const uint32_t c = color(tab ? su[px] : u_of(p, uvm, px), tab ? sv[py] : v_of(p, uvm, py));
+    store_rgb8(out, (int64_t)py * p.W + px, c);
+  });
 }
 
 // Flat-colour uint8 render of env b (src/kernels.cu:985-1119 minimal,
@@ -835,7 +926,7 @@ template <int OBS, bool LIFT>
 __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t* world,
                                            const float* pal, const Frame& f, int tid, int nt,
                                            const float* su, const float* sv) {
-  const int W = p.W, H = p.H, npix = W * H;
+  const int W = p.W, npix = W * p.H;
   const bool tab = uv_tabled(p);
   auto ucol = [&](int px) { return tab ? su[px] : u_of(p, kUvShaded, px); };
   auto vrow = [&](int py) { return tab ? sv[py] : v_of(p, kUvShaded, py); };
@@ -861,45 +952,19 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
     render_rgb8(p, b, kUvShaded, tid, nt, su, sv, color);
   } else {
     float* out = reinterpret_cast<float*>(p.obs) + (int64_t)b * npix * 3;
-    if ((W & 3) == 0) {
-      const int qpr = W >> 2, drow = nt / qpr, dcol = nt - drow * qpr;
-      int py = tid / qpr, qx = tid - py * qpr;
-      for (int q = tid; q < (npix >> 2); q += nt) {
-        const int px0 = qx << 2, p0 = py * W + px0;
-        const float v = vrow(py);
-        qx += dcol;
-        py += drow;
-        if (qx >= qpr) { qx -= qpr; ++py; }
-        float vals[12];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const float u = ucol(px0 + j);
-          if (f.fill_block >= 0) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) vals[j * 3 + c] = pal[f.fill_block * 3 + c];
-          } else {
-            trace<LIFT>(p, f, u, v, world, pal, &vals[j * 3]);
-          }
-        }
-        gfloat4* o = (gfloat4*)(out + (int64_t)p0 * 3);
-        o[0] = v4f{vals[0], vals[1], vals[2], vals[3]};
-        o[1] = v4f{vals[4], vals[5], vals[6], vals[7]};
-        o[2] = v4f{vals[8], vals[9], vals[10], vals[11]};
+    typedef MC_GLOBAL float gf32;
+    for_tiles(p, tid, nt, [&](int px, int py) {
+      float rgb[3];
+      if (f.fill_block >= 0) {
+        for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
+      } else {
+        trace<LIFT>(p, f, ucol(px), vrow(py), world, pal, rgb);
       }
-    } else {
-      for (int pi = tid; pi < npix; pi += nt) {
-        const int py = pi / W, px = pi - py * W;
-        const float v = vrow(py);
-        const float u = ucol(px);
-        float rgb[3];
-        if (f.fill_block >= 0) {
-          for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
-        } else {
-          trace<LIFT>(p, f, u, v, world, pal, rgb);
-        }
-        for (int c = 0; c < 3; ++c) out[(int64_t)pi * 3 + c] = rgb[c];
-      }
-    }
+      gf32* o = (gf32*)(out + ((int64_t)py * W + px) * 3);
+      o[0] = rgb[0];
+      o[1] = rgb[1];
+      o[2] = rgb[2];
+    });
   }
 }
 
@@ -1079,10 +1144,13 @@ __device__ __forceinline__ Frame env_update(const Params& p, int b, int8_t* worl
       p.step_count[b] = nsc;
       p.do_reset[b] = done ? 1 : 0;
       if (p.dones) p.dones[b] = done ? 1 : 0;
-      // Idempotent 0 -> 1 store: racing workgroups all write the same value.
+      // Idempotent 0 -> 1 store: racing workgroups all write the same value,
+      // and only a later launch reads it.  A plain store merges in each XCD's
+      // L2 and is written back when the launch ends; an agent-scope atomic
+      // store writes through, and 32K envs setting the same word in one step
+      // serialised for about 0.7 ms.
       if (p.ring && p.episode_length >= 0 && nsc >= p.episode_length)
-        __hip_atomic_store(&p.ring[(p.slot + 1) % 3], 1u, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        p.ring[(p.slot + 1) % 3] = 1u;
     }
     if (p.forward_in) {
       p.forward_in[b] = fwd;
@@ -1142,8 +1210,8 @@ __device__ __forceinline__ Frame reset_update(const Params& p, int b, const int8
       p.step_count[b] = 0;
     }
     const int32_t sc = r ? 0 : p.step_count[b];
-    if (p.episode_length >= 0 && sc >= p.episode_length)
-      __hip_atomic_store(&p.ring[p.slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Same idempotent flag store as the step's (read only by later launches).
+    if (p.episode_length >= 0 && sc >= p.episode_length) p.ring[p.slot] = 1u;
     p.cameras[b * 5 + 0] = x;
     p.cameras[b * 5 + 1] = y;
     p.cameras[b * 5 + 2] = z;
@@ -1206,6 +1274,15 @@ enum ObsMode { kObsU8 = 0, kObsF32 = 1, kObsU8Minimal = 2, kObsU8Prebasis = 3, k
 __host__ __device__ __forceinline__ int world_stride_bytes(int X, int Y, int Z) {
   return (X * Y * Z + 15) & ~15;
 }
+// A workgroup's worlds sit between two margins of one y-layer each, so the
+// in-world DDA may read the voxel one step past its world's edge (which it
+// then discards) without an address select.
+__host__ __device__ __forceinline__ int world_margin_bytes(int X, int Z) {
+  return (X * Z + 15) & ~15;
+}
+__host__ __device__ __forceinline__ int item_lds_bytes(int epw, int X, int Y, int Z) {
+  return epw * world_stride_bytes(X, Y, Z) + 2 * world_margin_bytes(X, Z);
+}
 
 template <int NT, int KIND, bool LIFT>
 __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, int wpe,
@@ -1219,7 +1296,7 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   const int b = b0 + e;
   const bool active = (e < epw) && (b < p.B);
   const int nvox = p.X * p.Y * p.Z;
-  int8_t* world = lds + e * world_stride_bytes(p.X, p.Y, p.Z);
+  int8_t* world = lds + world_margin_bytes(p.X, p.Z) + e * world_stride_bytes(p.X, p.Y, p.Z);
   int8_t* gworld = p.voxels + (int64_t)b * nvox;
   bool any_done = false;
   if (KIND == kItemStep && p.done_mode == 0 && p.ring) {
@@ -1500,6 +1577,59 @@ __global__ void k_debug_f16(const float* a, const float* b, uint16_t* ha, uint16
   ha[i] = __builtin_bit_cast(uint16_t, x);
   sum[i] = __builtin_bit_cast(uint16_t, z);
   lt[i] = (x < y) ? 1 : 0;
+}
+
+// Scan of the ray-setup arithmetic cores against the full IEEE operations
+// (test hook).  mode 0: 1 / x for the float bit patterns begin .. begin +
+// count - 1 with |x| in [2^-90, 2^90]; mode 1: sqrt(x) for the patterns with x
+// in [2^-60, 2^60]; mode 2: n / d for count hashed pairs, |d| in [2^-30, 2^30]
+// and n = 0 or |n| in [2^-60, 2^30].  out[0] += inputs checked, out[1] +=
+// mismatches, out[2] = one mismatching input (bits; mode 2: the denominator).
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+  return x;
+}
+__global__ void k_debug_fast_math(int mode, uint32_t begin, uint32_t count, uint32_t* out) {
+  uint32_t checked = 0, bad = 0, bad_in = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
+    const uint32_t bits = begin + (uint32_t)k;
+    if (mode == 0) {
+      const float x = __uint_as_float(bits);
+      const float ax = fabsf(x);
+      if (!(ax >= 0x1p-90f && ax <= 0x1p90f)) continue;
+      const float f = div_core(1.0f, x, recip_core(x));
+      const float g = 1.0f / x;
+      ++checked;
+      if (__float_as_uint(f) != __float_as_uint(g)) { ++bad; bad_in = bits; }
+    } else if (mode == 1) {
+      const float x = __uint_as_float(bits);
+      if (!(x >= 0x1p-60f && x <= 0x1p60f)) continue;
+      const float f = sqrt_core(x);
+      const float g = sqrtf(x);
+      ++checked;
+      if (__float_as_uint(f) != __float_as_uint(g)) { ++bad; bad_in = bits; }
+    } else {
+      const uint32_t h1 = mix32(bits * 2u + 1u), h2 = mix32(bits * 2u + 2u);
+      // |d| in [2^-30, 2^30): exponent field 97..156; n: 67..156, or 0
+      const float d = __uint_as_float((h1 & 0x807fffffu) | ((97u + (h1 >> 23) % 60u) << 23));
+      const float n = ((h2 & 255u) == 0u) ? 0.0f
+                      : __uint_as_float((h2 & 0x807fffffu) | ((67u + (h2 >> 23) % 90u) << 23));
+      const float f = div_core(n, d, recip_core(d));
+      const float g = n / d;
+      ++checked;
+      // The cores may return -0 for a zero quotient where IEEE gives +0.
+      if (__float_as_uint(f) != __float_as_uint(g) && !(f == 0.0f && g == 0.0f)) {
+        ++bad;
+        bad_in = __float_as_uint(d);
+      }
+    }
+  }
+  if (checked) atomicAdd(&out[0], checked);
+  if (bad) {
+    atomicAdd(&out[1], bad);
+    out[2] = bad_in;
+  }
 }
 
 __global__ void k_debug_sincos(const float* x, float* s, float* c, int n) {

@@ -132,10 +132,12 @@ def test_buttons_and_look_continuous():
     print(st.summary())
 
 
-@pytest.mark.parametrize("dims,img", [((24, 16, 20), (30, 20)), ((9, 9, 9), (17, 11))])
+@pytest.mark.parametrize("dims,img", [((24, 16, 20), (30, 20)), ((9, 9, 9), (17, 11)),
+                                      ((16, 16, 16), (20, 12))])
 def test_odd_shapes(dims, img):
     """Non-cubic world (Z not a multiple of 16), world size not a multiple of
-    16 bytes, image width not a multiple of 4."""
+    16 bytes, image width not a multiple of 4, image sides that end in half
+    an 8x8 tile."""
     X, Y, Z = dims
     W, H = img
     env, orc = _envs(batch_size=24, world_x=X, world_y=Y, world_z=Z, render_width=W,

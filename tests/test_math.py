@@ -1,9 +1,12 @@
 """The shared IEEE numerics (mc_math.h) against the host libm, which is
 independent of it: sin/cos must be within 1 ulp of libm's binary64 result
-rounded to binary32 everywhere, and in practice equal to it."""
+rounded to binary32 everywhere, and in practice equal to it.  On the GPU: the
+ray fast path's sqrt / reciprocal / quotient cores against the full IEEE
+operations."""
 import math
 
 import numpy as np
+import pytest
 
 import oracle
 
@@ -64,3 +67,31 @@ def test_half_fov_tangent():
         ref = np.float32(math.tan(float(half)))
         got = np.float32(oracle.half_fov_tan(fov))
         assert _ulp(np.array([got]), np.array([ref]))[0] <= 1, fov
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,ranges", [
+    # every float with |x| in [2^-90, 2^90] (both signs)
+    (0, [(0x12800000, 0x6C800000 - 0x12800000), (0x92800000, 0x6C800000 - 0x12800000)]),
+    # every float in [2^-60, 2^60]
+    (1, [(0x21800000, 0x5D800000 - 0x21800000)]),
+    # 2^32 hashed quotient pairs
+    (2, [(0, 0xFFFFFFFF)]),
+])
+def test_ray_setup_cores_equal_ieee(mode, ranges):
+    """The cores of the ray fast path (sqrt, reciprocal, quotient without
+    the IEEE sequences' scaling and fix-ups) return the same bits as the
+    full IEEE operations over their whole guarded operand range (reciprocal,
+    sqrt: exhaustive) or on 2^32 hashed pairs (quotient)."""
+    import ctypes
+    import torch
+    from minecu import _lib
+    L = _lib.load()
+    out = torch.zeros(3, dtype=torch.int32, device="cuda")
+    for begin, count in ranges:
+        _lib.check(L.mc_debug_fast_math(mode, begin, count, ctypes.c_void_p(out.data_ptr()),
+                                        torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    checked, bad, bad_in = (int(v) & 0xFFFFFFFF for v in out.cpu().tolist())
+    assert checked > 10 ** 9
+    assert bad == 0, f"{bad} mismatches, e.g. input bits {bad_in:#010x}"
