@@ -173,6 +173,9 @@ struct Frame {
   // render from a stored camera basis.
   int flat_fill;
   float su, sv;
+  // Starting packed position of the in-world DDA per direction octant (see
+  // OctPos), written by lanes 0-7 of each of the env's waves before the render.
+  uint32_t pos0[8];
 };
 
 // Largest world dimension of the in-world DDA loop (10-bit packed position).
@@ -540,6 +543,38 @@ __device__ __forceinline__ void ray_dir(float& x, float& y, float& z, float& ix,
   }
 }
 
+// Starting packed position of the in-world DDA for each of the 8 direction
+// octants (bit 0: ray_x >= 0, bit 1: ray_y >= 0, bit 2: ray_z >= 0).  Field
+// per axis: the eye voxel counted in the direction of travel, biased by
+// 512 - limit so that bit 9 of the field sets on reaching the limit; the limit
+// is the world extent, or with the upward-ray exit (ray_y >= 0) the extent of
+// the solids at or above eye height.  Uniform per env: 8 words in the env's
+// LDS frame, read per ray by octant (kept out of scalar registers, which the
+// camera frame and loop masks already fill).
+struct OctPos {
+  const uint32_t* v;  // Frame::pos0 in LDS
+  __device__ __forceinline__ uint32_t pick(bool bx, bool by, bool bz) const {
+    return v[(bx ? 1 : 0) | (by ? 2 : 0) | (bz ? 4 : 0)];
+  }
+};
+
+template <bool LIFT>
+__device__ __forceinline__ uint32_t oct_pos(const Params& p, const Frame& f, int k) {
+  const int X = p.X, Y = p.Y, Z = p.Z;
+  const bool bx = k & 1, by = (k >> 1) & 1, bz = (k >> 2) & 1;
+  int lim_x = X, lim_y = Y, lim_z = Z;
+  if (LIFT && f.lift && by) {
+    lim_y = max(0, min(Y, f.hy1 + 1));
+    lim_x = max(0, bx ? min(X, f.hx1 + 1) : min(X, X - f.hx0));
+    lim_z = max(0, bz ? min(Z, f.hz1 + 1) : min(Z, Z - f.hz0));
+  }
+  const int tx = bx ? f.ivx : X - 1 - f.ivx;
+  const int ty = by ? f.ivy : Y - 1 - f.ivy;
+  const int tz = bz ? f.ivz : Z - 1 - f.ivz;
+  return (uint32_t)(tx + 512 - lim_x) | ((uint32_t)(ty + 512 - lim_y) << 10) |
+         ((uint32_t)(tz + 512 - lim_z) << 20);
+}
+
 // ---------------------------------------------------------------------------
 // One ray: AABB cull + DDA + face shade + fog (src/kernels.cu:51-79, 354-469).
 // Writes the shaded colour in [0,1] to rgb.
@@ -561,8 +596,8 @@ __device__ __forceinline__ void ray_dir(float& x, float& y, float& z, float& ix,
 // position, so the same test ends the ray there.
 // ---------------------------------------------------------------------------
 template <bool LIFT>
-__device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, float v,
-                                      const int8_t* __restrict__ world,
+__device__ __forceinline__ void trace(const Params& p, const Frame& f, const OctPos& op, float u,
+                                      float v, const int8_t* __restrict__ world,
                                       const float* __restrict__ pal, float rgb[3]) {
   float ray_x = f.fx + u * f.rx + v * f.ux;
   float ray_y = f.fy + v * f.uy;
@@ -587,20 +622,8 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, 
     float tm_x = ((ray_x >= 0) ? f.dxp : f.dxn) * inv_x;
     float tm_y = ((ray_y >= 0) ? f.dyp : f.dyn) * inv_y;
     float tm_z = ((ray_z >= 0) ? f.dzp : f.dzn) * inv_z;
-    // Per-axis limits in travel coordinates (voxels counted from the world
-    // face the ray moves away from).
-    int lim_x = X, lim_y = Y, lim_z = Z;
-    if (LIFT && f.lift && step_y > 0) {
-      lim_y = max(0, min(Y, f.hy1 + 1));
-      lim_x = max(0, (step_x > 0) ? min(X, f.hx1 + 1) : min(X, X - f.hx0));
-      lim_z = max(0, (step_z > 0) ? min(Z, f.hz1 + 1) : min(Z, Z - f.hz0));
-    }
-    const int tx = (step_x > 0) ? f.ivx : X - 1 - f.ivx;
-    const int ty = (step_y > 0) ? f.ivy : Y - 1 - f.ivy;
-    const int tz = (step_z > 0) ? f.ivz : Z - 1 - f.ivz;
     constexpr uint32_t kGuard = (1u << 9) | (1u << 19) | (1u << 29);
-    uint32_t pos = (uint32_t)(tx + 512 - lim_x) | ((uint32_t)(ty + 512 - lim_y) << 10) |
-                   ((uint32_t)(tz + 512 - lim_z) << 20);
+    uint32_t pos = op.pick(step_x > 0, step_y > 0, step_z > 0);
     // LDS address of the current voxel.
     const int8_t* vp = world + (f.ivy * XZ + f.ivx * Z + f.ivz);
     bool out = (pos & kGuard) != 0u;
@@ -711,8 +734,8 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, float u, 
 // can never come back, which ends it too.
 // ---------------------------------------------------------------------------
 template <bool LIFT, typename T>
-__device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, float u, float v,
-                                          const int8_t* __restrict__ world) {
+__device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, const OctPos& op,
+                                          float u, float v, const int8_t* __restrict__ world) {
   constexpr float kInf = (sizeof(T) == 2) ? 65504.0f : 1e30f;
   float ray_x = f.fx + u * f.rx + v * f.ux;
   float ray_y = f.fy + v * f.uy;
@@ -730,18 +753,8 @@ __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, float
     T tm_x = (T)(((ray_x >= 0) ? f.dxp : f.dxn) * inv_x);
     T tm_y = (T)(((ray_y >= 0) ? f.dyp : f.dyn) * inv_y);
     T tm_z = (T)(((ray_z >= 0) ? f.dzp : f.dzn) * inv_z);
-    int lim_x = X, lim_y = Y, lim_z = Z;
-    if (LIFT && f.lift && step_y > 0) {
-      lim_y = max(0, min(Y, f.hy1 + 1));
-      lim_x = max(0, (step_x > 0) ? min(X, f.hx1 + 1) : min(X, X - f.hx0));
-      lim_z = max(0, (step_z > 0) ? min(Z, f.hz1 + 1) : min(Z, Z - f.hz0));
-    }
-    const int tx = (step_x > 0) ? f.ivx : X - 1 - f.ivx;
-    const int ty = (step_y > 0) ? f.ivy : Y - 1 - f.ivy;
-    const int tz = (step_z > 0) ? f.ivz : Z - 1 - f.ivz;
     constexpr uint32_t kGuard = (1u << 9) | (1u << 19) | (1u << 29);
-    uint32_t pos = (uint32_t)(tx + 512 - lim_x) | ((uint32_t)(ty + 512 - lim_y) << 10) |
-                   ((uint32_t)(tz + 512 - lim_z) << 20);
+    uint32_t pos = op.pick(step_x > 0, step_y > 0, step_z > 0);
     // LDS address of the current voxel.
     const int8_t* vp = world + (f.ivy * XZ + f.ivx * Z + f.ivz);
     bool out = (pos & kGuard) != 0u;
@@ -904,15 +917,15 @@ __device__ __forceinline__ void render_rgb8(const Params& p, int b, int uvm, int
 // then the sky; ids >= 13 show the sky.
 template <bool LIFT, typename T>
 __device__ __forceinline__ void render_env_flat(const Params& p, int b, const int8_t* world,
-                                                const uint32_t* flat, const Frame& f, int uvm,
-                                                int tid, int nt, const float* su,
-                                                const float* sv) {
+                                                const uint32_t* flat, const Frame& f,
+                                                const OctPos& op, int uvm, int tid, int nt,
+                                                const float* su, const float* sv) {
   const int npix = p.W * p.H;
   uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
   const bool per_env_scale = (uvm == kUvRaw);
   auto color = [&](float u, float v) -> uint32_t {
     if (per_env_scale) { u = u * f.su; v = v * f.sv; }
-    const int h = (f.flat_fill >= 0) ? f.flat_fill : trace_flat<LIFT, T>(p, f, u, v, world);
+    const int h = (f.flat_fill >= 0) ? f.flat_fill : trace_flat<LIFT, T>(p, f, op, u, v, world);
     return flat[(h >= 0 && h < kFlatSky) ? h : kFlatSky];
   };
   if (f.flat_fill >= 0 && ((npix * 3) & 15) == 0) {
@@ -924,8 +937,8 @@ __device__ __forceinline__ void render_env_flat(const Params& p, int b, const in
 
 template <int OBS, bool LIFT>
 __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t* world,
-                                           const float* pal, const Frame& f, int tid, int nt,
-                                           const float* su, const float* sv) {
+                                           const float* pal, const Frame& f, const OctPos& op,
+                                           int tid, int nt, const float* su, const float* sv) {
   const int W = p.W, npix = W * p.H;
   const bool tab = uv_tabled(p);
   auto ucol = [&](int px) { return tab ? su[px] : u_of(p, kUvShaded, px); };
@@ -945,7 +958,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       if (f.fill_block >= 0) {
         for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
       } else {
-        trace<LIFT>(p, f, u, v, world, pal, rgb);
+        trace<LIFT>(p, f, op, u, v, world, pal, rgb);
       }
       return to_u8(rgb[0]) | (to_u8(rgb[1]) << 8) | (to_u8(rgb[2]) << 16);
     };
@@ -958,7 +971,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       if (f.fill_block >= 0) {
         for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
       } else {
-        trace<LIFT>(p, f, ucol(px), vrow(py), world, pal, rgb);
+        trace<LIFT>(p, f, op, ucol(px), vrow(py), world, pal, rgb);
       }
       gf32* o = (gf32*)(out + ((int64_t)py * W + px) * 3);
       o[0] = rgb[0];
@@ -1338,13 +1351,19 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     __syncthreads();
   }
   if (active) {
+    // Every wave of the env writes the same 8 octant words before it reads
+    // them; a wave's LDS operations complete in order.
+    if ((et & (kWave - 1)) < 8)
+      frames[e].pos0[et & (kWave - 1)] = oct_pos<LIFT>(p, frames[e], et & (kWave - 1));
+    __builtin_amdgcn_wave_barrier();
     const Frame f = uniform_frame(&frames[e]);
+    const OctPos op{frames[e].pos0};
     if (uvm != kUvShaded && obs_mode == kObsU8Fp16)
-      render_env_flat<LIFT, _Float16>(p, b, world, flat, f, uvm, et, ent, su, sv);
+      render_env_flat<LIFT, _Float16>(p, b, world, flat, f, op, uvm, et, ent, su, sv);
     else if (uvm != kUvShaded)
-      render_env_flat<LIFT, float>(p, b, world, flat, f, uvm, et, ent, su, sv);
-    else if (obs_mode == kObsF32) render_env<1, LIFT>(p, b, world, pal, f, et, ent, su, sv);
-    else render_env<0, LIFT>(p, b, world, pal, f, et, ent, su, sv);
+      render_env_flat<LIFT, float>(p, b, world, flat, f, op, uvm, et, ent, su, sv);
+    else if (obs_mode == kObsF32) render_env<1, LIFT>(p, b, world, pal, f, op, et, ent, su, sv);
+    else render_env<0, LIFT>(p, b, world, pal, f, op, et, ent, su, sv);
     if (KIND == kItemStep && any_done) store_fresh_world(p, gworld, nullptr, nvox, et, ent);
   }
 }
