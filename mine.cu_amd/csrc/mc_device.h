@@ -614,14 +614,16 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
   const int step_y = (ray_y >= 0) ? 1 : -1;
   const int step_z = (ray_z >= 0) ? 1 : -1;
   const float td_x = fabsf(inv_x), td_y = fabsf(inv_y), td_z = fabsf(inv_z);
-  const int dix = step_x * Z, diy = step_y * XZ, diz = step_z;
+  const int dix = (ray_x >= 0) ? Z : -Z, diy = (ray_y >= 0) ? XZ : -XZ, diz = step_z;
   int hit_block = -1;
   int hit_face = -1;
   float t = 0.0f;
   if (f.inside) {
-    float tm_x = ((ray_x >= 0) ? f.dxp : f.dxn) * inv_x;
-    float tm_y = ((ray_y >= 0) ? f.dyp : f.dyn) * inv_y;
-    float tm_z = ((ray_z >= 0) ? f.dzp : f.dzn) * inv_z;
+    // dxp = dxn + 1 exactly: both differences are exact (Sterbenz), as
+    // floor(e) <= e < floor(e) + 1.
+    float tm_x = (f.dxn + ((ray_x >= 0) ? 1.0f : 0.0f)) * inv_x;
+    float tm_y = (f.dyn + ((ray_y >= 0) ? 1.0f : 0.0f)) * inv_y;
+    float tm_z = (f.dzn + ((ray_z >= 0) ? 1.0f : 0.0f)) * inv_z;
     constexpr uint32_t kGuard = (1u << 9) | (1u << 19) | (1u << 29);
     uint32_t pos = op.pick(step_x > 0, step_y > 0, step_z > 0);
     // LDS address of the current voxel.
@@ -702,10 +704,10 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
     }
   }
   if (hit_block >= 0 && hit_block < 13) {
-    float shade = 1.0f;
-    if (hit_face == 0) shade = 0.8f;
-    else if (hit_face == 2) shade = 0.9f;
-    else if (hit_face == 1 && step_y < 0) shade = 0.6f;
+    const float shade = (hit_face == 0)                   ? 0.8f
+                        : (hit_face == 2)                 ? 0.9f
+                        : (hit_face == 1 && step_y < 0)   ? 0.6f
+                                                          : 1.0f;
     // t / vd, or t * (1 / vd) when vd is a power of two (the same value).
     float fog = fminf(p.vd_pow2 ? t * p.inv_vd : t / vd, 1.0f);
     fog = fog * fog;
@@ -748,11 +750,11 @@ __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, const
   const int step_y = (ray_y >= 0) ? 1 : -1;
   const int step_z = (ray_z >= 0) ? 1 : -1;
   const T td_x = (T)fabsf(inv_x), td_y = (T)fabsf(inv_y), td_z = (T)fabsf(inv_z);
-  const int dix = step_x * Z, diy = step_y * XZ, diz = step_z;
+  const int dix = (ray_x >= 0) ? Z : -Z, diy = (ray_y >= 0) ? XZ : -XZ, diz = step_z;
   if (f.inside) {
-    T tm_x = (T)(((ray_x >= 0) ? f.dxp : f.dxn) * inv_x);
-    T tm_y = (T)(((ray_y >= 0) ? f.dyp : f.dyn) * inv_y);
-    T tm_z = (T)(((ray_z >= 0) ? f.dzp : f.dzn) * inv_z);
+    T tm_x = (T)((f.dxn + ((ray_x >= 0) ? 1.0f : 0.0f)) * inv_x);
+    T tm_y = (T)((f.dyn + ((ray_y >= 0) ? 1.0f : 0.0f)) * inv_y);
+    T tm_z = (T)((f.dzn + ((ray_z >= 0) ? 1.0f : 0.0f)) * inv_z);
     constexpr uint32_t kGuard = (1u << 9) | (1u << 19) | (1u << 29);
     uint32_t pos = op.pick(step_x > 0, step_y > 0, step_z > 0);
     // LDS address of the current voxel.
@@ -872,17 +874,13 @@ __device__ __forceinline__ void fill_rgb8(uint8_t* out, int npix, uint32_t rgb, 
 
 // Store packed 0x00BBGGRR at pixel p of a uint8 RGB image: a short and a byte,
 // the short at the even byte offset.
-__device__ __forceinline__ void store_rgb8(uint8_t* out, int64_t p, uint32_t c) {
-  const int64_t o = p * 3;
+__device__ __forceinline__ void store_rgb8(uint8_t* out, int p, uint32_t c) {
   typedef MC_GLOBAL uint16_t gu16;
   typedef MC_GLOBAL uint8_t gu8;
-  if ((p & 1) == 0) {
-    *(gu16*)(out + o) = (uint16_t)c;
-    *(gu8*)(out + o + 2) = (uint8_t)(c >> 16);
-  } else {
-    *(gu8*)(out + o) = (uint8_t)c;
-    *(gu16*)(out + o + 1) = (uint16_t)(c >> 8);
-  }
+  const bool even = (p & 1) == 0;
+  const uint32_t o = (uint32_t)p * 3u;
+  *(gu16*)(out + (even ? o : o + 1u)) = (uint16_t)(even ? c : c >> 8);
+  *(gu8*)(out + (even ? o + 2u : o)) = (uint8_t)(even ? c >> 16 : c);
 }
 
 // Tile loop over env b's image: fn(px, py) for every pixel, one per lane.
@@ -907,7 +905,7 @@ __device__ __forceinline__ void render_rgb8(const Params& p, int b, int uvm, int
   const bool tab = uv_tabled(p);
   for_tiles(p, tid, nt, [&](int px, int py) {
     const uint32_t c = color(tab ? su[px] : u_of(p, uvm, px), tab ? sv[py] : v_of(p, uvm, py));
-    store_rgb8(out, (int64_t)py * p.W + px, c);
+    store_rgb8(out, py * p.W + px, c);
   });
 }
 
