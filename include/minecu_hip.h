@@ -333,6 +333,14 @@ int mc_env_sync(mc_env* env, void* stream);
 int mc_env_destroy(mc_env* env);
 int mc_env_step(mc_env* env, const void* actions, int action_kind, const float* look,
                 uint8_t* dones_out, void* stream);
+/* Graph mode (no reference counterpart; the reference captures its 6-launch
+ * sequence in a process-global CUDA graph, kernels.cu:2338-2382): with on = 1
+ * the group's episode ring slot moves to device memory and each mc_env_step
+ * launches the step plus a one-thread slot advance, so a stream capture of
+ * mc_env_step (e.g. torch.cuda.graph) replays correctly any number of times.
+ * on = 0 reads the slot back (synchronises `stream`).  Groups in graph mode
+ * cannot join mc_batch_step. */
+int mc_env_set_graph_mode(mc_env* env, int on, void* stream);
 int mc_env_reset(mc_env* env, const uint8_t* mask, void* stream);
 int mc_env_render(mc_env* env, void* stream);
 

@@ -212,9 +212,11 @@ struct mc_env {
   mc_config cfg;
   mc_state st;
   mc::Params base;
-  uint32_t* ring;     // device words: episode flag ring [0..2] + reset flag [3]
+  uint32_t* ring;     // device words: episode flag ring [0..2], reset flag [3],
+                      // device-resident slot [4] (graph mode)
   mc::Params* desc;   // device copy of `base` (with ring), read by batched launches
-  int slot;           // ring slot the next step launch reads
+  int slot;           // ring slot the next step launch reads (host-tracked mode)
+  int graph_mode;     // 1: the slot lives in ring[4] and each step advances it
   Shape shape;        // launch shape for this group (default or tuned)
 };
 
@@ -340,11 +342,13 @@ int mc_step(const mc_config* cfg, const mc_state* st, const void* actions, int a
 
 namespace {
 int reset_with_ring(const mc_config* cfg, const mc_state* st, const uint8_t* mask,
-                    uint32_t* ring, int slot, Shape shape, void* stream) {
+                    uint32_t* ring, int slot, const uint32_t* slot_dev, Shape shape,
+                    void* stream) {
   mc::Params p = make_params(cfg, st);
   p.mask = mask;
   p.ring = ring;
   p.slot = slot;
+  p.slot_dev = slot_dev;
   hipLaunchKernelGGL(mc::k_ring_setup, dim3(1), dim3(1024), 0, S(stream), p, 1);
   const int r = check_launch("reset pre-pass");
   if (r) return r;
@@ -359,7 +363,7 @@ int mc_env_reset_state(const mc_config* cfg, const mc_state* st, const uint8_t* 
   r = validate_state(st);
   if (r) return r;
   if (!workspace) return fail(MC_E_INVALID, "workspace is NULL");
-  return reset_with_ring(cfg, st, mask, reinterpret_cast<uint32_t*>(workspace), 0,
+  return reset_with_ring(cfg, st, mask, reinterpret_cast<uint32_t*>(workspace), 0, nullptr,
                          default_shape(cfg->width, cfg->height, cfg->world_y), stream);
 }
 
@@ -378,6 +382,7 @@ int mc_env_create(const mc_config* cfg, const mc_state* st, mc_env** out) {
   e->ring = nullptr;
   e->desc = nullptr;
   e->slot = 0;
+  e->graph_mode = 0;
   e->shape = default_shape(cfg->width, cfg->height, cfg->world_y);
   hipError_t he = hipMalloc(reinterpret_cast<void**>(&e->ring), MC_WORKSPACE_BYTES);
   if (he == hipSuccess) he = hipMemset(e->ring, 0, MC_WORKSPACE_BYTES);
@@ -427,6 +432,7 @@ int mc_env_sync(mc_env* env, void* stream) {
   mc::Params p = env->base;
   p.ring = env->ring;
   p.slot = env->slot;
+  if (env->graph_mode) p.slot_dev = env->ring + 4;
   hipLaunchKernelGGL(mc::k_ring_setup, dim3(1), dim3(1024), 0, S(stream), p, 0);
   return check_launch("episode sync");
 }
@@ -453,14 +459,42 @@ int mc_env_step(mc_env* env, const void* actions, int action_kind, const float* 
   p.dones = dones_out;
   p.ring = env->ring;
   p.slot = env->slot;
+  if (env->graph_mode) {
+    p.slot_dev = env->ring + 4;
+    int r = launch_env(mc::kItemStep, p, env->cfg.obs_mode, env->shape, S(stream));
+    if (r) return r;
+    hipLaunchKernelGGL(mc::k_slot_advance, dim3(1), dim3(1), 0, S(stream), env->ring + 4);
+    return check_launch("slot advance");
+  }
   const int r = launch_env(mc::kItemStep, p, env->cfg.obs_mode, env->shape, S(stream));
   if (r == MC_OK) env->slot = (env->slot + 1) % 3;
   return r;
 }
 
+int mc_env_set_graph_mode(mc_env* env, int on, void* stream) {
+  if (!env) return fail(MC_E_INVALID, "env is NULL");
+  if ((on != 0) == (env->graph_mode != 0)) return MC_OK;
+  if (on) {
+    hipLaunchKernelGGL(mc::k_slot_set, dim3(1), dim3(1), 0, S(stream), env->ring + 4,
+                       (uint32_t)env->slot);
+    const int r = check_launch("slot set");
+    if (r) return r;
+    env->graph_mode = 1;
+    return MC_OK;
+  }
+  uint32_t v = 0;
+  hipError_t he = hipMemcpyAsync(&v, env->ring + 4, sizeof v, hipMemcpyDeviceToHost, S(stream));
+  if (he == hipSuccess) he = hipStreamSynchronize(S(stream));
+  if (he != hipSuccess) return fail(MC_E_HIP, "slot read-back: %s", hipGetErrorString(he));
+  env->slot = (int)(v % 3u);
+  env->graph_mode = 0;
+  return MC_OK;
+}
+
 int mc_env_reset(mc_env* env, const uint8_t* mask, void* stream) {
   if (!env) return fail(MC_E_INVALID, "env is NULL");
-  return reset_with_ring(&env->cfg, &env->st, mask, env->ring, env->slot, env->shape, stream);
+  return reset_with_ring(&env->cfg, &env->st, mask, env->ring, env->slot,
+                         env->graph_mode ? env->ring + 4 : nullptr, env->shape, stream);
 }
 
 int mc_env_render(mc_env* env, void* stream) {
@@ -903,6 +937,8 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
       return fail(MC_E_INVALID, "group %d: look is NULL", i);
     for (int j = 0; j < i; ++j)
       if (envs[j] == envs[i]) return fail(MC_E_INVALID, "group %d listed twice", i);
+    if (envs[i]->graph_mode)
+      return fail(MC_E_UNSUPPORTED, "group %d is in graph mode (its slot is device-resident)", i);
     if (envs[i]->shape.wpe == 8) nt = 512;
     lift = lift || envs[i]->shape.lift;
   }
@@ -1043,7 +1079,9 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
     if (hipMemcpy(acts, ha.data(), ha.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
       return fail(MC_E_HIP, "autotune: action upload failed");
     int r = mc_regen_worlds(&c, s.voxels, nullptr, stream);
-    if (!r) r = reset_with_ring(&c, &s, nullptr, (uint32_t*)ring, 0, default_shape(c.width, c.height, c.world_y), stream);
+    if (!r)
+      r = reset_with_ring(&c, &s, nullptr, (uint32_t*)ring, 0, nullptr,
+                          default_shape(c.width, c.height, c.world_y), stream);
     mc::Params p = make_params(&c, &s);
     p.ring = (uint32_t*)ring;
     p.action_kind = MC_ACT_DISCRETE;

@@ -119,6 +119,10 @@ struct Params {
   // ring[3] is the reset path's "regenerate" flag.
   uint32_t* ring;
   int slot;
+  // Graph-replayable launches read the slot from device memory instead
+  // (NULL: use `slot`); the host advances it with k_slot_advance after each
+  // step launch, so a captured step replays with the right slot.
+  const uint32_t* slot_dev;
   // done_mode 0: done = step_count >= episode_length, with the MineEnv
   // bookkeeping (step_count, do_reset, dones, regeneration).  done_mode 1:
   // done = do_reset[b] as given, nothing else touched (launch_step_uint8).
@@ -1035,7 +1039,7 @@ __device__ __forceinline__ EnvIn load_env_in(const Params& p, int b) {
   return in;
 }
 
-__device__ __forceinline__ Frame env_update(const Params& p, int b, int8_t* world,
+__device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, int8_t* world,
                                             int8_t* gworld, bool any_done, const EnvIn& in) {
   const int lane = threadIdx.x & (kWave - 1);
   const int sc = in.sc;
@@ -1161,7 +1165,7 @@ __device__ __forceinline__ Frame env_update(const Params& p, int b, int8_t* worl
       // store writes through, and 32K envs setting the same word in one step
       // serialised for about 0.7 ms.
       if (p.ring && p.episode_length >= 0 && nsc >= p.episode_length)
-        p.ring[(p.slot + 1) % 3] = 1u;
+        p.ring[(slot + 1) % 3] = 1u;
     }
     if (p.forward_in) {
       p.forward_in[b] = fwd;
@@ -1197,7 +1201,8 @@ __device__ __forceinline__ void load_palette(float* pal, uint32_t* flat, int tid
 // Wave-0 logic of MineEnv.reset for env b (minecu/__init__.py:270-302):
 // respawn when masked (reset_kernel, kernels.cu:1775-1785), step_count = 0,
 // report "done on the next step" into the episode ring, refresh the camera.
-__device__ __forceinline__ Frame reset_update(const Params& p, int b, const int8_t* world) {
+__device__ __forceinline__ Frame reset_update(const Params& p, int slot, int b,
+                                              const int8_t* world) {
   const int lane = threadIdx.x & (kWave - 1);
   const bool r = p.mask ? (p.mask[b] != 0) : true;
   float x = p.positions[b * 3 + 0], y = p.positions[b * 3 + 1], z = p.positions[b * 3 + 2];
@@ -1222,7 +1227,7 @@ __device__ __forceinline__ Frame reset_update(const Params& p, int b, const int8
     }
     const int32_t sc = r ? 0 : p.step_count[b];
     // Same idempotent flag store as the step's (read only by later launches).
-    if (p.episode_length >= 0 && sc >= p.episode_length) p.ring[p.slot] = 1u;
+    if (p.episode_length >= 0 && sc >= p.episode_length) p.ring[slot] = 1u;
     p.cameras[b * 5 + 0] = x;
     p.cameras[b * 5 + 1] = y;
     p.cameras[b * 5 + 2] = z;
@@ -1309,10 +1314,11 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   const int nvox = p.X * p.Y * p.Z;
   int8_t* world = lds + world_margin_bytes(p.X, p.Z) + e * world_stride_bytes(p.X, p.Y, p.Z);
   int8_t* gworld = p.voxels + (int64_t)b * nvox;
+  const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
   bool any_done = false;
   if (KIND == kItemStep && p.done_mode == 0 && p.ring) {
-    any_done = p.ring[p.slot] != 0u;
-    if (b == 0 && et == 0) p.ring[(p.slot + 2) % 3] = 0u;
+    any_done = p.ring[slot] != 0u;
+    if (b == 0 && et == 0) p.ring[(slot + 2) % 3] = 0u;
   }
   const bool regen = (KIND == kItemResetRender) && (p.ring[3] != 0u);
   EnvIn in;
@@ -1333,8 +1339,8 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   __syncthreads();
   if (active && et < kWave) {
     Frame f;
-    if (KIND == kItemStep) f = env_update(p, b, world, gworld, any_done, in);
-    else if (KIND == kItemResetRender) f = reset_update(p, b, world);
+    if (KIND == kItemStep) f = env_update(p, slot, b, world, gworld, any_done, in);
+    else if (KIND == kItemResetRender) f = reset_update(p, slot, b, world);
     else if (KIND == kItemRenderBasis) f = frame_from_basis(p, b, world);
     else f = camera_update(p, b, world, KIND == kItemRenderState);
     if (et == 0) {
@@ -1473,6 +1479,7 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
 //           ring[slot] = ring[(slot+1)%3] = 0
 __global__ __launch_bounds__(1024) void k_ring_setup(Params p, int mode) {
   uint32_t any = 0;
+  const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
   if (mode == 0) {
     if (p.episode_length >= 0)
       for (int i = threadIdx.x; i < p.B; i += 1024) any |= (p.step_count[i] >= p.episode_length);
@@ -1484,12 +1491,12 @@ __global__ __launch_bounds__(1024) void k_ring_setup(Params p, int mode) {
   any = __syncthreads_or(any);
   if (threadIdx.x == 0) {
     if (mode == 0) {
-      p.ring[p.slot] = any ? 1u : 0u;
+      p.ring[slot] = any ? 1u : 0u;
     } else {
       p.ring[3] = any ? 1u : 0u;
-      p.ring[p.slot] = 0u;
+      p.ring[slot] = 0u;
     }
-    p.ring[(p.slot + 1) % 3] = 0u;
+    p.ring[(slot + 1) % 3] = 0u;
   }
 }
 
@@ -1648,6 +1655,12 @@ __global__ void k_debug_fast_math(int mode, uint32_t begin, uint32_t count, uint
     out[2] = bad_in;
   }
 }
+
+// Graph-replayable steps: advance the device-resident ring slot (one thread,
+// launched after the step on the same stream).
+__global__ void k_slot_advance(uint32_t* slot) { slot[0] = (slot[0] + 1u) % 3u; }
+// Set the device-resident slot (graph mode switch-on).
+__global__ void k_slot_set(uint32_t* slot, uint32_t v) { slot[0] = v; }
 
 __global__ void k_debug_sincos(const float* x, float* s, float* c, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -317,6 +317,32 @@ class MineEnv:
         _lib.check(_lib.load().mc_env_set_shape(self._handle, threads, waves_per_env, lift),
                    "set_shape")
 
+    def set_graph_mode(self, on: bool = True) -> None:
+        """Make step() replayable from a captured graph: the episode-ring slot
+        moves to device memory and each step also launches a one-thread slot
+        advance.  Off by default (one launch per step).  Groups in graph mode
+        cannot be stepped by EnvBatch / StepServer."""
+        _lib.check(_lib.load().mc_env_set_graph_mode(self._handle, int(bool(on)), self._stream()),
+                   "set_graph_mode")
+        self._graph_mode = bool(on)
+
+    def capture_step(self, actions: torch.Tensor, look: Optional[torch.Tensor] = None):
+        """Capture one step on static input buffers into a torch CUDA graph
+        (the reference captures its kernel sequence into a CUDA graph,
+        src/kernels.cu:2338-2382).  Returns (graph, obs, rewards, dones): fill
+        `actions` (and `look`) in place, call graph.replay(), read the outputs.
+        Every replay is one environment step, exactly as step() would be."""
+        self.set_graph_mode(True)
+        a, kind, lk = self._prepare_actions(actions, look)
+        if a.data_ptr() != actions.data_ptr() or (lk is not None and lk.data_ptr() != look.data_ptr()):
+            raise ValueError("capture_step needs contiguous device tensors of the action dtype "
+                             "(int64 [B] ids, or int8 [B,8] buttons with float32 [B,2] look)")
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            obs, rewards, dones = self.step(actions, look)
+        return g, obs, rewards, dones
+
     def _render(self) -> torch.Tensor:
         """minecu/__init__.py:446-466: cameras from state, then render."""
         _lib.check(_lib.load().mc_env_render(self._handle, self._stream()), "render")
