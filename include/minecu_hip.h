@@ -353,6 +353,9 @@ int mc_env_render(mc_env* env, void* stream);
  * shape after the LDS fit (lift may be NULL). */
 int mc_env_set_shape(mc_env* env, int threads, int waves_per_env, int lift);
 int mc_env_get_shape(const mc_env* env, int* threads, int* waves_per_env, int* lift);
+/* Copy of the group's configuration (for schedulers that weigh groups by
+ * their per-step work). */
+int mc_env_get_config(const mc_env* env, mc_config* out);
 
 /* ---- batched groups (no reference counterpart: the reference runs one
  * fixed-shape MineEnv per process-global graph) --------------------------- */

@@ -30,9 +30,13 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef MC_GLOBAL v4u guint4;
 typedef MC_GLOBAL uint32_t guint32;
 typedef MC_GLOBAL v4f gfloat4;
-// Bytes of world staged in LDS per workgroup: 64 KiB less room for the
-// kernels' static LDS (palette, frames, u/v tables, work-item bookkeeping).
-constexpr int kMaxLdsWorld = 65536 - 4096;
+// LDS of one workgroup: gfx950 lets a workgroup allocate up to 160 KiB.
+// kStaticLdsReserve bounds the kernels' static LDS (palette, colour table,
+// frames, u/v tables, work-item bookkeeping); the rest holds worlds.  The host
+// also caps the world bytes by the device's reported per-workgroup maximum.
+constexpr int kLdsPerWorkgroup = 160 * 1024;
+constexpr int kStaticLdsReserve = 8192;
+constexpr int kMaxLdsWorld = kLdsPerWorkgroup - kStaticLdsReserve;
 
 // Block ids (src/kernels.cu:11-24).
 constexpr int8_t kAir = -1, kGrass = 0, kDirt = 1, kOakLog = 3, kLeaves = 4, kBrick = 8,
@@ -1417,6 +1421,11 @@ struct MultiArgs {
   uint32_t* counters;          // [2] device words, zero between launches
   const MultiEntry* entries;   // [n_groups] device memory
 };
+
+static_assert(sizeof(float) * (kPaletteLen + 2 * kMaxUV) + sizeof(uint32_t) * kFlatLen +
+                      sizeof(Frame) * (512 / kWave) + sizeof(int) * (kMaxGroups + 1) + 64 <=
+                  kStaticLdsReserve,
+              "static LDS of the env kernels exceeds its reserve");
 
 template <int NT, bool LIFT>
 __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {

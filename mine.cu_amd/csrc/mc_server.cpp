@@ -60,6 +60,7 @@ struct Group {
   mc_env* env;
   int64_t bytes;
   int batch;
+  double cost;     // relative work of one step (batch x per-env rays and voxels)
   int state;
   bool busy;       // a request of this group is on the device
   int pending;     // queued requests
@@ -89,6 +90,7 @@ struct mc_server {
   std::condition_variable cv;
   int64_t budget = 0;
   int64_t used = 0;
+  double admitted_cost = 0.0;  // summed step cost of the admitted groups
   std::vector<Group> groups;
   std::deque<int> waiting;
   std::deque<Req> pending;
@@ -110,6 +112,7 @@ struct mc_server {
       Group& g = groups[waiting.front()];
       if (used + g.bytes > budget) break;
       used += g.bytes;
+      admitted_cost += g.cost;
       g.state = kAdmitted;
       waiting.pop_front();
     }
@@ -162,12 +165,21 @@ struct mc_server {
             break;
           }
         if (stream >= 0 && !events.empty()) {
+          // With several streams a launch takes at most its share of the
+          // admitted work, so when every group is waiting (a saturated, closed
+          // loop) the groups form cohorts whose launches queue behind each
+          // other: the device starts the next one while the host retires the
+          // last and gathers its successors.
+          const double cap = streams.size() > 1 ? admitted_cost / (double)streams.size() : 1e300;
+          double taken = 0.0;
           for (auto it = pending.begin();
                it != pending.end() && (int)take.size() < 64;) {
             Group& g = groups[it->gid];
             bool dup = false;
             for (const Req& q : take) dup |= (q.gid == it->gid);
             if (g.state == kAdmitted && !g.busy && !dup) {
+              if (!take.empty() && taken + g.cost > cap * 1.0001) break;
+              taken += g.cost;
               take.push_back(*it);
               g.busy = true;
               g.pending--;
@@ -291,10 +303,17 @@ int mc_server_add_group(mc_server* s, mc_env* env, int64_t bytes, int batch_size
   if (!s || !env || !gid || bytes < 0 || batch_size < 1) return sfail(MC_E_INVALID, "bad argument");
   std::lock_guard<std::mutex> lk(s->mu);
   if (bytes > s->budget) return sfail(MC_E_INVALID, "group needs more than the whole budget");
+  mc_config c{};
+  if (mc_env_get_config(env, &c) != MC_OK) return sfail(MC_E_INVALID, "bad env handle");
   Group g{};
   g.env = env;
   g.bytes = bytes;
   g.batch = batch_size;
+  // Step time per env on MI355X fits rays x world height (DDA length) plus
+  // twice the world's voxels (staging, scan, regeneration) to within ~15%
+  // over 16^3/32^3 worlds and 32^2/64^2 images.
+  g.cost = (double)batch_size * ((double)c.width * c.height * c.world_y +
+                                 2.0 * c.world_x * c.world_y * c.world_z);
   g.state = kWaiting;
   *gid = (int)s->groups.size();
   s->groups.push_back(g);
@@ -317,7 +336,10 @@ int mc_server_remove_group(mc_server* s, int gid) {
   if (gid < 0 || gid >= (int)s->groups.size()) return sfail(MC_E_INVALID, "no group %d", gid);
   Group& g = s->groups[gid];
   if (g.busy || g.pending > 0) return sfail(MC_E_INVALID, "group %d has requests in flight", gid);
-  if (g.state == kAdmitted) s->used -= g.bytes;
+  if (g.state == kAdmitted) {
+    s->used -= g.bytes;
+    s->admitted_cost -= g.cost;
+  }
   if (g.state == kWaiting)
     s->waiting.erase(std::remove(s->waiting.begin(), s->waiting.end(), gid), s->waiting.end());
   g.state = kRemoved;

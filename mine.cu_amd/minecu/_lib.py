@@ -80,6 +80,7 @@ SIGNATURES = {
     "mc_env_render": ([_P, _P], _I),
     "mc_env_set_shape": ([_P, _I, _I, _I], _I),
     "mc_env_get_shape": ([_P, _P, _P, _P], _I),
+    "mc_env_get_config": ([_P, _P], _I),
     "mc_batch_create": ([_P], _I),
     "mc_batch_destroy": ([_P], _I),
     "mc_batch_set_persistent": ([_P, _I], _I),

@@ -150,6 +150,23 @@ def test_odd_shapes(dims, img):
     print(st.summary())
 
 
+@pytest.mark.parametrize("dims,img", [((48, 48, 48), (32, 32)), ((56, 44, 60), (40, 24))])
+def test_large_lds_worlds(dims, img):
+    """Worlds above 64 KiB staged in one workgroup's LDS: 48^3 (the
+    reference's largest benchmarked world, 110,592 B) and a non-cubic world
+    near the 152 KiB budget, through falling, breaking and regeneration."""
+    X, Y, Z = dims
+    W, H = img
+    env, orc = _envs(batch_size=6, world_x=X, world_y=Y, world_z=Z, render_width=W,
+                     render_height=H, obs_dtype="uint8", episode_length=25)
+    st = Stats()
+    env.reset()
+    orc.reset()
+    compare_env(env, orc, st, "reset")
+    run_pair(env, orc, _break_heavy(55, 6, 12), st, f"large {dims} {img}", check_every=3)
+    print(st.summary())
+
+
 def test_partial_reset_mask():
     env, orc = _envs(batch_size=32, world_size=16, render_width=32, render_height=32,
                      obs_dtype="uint8", episode_length=25)
