@@ -364,8 +364,9 @@ typedef struct mc_batch mc_batch;
 
 int mc_batch_create(mc_batch** out);
 int mc_batch_destroy(mc_batch* b);
-/* 1 (default): persistent launch with an atomic work counter; 0: one
- * workgroup per work item. */
+/* Launch form of mc_batch_step: 1 (default) persistent workgroups that take
+ * work items from an atomic counter; 0: one workgroup per work item.
+ * mc_batch_autotune times both on the given mix and keeps the faster. */
 int mc_batch_set_persistent(mc_batch* b, int persistent);
 
 /* One step of n (<= 64) env groups -- any mix of batch sizes, world sizes,
@@ -377,10 +378,14 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
                   const int* action_kinds, const float* const* looks, uint8_t* const* dones,
                   int n, void* stream);
 
-/* Autotuner: for every distinct (world, image, obs) shape among envs, times
- * each candidate launch shape on a scratch group (never the caller's state)
- * for `steps` steps and keeps the fastest; sets the shape of every listed env.
- * Results are cached in the batch object.  Synchronises `stream`. */
+/* Autotuner for a mix of groups: steps a scratch mirror of the mix (same
+ * configs, never the caller's state) through mc_batch_step and, bucket by
+ * bucket (distinct world, image and obs mode; most work first), keeps the
+ * launch shape that shortens the whole mixed launch; the result must also beat
+ * the default shapes side by side.  Then picks the launch form (persistent or
+ * not, mc_batch_set_persistent).  Timings are best of two runs of `steps`
+ * steps.  Sets the shape of every listed env and records each bucket's shape
+ * in the batch (mc_batch_tuned_shape).  Synchronises `stream`. */
 int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* stream);
 int mc_batch_tuned_shape(const mc_batch* b, const mc_config* cfg, int* threads,
                          int* waves_per_env, int* lift);

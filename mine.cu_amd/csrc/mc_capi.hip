@@ -898,7 +898,7 @@ int occupancy_blocks(mc_batch* b, int lds) {
     if (o.nt == NT && o.lift == (int)LIFT && o.lds == lds) return o.blocks;
   int nb = 0;
   const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &nb, mc::k_multi<NT, LIFT>, NT, (size_t)lds);
+      &nb, mc::k_multi<NT, LIFT, true>, NT, (size_t)lds);
   if (e != hipSuccess || nb < 1) nb = 1;
   b->occ.push_back({NT, (int)LIFT, lds, nb});
   return nb;
@@ -906,10 +906,13 @@ int occupancy_blocks(mc_batch* b, int lds) {
 
 template <int NT, bool LIFT>
 int launch_multi(mc_batch* b, const mc::MultiArgs& a, int lds, hipStream_t st) {
-  int grid = a.total_items;
-  if (a.persistent) grid = std::min(grid, occupancy_blocks<NT, LIFT>(b, lds) * b->cus);
-  if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((mc::k_multi<NT, LIFT>), dim3(grid), dim3(NT), (size_t)lds, st, a);
+  if (b->persistent) {
+    const int grid = std::max(1, std::min(a.total_items, occupancy_blocks<NT, LIFT>(b, lds) * b->cus));
+    hipLaunchKernelGGL((mc::k_multi<NT, LIFT, true>), dim3(grid), dim3(NT), (size_t)lds, st, a);
+  } else {
+    hipLaunchKernelGGL((mc::k_multi<NT, LIFT, false>), dim3(std::max(1, a.total_items)), dim3(NT),
+                       (size_t)lds, st, a);
+  }
   return check_launch("batched step launch");
 }
 }  // namespace
@@ -1034,7 +1037,6 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
   mc::MultiArgs a;
   a.n_groups = n;
   a.total_items = items;
-  a.persistent = b->persistent;
   a.counters = b->counters + 2 * slot;
   a.entries = de;
   const int r = (nt == 512) ? (lift ? launch_multi<512, true>(b, a, lds, S(stream))
@@ -1168,33 +1170,59 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
   std::vector<int> kinds(n, MC_ACT_DISCRETE);
   for (int i = 0; i < n; ++i) es[i] = mir[i]->env;
   int tick = 0;
-  // k steps of the mirror; *ms receives their time when ms is non-NULL.
-  auto run = [&](int k, float* ms) -> int {
+  // One step of the mirror under configuration c (per-bucket shapes and the
+  // launch form); with timing, the launch is bracketed by events and waited.
+  struct Config {
+    std::vector<Shape> shape;
+    int persistent;
+  };
+  auto step_as = [&](const Config& c, float* ms) -> int {
+    for (int i = 0; i < n; ++i) mir[i]->env->shape = c.shape[bucket[i]];
+    b->persistent = c.persistent;
+    for (int i = 0; i < n; ++i) as[i] = (const int64_t*)acts + (size_t)(tick % kActionSteps) * maxB;
+    ++tick;
     if (ms && hipEventRecord(e0, st) != hipSuccess) return fail(MC_E_HIP, "autotune: event");
-    for (int t = 0; t < k; ++t, ++tick) {
-      for (int i = 0; i < n; ++i) as[i] = (const int64_t*)acts + (size_t)(tick % kActionSteps) * maxB;
-      const int r = mc_batch_step(b, es.data(), as.data(), kinds.data(), nullptr, nullptr, n, stream);
-      if (r) return r;
-    }
-    if (!ms) return MC_OK;
+    const int e = mc_batch_step(b, es.data(), as.data(), kinds.data(), nullptr, nullptr, n, stream);
+    if (e || !ms) return e;
     if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
         hipEventElapsedTime(ms, e0, e1) != hipSuccess)
       return fail(MC_E_HIP, "autotune: timing failed");
     return MC_OK;
   };
-  std::vector<Shape> choice(keys.size());
+  // Mean launch time of x and y over `steps` launches each, alternating
+  // launch by launch: the mirror's cost drifts as its agents move, and the
+  // alternation gives both the same drift.
+  auto duel = [&](const Config& x, const Config& y, float* tx, float* ty) -> int {
+    int e = step_as(x, nullptr);
+    if (!e) e = step_as(y, nullptr);
+    *tx = *ty = 0.0f;
+    for (int k = 0; k < 2 * steps && !e; ++k) {
+      float ms = 0.0f;
+      e = step_as((k & 1) ? y : x, &ms);
+      *((k & 1) ? ty : tx) += ms;
+    }
+    return e;
+  };
+  const int caller_form = b->persistent;
+  Config cur{std::vector<Shape>(keys.size()), caller_form};
   for (size_t j = 0; j < keys.size(); ++j) {
     const int X = std::get<0>(keys[j]), Y = std::get<1>(keys[j]), Z = std::get<2>(keys[j]);
-    choice[j] = default_shape(std::get<3>(keys[j]), std::get<4>(keys[j]), X, Y, Z);
+    cur.shape[j] = default_shape(std::get<3>(keys[j]), std::get<4>(keys[j]), X, Y, Z);
   }
-  auto apply = [&]() {
-    for (int i = 0; i < n; ++i) mir[i]->env->shape = choice[bucket[i]];
-  };
+  const Config defaults = cur;
   // Warm-up: agents land and the mirror reaches its steady state.
-  apply();
-  int r = run(20, nullptr);
-  // Coordinate descent, most expensive bucket first: a bucket takes the
-  // candidate that beats the current assignment by more than 1.5%.
+  int r = MC_OK;
+  for (int t = 0; t < 20 && !r; ++t) r = step_as(cur, nullptr);
+  // Launch form first (it decides how the buckets share the device).
+  if (!r) {
+    Config other = cur;
+    other.persistent = !cur.persistent;
+    float tc = 0.0f, to = 0.0f;
+    r = duel(cur, other, &tc, &to);
+    if (!r && to < tc) cur = other;
+  }
+  // Coordinate descent, most expensive bucket first: a bucket takes a
+  // candidate shape that beats the current configuration by more than 1.5%.
   std::vector<int> order(keys.size());
   for (size_t j = 0; j < order.size(); ++j) order[j] = (int)j;
   std::stable_sort(order.begin(), order.end(),
@@ -1202,40 +1230,35 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
   const Shape cands[] = {{64, 1, 1},  {256, 1, 1}, {256, 2, 1}, {256, 4, 1}, {512, 8, 1},
                          {64, 1, 0},  {256, 1, 0}, {256, 2, 0}, {256, 4, 0}, {512, 8, 0}};
   for (int j : order) {
-    if (r) break;
     const int X = std::get<0>(keys[j]), Y = std::get<1>(keys[j]), Z = std::get<2>(keys[j]);
-    float base_ms = 0.0f;
-    r = run(2, nullptr);
-    if (!r) r = run(steps, &base_ms);
-    Shape best = choice[j];
-    float best_ms = base_ms;
     for (const Shape& cand : cands) {
       if (r) break;
-      if (cand.nt == choice[j].nt && cand.wpe == choice[j].wpe && cand.lift == choice[j].lift)
-        continue;
+      const Shape& now = cur.shape[j];
+      if (cand.nt == now.nt && cand.wpe == now.wpe && cand.lift == now.lift) continue;
       if (fit_lds(cand, X, Y, Z).wpe != cand.wpe) continue;  // does not fit as asked
-      const Shape keep = choice[j];
-      choice[j] = cand;
-      apply();
-      float ms = 0.0f;
-      r = run(2, nullptr);
-      if (!r) r = run(steps, &ms);
-      choice[j] = keep;
-      apply();
-      if (!r && ms < best_ms) {
-        best_ms = ms;
-        best = cand;
-      }
+      Config next = cur;
+      next.shape[j] = cand;
+      float tc = 0.0f, tn = 0.0f;
+      r = duel(cur, next, &tc, &tn);
+      if (!r && tn < tc * 0.985f) cur = next;
     }
-    if (!r && best_ms < base_ms * 0.985f) choice[j] = best;
-    apply();
+  }
+  // The result must beat the default configuration side by side.
+  if (!r) {
+    float tc = 0.0f, td = 0.0f;
+    r = duel(cur, defaults, &tc, &td);
+    if (!r && !(tc < td * 0.99f)) cur = defaults;
   }
   if (!r && hipStreamSynchronize(st) != hipSuccess) r = fail(MC_E_HIP, "autotune: sync failed");
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (r) return r;
-  for (size_t j = 0; j < keys.size(); ++j) b->tuned[keys[j]] = choice[j];
-  for (int i = 0; i < n; ++i) envs[i]->shape = choice[bucket[i]];
+  if (r) {
+    b->persistent = caller_form;
+    return r;
+  }
+  b->persistent = cur.persistent;
+  for (size_t j = 0; j < keys.size(); ++j) b->tuned[keys[j]] = cur.shape[j];
+  for (int i = 0; i < n; ++i) envs[i]->shape = cur.shape[bucket[i]];
   return MC_OK;
 }
 

@@ -1417,7 +1417,6 @@ struct MultiEntry {
 struct MultiArgs {
   int n_groups;
   int total_items;
-  int persistent;
   uint32_t* counters;          // [2] device words, zero between launches
   const MultiEntry* entries;   // [n_groups] device memory
 };
@@ -1427,38 +1426,49 @@ static_assert(sizeof(float) * (kPaletteLen + 2 * kMaxUV) + sizeof(uint32_t) * kF
                   kStaticLdsReserve,
               "static LDS of the env kernels exceeds its reserve");
 
-template <int NT, bool LIFT>
+// PERSIST: a grid sized to the resident capacity whose workgroups loop over
+// items taken from an atomic counter; otherwise one workgroup per item (the
+// loop is compiled out; the hardware dispatcher then balances the load, and
+// the kernel needs fewer registers: 73 VGPRs against 118 on gfx950).
+template <int NT, bool LIFT, bool PERSIST>
 __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
   __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
-  __shared__ int s_begin[kMaxGroups];
   __shared__ int s_item;
   __shared__ float s_u[kMaxUV], s_v[kMaxUV];
+  // The entry table and the descriptors are written by the host before the
+  // launch and only read here: constant address space, so uniform reads are
+  // scalar loads.
+  typedef __attribute__((address_space(4))) const MultiEntry CEntry;
+  typedef __attribute__((address_space(4))) const uint32_t CWord;
+  CEntry* entries = (CEntry*)a.entries;
   load_palette(s_pal, s_flat, threadIdx.x);
-  if (threadIdx.x < a.n_groups) s_begin[threadIdx.x] = a.entries[threadIdx.x].item_begin;
+  // First item of group `lane` (past the last group: never <= an item).
+  const int lane = threadIdx.x & (kWave - 1);
+  const int my_begin = (lane < a.n_groups) ? entries[lane].item_begin : 0x7fffffff;
   int item = blockIdx.x;
   for (;;) {
-    if (a.persistent) {
+    if (PERSIST) {
       __syncthreads();  // the previous item is done with LDS and s_item
       if (threadIdx.x == 0) s_item = (int)atomicAdd(&a.counters[0], 1u);
     }
     __syncthreads();
-    if (a.persistent) item = s_item;
+    if (PERSIST) item = __builtin_amdgcn_readfirstlane(s_item);
     if (item >= a.total_items) break;
-    int g = 0;
-    while (g + 1 < a.n_groups && s_begin[g + 1] <= item) ++g;
-    const MultiEntry& me = a.entries[g];
-    // The descriptor is uniform over the workgroup: keep it in scalar
+    // Items are numbered group by group: the item's group is the number of
+    // groups that begin at or before it, less one.
+    const int g = __builtin_amdgcn_readfirstlane(__popcll(__ballot(my_begin <= item)) - 1);
+    CEntry& me = entries[g];
+    // The descriptor is uniform over the workgroup: scalar loads into scalar
     // registers (the vector file is needed by the ray loops).
     Params p;
     {
-      const uint32_t* src = reinterpret_cast<const uint32_t*>(me.desc);
+      CWord* src = (CWord*)me.desc;
       uint32_t* dst = reinterpret_cast<uint32_t*>(&p);
 #pragma unroll
-      for (int i = 0; i < (int)(sizeof(Params) / 4); ++i)
-        dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+      for (int i = 0; i < (int)(sizeof(Params) / 4); ++i) dst[i] = src[i];
     }
     p.actions = me.actions;
     p.look = me.look;
@@ -1467,11 +1477,11 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
     p.slot = me.slot;
     const int wpe = me.wpe;
     const int epw = (NT / kWave) / wpe;
-    run_item<NT, kItemStep, LIFT>(p, me.obs_mode, (item - s_begin[g]) * epw, wpe, s_lds, s_pal,
+    run_item<NT, kItemStep, LIFT>(p, me.obs_mode, (item - me.item_begin) * epw, wpe, s_lds, s_pal,
                                   s_flat, s_frames, s_u, s_v);
-    if (!a.persistent) break;
+    if (!PERSIST) break;
   }
-  if (a.persistent && threadIdx.x == 0) {
+  if (PERSIST && threadIdx.x == 0) {
     __threadfence();
     const uint32_t prev = atomicAdd(&a.counters[1], 1u);
     if (prev == gridDim.x - 1) {
