@@ -159,6 +159,8 @@ mc::Params make_params(const mc_config* c, const mc_state* s) {
     p.rewards = s->rewards;
     p.basis = s->basis;
   }
+  p.ys = c->world_y;
+  p.fb = nullptr;
   return p;
 }
 
@@ -209,7 +211,7 @@ template <int NT, int KIND, bool LIFT>
 int launch_kind(const mc::Params& p, int obs_mode, int wpe, hipStream_t st) {
   const int epw = (NT / mc::kWave) / wpe;
   const int grid = (p.B + epw - 1) / epw;
-  const size_t lds = (size_t)mc::item_lds_bytes(epw, p.X, p.Y, p.Z);
+  const size_t lds = (size_t)mc::item_lds_bytes(epw, p.X, p.ys, p.Z);
   hipLaunchKernelGGL((mc::k_env<NT, KIND, LIFT>), dim3(grid), dim3(NT), lds, st, p, obs_mode,
                      wpe);
   return check_launch("env kernel launch");
@@ -229,9 +231,16 @@ int launch_nt(int kind, const mc::Params& p, int obs_mode, int wpe, hipStream_t 
   }
 }
 
-int launch_env(int kind, const mc::Params& p, int obs_mode, Shape s, hipStream_t st) {
-  s = fit_lds(s, p.X, p.Y, p.Z);
-  if (mc::item_lds_bytes(epw_of(s), p.X, p.Y, p.Z) > lds_budget())
+// Staged layers below Y (Params::ys) apply to step items with a deferred-env
+// list only; every other launch stages whole worlds.
+int launch_env(int kind, const mc::Params& q, int obs_mode, Shape s, hipStream_t st) {
+  mc::Params p = q;
+  if (kind != mc::kItemStep || !p.fb || p.ys < 1 || p.ys > p.Y) {
+    p.ys = p.Y;
+    p.fb = nullptr;
+  }
+  s = fit_lds(s, p.X, p.ys, p.Z);
+  if (mc::item_lds_bytes(epw_of(s), p.X, p.ys, p.Z) > lds_budget())
     return fail(MC_E_UNSUPPORTED, "world does not fit the LDS budget");
   switch (s.nt * 2 + s.lift) {
     case 128: return launch_nt<64, false>(kind, p, obs_mode, s.wpe, st);
@@ -241,6 +250,42 @@ int launch_env(int kind, const mc::Params& p, int obs_mode, Shape s, hipStream_t
     case 1024: return launch_nt<512, false>(kind, p, obs_mode, s.wpe, st);
     default: return launch_nt<512, true>(kind, p, obs_mode, s.wpe, st);
   }
+}
+
+// Full-height pass over the envs a staged-height step deferred (p.fb, the
+// same slot): one env per workgroup, whole world in LDS.
+int launch_deferred(const mc::Params& q, int obs_mode, int lift, hipStream_t st) {
+  mc::Params p = q;
+  p.ys = p.Y;
+  const int full = mc::item_lds_bytes(1, p.X, p.Y, p.Z);
+  if (full > lds_budget()) return fail(MC_E_UNSUPPORTED, "world does not fit the LDS budget");
+  const dim3 grid(std::min(p.B, 128));
+  if (full > 80 * 1024) {
+    if (lift) hipLaunchKernelGGL((mc::k_env_fb<512, true>), grid, dim3(512), full, st, p, obs_mode);
+    else hipLaunchKernelGGL((mc::k_env_fb<512, false>), grid, dim3(512), full, st, p, obs_mode);
+  } else {
+    if (lift) hipLaunchKernelGGL((mc::k_env_fb<256, true>), grid, dim3(256), full, st, p, obs_mode);
+    else hipLaunchKernelGGL((mc::k_env_fb<256, false>), grid, dim3(256), full, st, p, obs_mode);
+  }
+  return check_launch("deferred-env launch");
+}
+
+// Layers a MineEnv step stages in LDS: the fresh world's solids end at
+// ground_height + 4 (the house roof); the agent's eye stays below
+// max(spawn_y, Y/4 + jump apex) + 1.6 (the physics has no block collision:
+// only gravity, jumps and the ground clamp at Y/4), and the raycast reaches
+// 4 above the eye.  Envs outside these bounds are deferred to the
+// full-height pass, so this only sizes the fast path.
+int staged_layers(const mc_config& c) {
+  const int Y = c.world_y;
+  if (c.world_x > mc::kMaxInsideDim || Y > mc::kMaxInsideDim || c.world_z > mc::kMaxInsideDim)
+    return Y;
+  if (!(c.gravity < 0.0f) || !(c.jump_vel >= 0.0f) || !(c.spawn_y < (float)Y)) return Y;
+  const float apex = c.jump_vel * c.jump_vel / (-2.0f * c.gravity);
+  const float eye = std::max(c.spawn_y, (float)(Y / 4) + apex) + 1.6f;
+  const float need = std::max((float)(c.ground_height + 5), std::ceil(eye + 4.5f) + 1.0f);
+  if (!(need < (float)Y)) return Y;
+  return std::max(1, (int)need);
 }
 
 }  // namespace
@@ -255,6 +300,9 @@ struct mc_env {
   int slot;           // ring slot the next step launch reads (host-tracked mode)
   int graph_mode;     // 1: the slot lives in ring[4] and each step advances it
   Shape shape;        // launch shape for this group (default or tuned)
+  int shape_default;  // shape not set by the caller: steps size it for ys
+  int ys;             // layers a step stages in LDS (world_y: all)
+  uint32_t* fb;       // [3][B + 2] deferred-env lists (NULL when ys == world_y)
 };
 
 extern "C" {
@@ -419,8 +467,16 @@ int mc_env_create(const mc_config* cfg, const mc_state* st, mc_env** out) {
   e->slot = 0;
   e->graph_mode = 0;
   e->shape = default_shape(*cfg);
+  e->shape_default = 1;
+  e->ys = staged_layers(*cfg);
+  e->fb = nullptr;
   hipError_t he = hipMalloc(reinterpret_cast<void**>(&e->ring), MC_WORKSPACE_BYTES);
   if (he == hipSuccess) he = hipMemset(e->ring, 0, MC_WORKSPACE_BYTES);
+  if (he == hipSuccess && e->ys < cfg->world_y) {
+    const size_t fb_bytes = sizeof(uint32_t) * 3 * ((size_t)cfg->batch_size + 2);
+    he = hipMalloc(reinterpret_cast<void**>(&e->fb), fb_bytes);
+    if (he == hipSuccess) he = hipMemset(e->fb, 0, fb_bytes);
+  }
   if (he == hipSuccess) {
     e->base.ring = e->ring;
     he = hipMalloc(reinterpret_cast<void**>(&e->desc), sizeof(mc::Params));
@@ -430,6 +486,7 @@ int mc_env_create(const mc_config* cfg, const mc_state* st, mc_env** out) {
   if (he != hipSuccess) {
     if (e->ring) (void)hipFree(e->ring);
     if (e->desc) (void)hipFree(e->desc);
+    if (e->fb) (void)hipFree(e->fb);
     delete e;
     return fail(MC_E_HIP, "workspace: %s", hipGetErrorString(he));
   }
@@ -440,13 +497,15 @@ int mc_env_create(const mc_config* cfg, const mc_state* st, mc_env** out) {
 int mc_env_set_shape(mc_env* env, int threads, int waves_per_env, int lift) {
   if (!env) return fail(MC_E_INVALID, "env is NULL");
   const Shape d = default_shape(env->cfg);
-  if (threads == 0 && waves_per_env == 0) {
+  env->shape_default = (threads == 0 && waves_per_env == 0);
+  if (env->shape_default) {
     env->shape = Shape{d.nt, d.wpe, lift < 0 ? d.lift : lift};
   } else {
     env->shape = Shape{threads, waves_per_env, lift < 0 ? env->shape.lift : lift};
   }
   if (!valid_shape(env->shape)) {
     env->shape = d;
+    env->shape_default = 1;
     return fail(MC_E_INVALID, "shape (%d threads, %d waves/env, lift %d) not supported",
                 threads, waves_per_env, lift);
   }
@@ -482,6 +541,7 @@ int mc_env_destroy(mc_env* env) {
   if (!env) return MC_OK;
   if (env->ring) (void)hipFree(env->ring);
   if (env->desc) (void)hipFree(env->desc);
+  if (env->fb) (void)hipFree(env->fb);
   delete env;
   return MC_OK;
 }
@@ -500,16 +560,28 @@ int mc_env_step(mc_env* env, const void* actions, int action_kind, const float* 
   p.dones = dones_out;
   p.ring = env->ring;
   p.slot = env->slot;
+  if (env->graph_mode) p.slot_dev = env->ring + 4;
+  // Staged height: the step launch defers the envs it cannot run to a
+  // full-height pass over the same slot, launched right behind it.
+  Shape s = env->shape;
+  if (env->fb) {
+    p.ys = env->ys;
+    p.fb = env->fb;
+    if (env->shape_default) {
+      const mc_config& c = env->cfg;
+      const Shape d = default_shape(c.width, c.height, c.world_x, env->ys, c.world_z);
+      s = Shape{d.nt, d.wpe, s.lift};
+    }
+  }
+  int r = launch_env(mc::kItemStep, p, env->cfg.obs_mode, s, S(stream));
+  if (!r && env->fb) r = launch_deferred(p, env->cfg.obs_mode, s.lift, S(stream));
+  if (r) return r;
   if (env->graph_mode) {
-    p.slot_dev = env->ring + 4;
-    int r = launch_env(mc::kItemStep, p, env->cfg.obs_mode, env->shape, S(stream));
-    if (r) return r;
     hipLaunchKernelGGL(mc::k_slot_advance, dim3(1), dim3(1), 0, S(stream), env->ring + 4);
     return check_launch("slot advance");
   }
-  const int r = launch_env(mc::kItemStep, p, env->cfg.obs_mode, env->shape, S(stream));
-  if (r == MC_OK) env->slot = (env->slot + 1) % 3;
-  return r;
+  env->slot = (env->slot + 1) % 3;
+  return MC_OK;
 }
 
 int mc_env_set_graph_mode(mc_env* env, int on, void* stream) {

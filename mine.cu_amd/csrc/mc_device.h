@@ -154,6 +154,15 @@ struct Params {
   // [B, 14] camera basis (src/kernels.cu:503-561; NULL: none).  Step items
   // write it in the prebasis modes; render-basis items read it.
   float* basis;
+  // Layers y < ys of each world are staged in LDS (ys = Y: all).  With ys < Y
+  // a step item first checks that its env's layers y >= ys are all air and
+  // that the step cannot reach them (eyes and the raycast stay below ys);
+  // rays that climb to ys then end as sky, which is the reference's result.
+  // An env that fails the check is left untouched and listed in
+  // fb[slot * (B + 2)]: [count, finished workgroups, env...] for a
+  // full-height pass (k_env_fb), which empties the list when it ends.
+  int ys;
+  uint32_t* fb;
 };
 
 // Camera frame of one env, published by wave 0 through LDS.
@@ -184,6 +193,8 @@ struct Frame {
   // Starting packed position of the in-world DDA per direction octant (see
   // OctPos), written by lanes 0-7 of each of the env's waves before the render.
   uint32_t pos0[8];
+  // The env left this item untouched for the full-height pass (Params::fb).
+  int deferred;
 };
 
 // Largest world dimension of the in-world DDA loop (10-bit packed position).
@@ -306,7 +317,7 @@ __device__ __forceinline__ void store_fresh_world(const Params& p, int8_t* gworl
 // flight before their LDS writes.
 __device__ __forceinline__ void stage_world(int8_t* lds, const int8_t* gworld, int nvox, int tid,
                                             int nt) {
-  if ((nvox & 15) == 0) {
+  if (((nvox | (int)(uintptr_t)gworld) & 15) == 0) {
     const guint4* g = (const guint4*)gworld;
     v4u* l = reinterpret_cast<v4u*>(lds);
     const int n16 = nvox >> 4;
@@ -322,6 +333,21 @@ __device__ __forceinline__ void stage_world(int8_t* lds, const int8_t* gworld, i
   } else {
     for (int i = tid; i < nvox; i += nt) lds[i] = gworld[i];
   }
+}
+
+// Whether any of the n voxels at g (this thread's share) is not air.
+__device__ __forceinline__ bool any_not_air(const int8_t* g, int n, int tid, int nt) {
+  bool found = false;
+  if ((((uintptr_t)g | (uintptr_t)n) & 15) == 0) {
+    const guint4* g4 = (const guint4*)g;
+    for (int c = tid; c < (n >> 4); c += nt) {
+      const v4u w = g4[c];
+      found |= (w.x & w.y & w.z & w.w) != 0xffffffffu;
+    }
+  } else {
+    for (int i = tid; i < n; i += nt) found |= g[i] != kAir;
+  }
+  return found;
 }
 
 // ---------------------------------------------------------------------------
@@ -365,6 +391,7 @@ __device__ __forceinline__ void finish_frame(const Params& p, Frame& f, const in
   // The solid-extent scan reads whole 16-byte rows of z; an eye inside a
   // solid needs no ray at all.
   f.lift = (f.fill_block < 0) && (f.flat_fill < 0) && ((p.Z & 15) == 0);
+  f.deferred = 0;
 }
 
 __device__ __forceinline__ Frame make_frame(const Params& p, float x, float y, float z,
@@ -418,7 +445,7 @@ __device__ __forceinline__ void store_basis(const Params& p, int b, const Frame&
 // 0), reduced over the env's threads into f (LDS) with LDS atomics.
 __device__ __forceinline__ void solid_extent(const Params& p, const int8_t* world, Frame* f,
                                              int y0, int tid, int nt) {
-  const int X = p.X, Y = p.Y, zc = p.Z >> 4;
+  const int X = p.X, Y = p.ys, zc = p.Z >> 4;
   int xmin = INT_MAX, xmax = INT_MIN, zmin = INT_MAX, zmax = INT_MIN, ymax = INT_MIN;
   if (y0 < 0) y0 = 0;
   if (y0 < Y) {
@@ -570,9 +597,9 @@ template <bool LIFT>
 __device__ __forceinline__ uint32_t oct_pos(const Params& p, const Frame& f, int k) {
   const int X = p.X, Y = p.Y, Z = p.Z;
   const bool bx = k & 1, by = (k >> 1) & 1, bz = (k >> 2) & 1;
-  int lim_x = X, lim_y = Y, lim_z = Z;
+  int lim_x = X, lim_y = by ? p.ys : Y, lim_z = Z;
   if (LIFT && f.lift && by) {
-    lim_y = max(0, min(Y, f.hy1 + 1));
+    lim_y = max(0, min(p.ys, f.hy1 + 1));
     lim_x = max(0, bx ? min(X, f.hx1 + 1) : min(X, X - f.hx0));
     lim_z = max(0, bz ? min(Z, f.hz1 + 1) : min(Z, Z - f.hz0));
   }
@@ -1044,7 +1071,8 @@ __device__ __forceinline__ EnvIn load_env_in(const Params& p, int b) {
 }
 
 __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, int8_t* world,
-                                            int8_t* gworld, bool any_done, const EnvIn& in) {
+                                            int8_t* gworld, bool any_done, const EnvIn& in,
+                                            bool upper_solid) {
   const int lane = threadIdx.x & (kWave - 1);
   const int sc = in.sc;
   const bool done = in.done;
@@ -1088,6 +1116,28 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
   }
   nx = fminf(fmaxf(nx, 1.0f), (float)p.X - 1.0f);
   nz = fminf(fmaxf(nz, 1.0f), (float)p.Z - 1.0f);
+
+  // Staged height (Params::ys): the env runs here only when its layers
+  // y >= ys are air, the raycast's samples (at most 3.975 from the eye) stay
+  // below ys, and the rendered eye (the respawn point when done) lies in the
+  // world below ys.  Otherwise nothing is written and the env is listed for
+  // the full-height pass.
+  if (p.ys < p.Y) {
+    const float rx = done ? p.spawn_x : nx, rz = done ? p.spawn_z : nz;
+    const float ry = (done ? p.spawn_y : ny) + 1.6f, ys = (float)p.ys;
+    const bool ok = !upper_solid && (ny + 1.6f) + 4.5f < ys && rx >= 0.0f && rx < (float)p.X &&
+                    ry >= 0.0f && ry < ys && rz >= 0.0f && rz < (float)p.Z;
+    if (!ok) {
+      if (lane == 0) {
+        uint32_t* list = p.fb + (size_t)slot * (p.B + 2);
+        const uint32_t k = atomicAdd(list, 1u);
+        if (k < (uint32_t)p.B) list[2 + k] = (uint32_t)b;
+      }
+      Frame f;
+      f.deferred = 1;
+      return f;
+    }
+  }
 
   // Raycast: sample k of 32 on lane k, first solid in-bounds sample by ballot.
   float reward = 0.0f;
@@ -1304,10 +1354,11 @@ __host__ __device__ __forceinline__ int item_lds_bytes(int epw, int X, int Y, in
   return epw * world_stride_bytes(X, Y, Z) + 2 * world_margin_bytes(X, Z);
 }
 
+// upper: one word per wave of the workgroup (staged-height check).
 template <int NT, int KIND, bool LIFT>
 __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, int wpe,
                                          int8_t* lds, const float* pal, const uint32_t* flat,
-                                         Frame* frames, float* su, float* sv) {
+                                         Frame* frames, float* su, float* sv, int* upper) {
   const int tid = threadIdx.x, wave = tid / kWave;
   const int epw = (NT / kWave) / wpe;
   const int e = wave / wpe;                                 // env slot within the item
@@ -1316,7 +1367,11 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   const int b = b0 + e;
   const bool active = (e < epw) && (b < p.B);
   const int nvox = p.X * p.Y * p.Z;
-  int8_t* world = lds + world_margin_bytes(p.X, p.Z) + e * world_stride_bytes(p.X, p.Y, p.Z);
+  // Staged layers (Params::ys; always all of them outside step items).
+  const bool staged = (KIND == kItemStep) && p.ys < p.Y;
+  const int nstage = staged ? p.X * p.ys * p.Z : nvox;
+  int8_t* world = lds + world_margin_bytes(p.X, p.Z) +
+                  e * world_stride_bytes(p.X, staged ? p.ys : p.Y, p.Z);
   int8_t* gworld = p.voxels + (int64_t)b * nvox;
   const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
   bool any_done = false;
@@ -1338,27 +1393,37 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   fill_uv(p, uvm, su, sv, tid, NT);
   if (active) {
     if (regen) store_fresh_world(p, gworld, world, nvox, et, ent);
-    else stage_world(world, gworld, nvox, et, ent);
+    else stage_world(world, gworld, nstage, et, ent);
+  }
+  if (staged) {
+    const bool found = active && any_not_air(gworld + nstage, nvox - nstage, et, ent);
+    const bool w = __ballot(found) != 0ull;
+    if ((tid & (kWave - 1)) == 0) upper[wave] = w ? 1 : 0;
   }
   __syncthreads();
   if (active && et < kWave) {
+    bool upper_solid = false;
+    if (staged)
+      for (int k = 0; k < wpe; ++k) upper_solid |= upper[e * wpe + k] != 0;
     Frame f;
-    if (KIND == kItemStep) f = env_update(p, slot, b, world, gworld, any_done, in);
+    if (KIND == kItemStep)
+      f = env_update(p, slot, b, world, gworld, any_done, in, upper_solid);
     else if (KIND == kItemResetRender) f = reset_update(p, slot, b, world);
     else if (KIND == kItemRenderBasis) f = frame_from_basis(p, b, world);
     else f = camera_update(p, b, world, KIND == kItemRenderState);
     if (et == 0) {
       frames[e] = f;
-      if (KIND == kItemStep && p.basis) store_basis(p, b, f);
+      if (KIND == kItemStep && p.basis && !f.deferred) store_basis(p, b, f);
     }
   }
   __syncthreads();
+  const bool go = active && !frames[e].deferred;
   if (LIFT) {
-    if (active && frames[e].lift)
+    if (go && frames[e].lift)
       solid_extent(p, world, &frames[e], (int)floorf(frames[e].ey), et, ent);
     __syncthreads();
   }
-  if (active) {
+  if (go) {
     // Every wave of the env writes the same 8 octant words before it reads
     // them; a wave's LDS operations complete in order.
     if ((et & (kWave - 1)) < 8)
@@ -1388,10 +1453,43 @@ __global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
   __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
   __shared__ float s_u[kMaxUV], s_v[kMaxUV];
+  __shared__ int s_upper[NT / kWave];
   load_palette(s_pal, s_flat, threadIdx.x);
   const int epw = (NT / kWave) / wpe;
   run_item<NT, KIND, LIFT>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_flat, s_frames,
-                           s_u, s_v);
+                           s_u, s_v, s_upper);
+}
+
+// Full-height pass after a staged-height step launch (Params::ys): the envs
+// that launch deferred, listed in fb[slot * (B + 2)], one env per workgroup
+// (all NT / 64 waves on it).  Launched with p.ys = Y; the list is complete
+// because the step launch precedes this one on the stream.  The last
+// workgroup to finish empties the list for the next launch on this slot.
+template <int NT, bool LIFT>
+__global__ __launch_bounds__(NT) void k_env_fb(Params p, int obs_mode) {
+  extern __shared__ __align__(16) int8_t s_lds[];
+  __shared__ float s_pal[kPaletteLen];
+  __shared__ uint32_t s_flat[kFlatLen];
+  __shared__ Frame s_frames[NT / kWave];
+  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
+  __shared__ int s_upper[NT / kWave];
+  const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
+  uint32_t* list = p.fb + (size_t)slot * (p.B + 2);
+  const int n = (int)min(list[0], (uint32_t)p.B);
+  load_palette(s_pal, s_flat, threadIdx.x);
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    __syncthreads();  // the previous env is done with LDS
+    run_item<NT, kItemStep, LIFT>(p, obs_mode, (int)list[2 + i], NT / kWave, s_lds, s_pal, s_flat,
+                                  s_frames, s_u, s_v, s_upper);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&list[1], 1u) == gridDim.x - 1) {
+      list[0] = 0u;
+      list[1] = 0u;
+      __threadfence();
+    }
+  }
 }
 
 // Several env groups (uneven batch sizes, world sizes, resolutions) in one
@@ -1422,7 +1520,7 @@ struct MultiArgs {
 };
 
 static_assert(sizeof(float) * (kPaletteLen + 2 * kMaxUV) + sizeof(uint32_t) * kFlatLen +
-                      sizeof(Frame) * (512 / kWave) + sizeof(int) * (kMaxGroups + 1) + 64 <=
+                      sizeof(Frame) * (512 / kWave) + sizeof(int) * (kMaxGroups + 1 + 512 / kWave) + 64 <=
                   kStaticLdsReserve,
               "static LDS of the env kernels exceeds its reserve");
 
@@ -1438,6 +1536,7 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
   __shared__ Frame s_frames[NT / kWave];
   __shared__ int s_item;
   __shared__ float s_u[kMaxUV], s_v[kMaxUV];
+  __shared__ int s_upper[NT / kWave];
   // The entry table and the descriptors are written by the host before the
   // launch and only read here: constant address space, so uniform reads are
   // scalar loads.
@@ -1478,7 +1577,7 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
     const int wpe = me.wpe;
     const int epw = (NT / kWave) / wpe;
     run_item<NT, kItemStep, LIFT>(p, me.obs_mode, (item - me.item_begin) * epw, wpe, s_lds, s_pal,
-                                  s_flat, s_frames, s_u, s_v);
+                                  s_flat, s_frames, s_u, s_v, s_upper);
     if (!PERSIST) break;
   }
   if (PERSIST && threadIdx.x == 0) {

@@ -167,6 +167,46 @@ def test_large_lds_worlds(dims, img):
     print(st.summary())
 
 
+@pytest.mark.parametrize("obs_dtype", ["uint8", "uint8_prebasis"])
+def test_staged_height_deferral(obs_dtype):
+    """Worlds above the LDS-staged layers: envs with solids up there (a ceiling
+    slab over the view, one lone block) or with agents standing high take the
+    full-height pass; every env equals the oracle on every step, through done
+    and regeneration steps, with batched launches of the same group in between
+    (they advance the episode slot without the staged path)."""
+    import minecu
+    B, T, Y = 16, 36, 32
+    env, orc = _envs(batch_size=B, world_size=Y, render_width=32, render_height=24,
+                     obs_dtype=obs_dtype, episode_length=12)
+    st = Stats()
+    env.reset()
+    orc.reset()
+    torch.cuda.synchronize()
+    vox = env.voxels.cpu().numpy()
+    pos = env.positions.cpu().numpy()
+    vox[0:4, 24, :, :] = 9          # cobblestone ceiling over envs 0-3
+    vox[4, 30, 16, 20] = 8          # one brick near the top of env 4
+    pos[5:8, 1] = 26.0              # agents of envs 5-7 start high and fall
+    env.voxels.copy_(torch.from_numpy(vox))
+    env.positions.copy_(torch.from_numpy(pos))
+    orc.voxels[...] = vox
+    orc.positions[...] = pos
+    acts = _break_heavy(T, B, 21)
+    batch = minecu.EnvBatch()
+    for t in range(T):
+        a = acts[t]
+        if t % 4 == 2:
+            og, rg, dg = batch.step([env], [torch.as_tensor(a).cuda()])[0]
+        else:
+            og, rg, dg = env.step(torch.as_tensor(a).cuda())
+        oo, ro, do = orc.step(np.asarray(a))
+        compare_exact("rewards", rg.cpu().numpy(), ro, f"staged step {t}")
+        compare_exact("dones", dg.cpu().numpy(), do, f"staged step {t}")
+        compare_env(env, orc, st, f"staged step {t}")
+    batch.close()
+    print(st.summary())
+
+
 def test_partial_reset_mask():
     env, orc = _envs(batch_size=32, world_size=16, render_width=32, render_height=32,
                      obs_dtype="uint8", episode_length=25)
