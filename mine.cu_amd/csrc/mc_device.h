@@ -591,6 +591,10 @@ struct OctPos {
   __device__ __forceinline__ uint32_t pick(bool bx, bool by, bool bz) const {
     return v[(bx ? 1 : 0) | (by ? 2 : 0) | (bz ? 4 : 0)];
   }
+  // The same word read again from LDS (not kept in a register meanwhile).
+  __device__ __forceinline__ uint32_t reread(bool bx, bool by, bool bz) const {
+    return reinterpret_cast<const volatile uint32_t*>(v)[(bx ? 1 : 0) | (by ? 2 : 0) | (bz ? 4 : 0)];
+  }
 };
 
 template <bool LIFT>
@@ -666,11 +670,11 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
     bool out = (pos & kGuard) != 0u;
     int blk = (int)*vp;
     bool lim = (max_steps <= 0) | (t >= vd);
-    bool last_x = false, last_y = false;
-    int step = 0;
+    uint32_t pinc = 0u;  // the last step's increment of pos: its axis
     // The reference's iteration is (limit check, lookup, advance); here the
     // loop test holds the limit check and the lookup, the advance is
     // branch-free, and the next voxel's LDS read is issued before the test.
+    int step = 0;
     while (!(lim | out | (blk >= 0))) {
       const bool cx = (tm_x < tm_y) & (tm_x < tm_z);
       const bool cy = !cx & (tm_y < tm_z);
@@ -680,17 +684,24 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
       tm_y = cy ? tm_y + td_y : tm_y;
       tm_z = cz ? tm_z + td_z : tm_z;
       vp += cx ? dix : (cy ? diy : diz);
-      pos += cx ? 1u : (cy ? (1u << 10) : (1u << 20));
+      pinc = cx ? 1u : (cy ? (1u << 10) : (1u << 20));
+      pos += pinc;
       out = (pos & kGuard) != 0u;
       blk = (int)*vp;  // one step past the edge at most: a margin byte
       ++step;
       lim = (step >= max_steps) | (!p.vd_far & (t >= vd));
-      last_x = cx;
-      last_y = cy;
     }
-    if (!lim & !out & (blk >= 0)) {
+    // The walk's results are read back from vector registers (pos, blk, t,
+    // pinc; the step count is the sum of pos's fields' advances, the start
+    // word read again from LDS): per-lane booleans kept past the loop would
+    // be merged into scalar lane masks on every iteration.
+    const uint32_t moved = pos - op.reread(step_x > 0, step_y > 0, step_z > 0);
+    const int steps = (int)((moved & 0x3ffu) + ((moved >> 10) & 0x3ffu) + (moved >> 20));
+    const bool lim_end = (steps >= max_steps) | ((steps == 0 | !p.vd_far) & (t >= vd));
+    asm volatile("" : "+v"(blk));  // a fresh test, not the loop's lane mask
+    if (!lim_end & ((pos & kGuard) == 0u) & (blk >= 0)) {
       hit_block = blk;
-      hit_face = (step == 0) ? -1 : (last_x ? 0 : (last_y ? 1 : 2));
+      hit_face = (pinc == 0u) ? -1 : (pinc == 1u ? 0 : (pinc == (1u << 10) ? 1 : 2));
     }
   } else {
     const float t1 = (0.0f - f.ex) * inv_x, t2 = ((float)X - f.ex) * inv_x;
@@ -812,7 +823,12 @@ __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, const
       ++step;
       lim = step >= max_steps;
     }
-    return (!lim & !out & (blk >= 0)) ? blk : -1;
+    // Results from vector registers (see trace()): the step count is the sum
+    // of pos's fields' advances.
+    const uint32_t moved = pos - op.reread(step_x > 0, step_y > 0, step_z > 0);
+    const int steps = (int)((moved & 0x3ffu) + ((moved >> 10) & 0x3ffu) + (moved >> 20));
+    asm volatile("" : "+v"(blk));  // a fresh test, not the loop's lane mask
+    return ((steps < max_steps) & ((pos & kGuard) == 0u) & (blk >= 0)) ? blk : -1;
   }
   int vx = (int)floorf(f.ex), vy = (int)floorf(f.ey), vz = (int)floorf(f.ez);
   T tm_x = (T)((ray_x >= 0) ? ((float)(vx + 1) - f.ex) * inv_x : ((float)vx - f.ex) * inv_x);
@@ -919,15 +935,25 @@ __device__ __forceinline__ void store_rgb8(uint8_t* out, int p, uint32_t c) {
 }
 
 // Tile loop over env b's image: fn(px, py) for every pixel, one per lane.
+// The wave index is uniform: the tile row and column advance in scalar
+// registers.
 template <typename PixelFn>
 __device__ __forceinline__ void for_tiles(const Params& p, int tid, int nt, PixelFn fn) {
-  const int lane = tid & (kWave - 1), wv = tid >> 6, nwv = nt >> 6;
+  const int lane = tid & (kWave - 1), nwv = nt >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int tx = lane & 7, ty = lane >> 3;
   const int tcols = (p.W + 7) >> 3, ntiles = tcols * ((p.H + 7) >> 3);
+  const int dr = nwv / tcols, dc = nwv - dr * tcols;
+  int tr = wv / tcols, tc = wv - tr * tcols;
   for (int t = wv; t < ntiles; t += nwv) {
-    const int tr = t / tcols, tc = t - tr * tcols;
     const int py = tr * 8 + ty, px = tc * 8 + tx;
     if (py < p.H && px < p.W) fn(px, py);
+    tr += dr;
+    tc += dc;
+    if (tc >= tcols) {
+      tc -= tcols;
+      ++tr;
+    }
   }
 }
 
