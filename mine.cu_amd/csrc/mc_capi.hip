@@ -259,7 +259,7 @@ int launch_deferred(const mc::Params& q, int obs_mode, int lift, hipStream_t st)
   p.ys = p.Y;
   const int full = mc::item_lds_bytes(1, p.X, p.Y, p.Z);
   if (full > lds_budget()) return fail(MC_E_UNSUPPORTED, "world does not fit the LDS budget");
-  const dim3 grid(std::min(p.B, 128));
+  const dim3 grid(std::min(p.B, 64));
   if (full > 80 * 1024) {
     if (lift) hipLaunchKernelGGL((mc::k_env_fb<512, true>), grid, dim3(512), full, st, p, obs_mode);
     else hipLaunchKernelGGL((mc::k_env_fb<512, false>), grid, dim3(512), full, st, p, obs_mode);
@@ -481,8 +481,16 @@ int mc_env_create(const mc_config* cfg, const mc_state* st, mc_env** out) {
     e->base.ring = e->ring;
     he = hipMalloc(reinterpret_cast<void**>(&e->desc), sizeof(mc::Params));
   }
-  if (he == hipSuccess)
-    he = hipMemcpy(e->desc, &e->base, sizeof(mc::Params), hipMemcpyHostToDevice);
+  if (he == hipSuccess) {
+    // Batched steps read this copy: it carries the staged height and the
+    // deferred-env lists (the host-side base stages whole worlds).
+    mc::Params d = e->base;
+    if (e->fb) {
+      d.ys = e->ys;
+      d.fb = e->fb;
+    }
+    he = hipMemcpy(e->desc, &d, sizeof(mc::Params), hipMemcpyHostToDevice);
+  }
   if (he != hipSuccess) {
     if (e->ring) (void)hipFree(e->ring);
     if (e->desc) (void)hipFree(e->desc);
@@ -1063,16 +1071,23 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
     double cost;
   };
   std::vector<G> gs(n);
-  int lds = 0;
+  int lds = 0, lds_full = 0;  // the step launch; the full-height pass (0: none)
   for (int i = 0; i < n; ++i) {
     const mc_config& c = envs[i]->cfg;
+    const int ys = envs[i]->fb ? envs[i]->ys : c.world_y;  // layers the descriptor stages
     Shape s{nt, std::min(envs[i]->shape.wpe, nt / mc::kWave), lift ? 1 : 0};
-    s = fit_lds(s, c.world_x, c.world_y, c.world_z);
+    s = fit_lds(s, c.world_x, ys, c.world_z);
     const int stride = mc::world_stride_bytes(c.world_x, c.world_y, c.world_z);
-    const int item_lds = mc::item_lds_bytes(epw_of(s), c.world_x, c.world_y, c.world_z);
+    const int item_lds = mc::item_lds_bytes(epw_of(s), c.world_x, ys, c.world_z);
     if (item_lds > lds_budget())
       return fail(MC_E_UNSUPPORTED, "group %d: world does not fit the LDS budget", i);
     lds = std::max(lds, item_lds);
+    if (envs[i]->fb) {
+      const int full = mc::item_lds_bytes(1, c.world_x, c.world_y, c.world_z);
+      if (full > lds_budget())
+        return fail(MC_E_UNSUPPORTED, "group %d: world does not fit the LDS budget", i);
+      lds_full = std::max(lds_full, full);
+    }
     const int epw = epw_of(s);
     const double per_env = (double)c.width * c.height * (1.0 + c.world_y / 16.0) + stride;
     gs[i] = G{i, s.wpe, (c.batch_size + epw - 1) / epw, per_env * epw};
@@ -1116,6 +1131,19 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
                             : (lift ? launch_multi<256, true>(b, a, lds, S(stream))
                                     : launch_multi<256, false>(b, a, lds, S(stream)));
   if (r) return r;
+  if (lds_full > 0) {
+    // Full-height pass over the envs the staged groups deferred.
+    const dim3 grid(64);
+    if (lds_full > 80 * 1024) {
+      if (lift) hipLaunchKernelGGL((mc::k_multi_fb<512, true>), grid, dim3(512), lds_full, S(stream), a);
+      else hipLaunchKernelGGL((mc::k_multi_fb<512, false>), grid, dim3(512), lds_full, S(stream), a);
+    } else {
+      if (lift) hipLaunchKernelGGL((mc::k_multi_fb<256, true>), grid, dim3(256), lds_full, S(stream), a);
+      else hipLaunchKernelGGL((mc::k_multi_fb<256, false>), grid, dim3(256), lds_full, S(stream), a);
+    }
+    const int rf = check_launch("batched deferred-env launch");
+    if (rf) return rf;
+  }
   e = hipEventRecord(b->ev[slot], S(stream));
   if (e != hipSuccess) return fail(MC_E_HIP, "batch event: %s", hipGetErrorString(e));
   b->ev_live[slot] = true;

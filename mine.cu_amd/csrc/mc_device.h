@@ -1502,6 +1502,7 @@ __global__ __launch_bounds__(NT) void k_env_fb(Params p, int obs_mode) {
   const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
   uint32_t* list = p.fb + (size_t)slot * (p.B + 2);
   const int n = (int)min(list[0], (uint32_t)p.B);
+  if (n == 0) return;  // nothing deferred: the list is already empty
   load_palette(s_pal, s_flat, threadIdx.x);
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     __syncthreads();  // the previous env is done with LDS
@@ -1613,6 +1614,62 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
       a.counters[0] = 0u;
       a.counters[1] = 0u;
       __threadfence();
+    }
+  }
+}
+
+// Full-height pass after a batched launch whose groups stage fewer layers
+// (Params::ys, fb in their descriptors): group by group, the envs each group
+// deferred, one env per workgroup with its whole world in LDS (the launch's
+// LDS fits the largest).  The last workgroup done with a group's list empties
+// it for the next launch on that slot.
+template <int NT, bool LIFT>
+__global__ __launch_bounds__(NT) void k_multi_fb(MultiArgs a) {
+  extern __shared__ __align__(16) int8_t s_lds[];
+  __shared__ float s_pal[kPaletteLen];
+  __shared__ uint32_t s_flat[kFlatLen];
+  __shared__ Frame s_frames[NT / kWave];
+  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
+  __shared__ int s_upper[NT / kWave];
+  typedef __attribute__((address_space(4))) const MultiEntry CEntry;
+  typedef __attribute__((address_space(4))) const uint32_t CWord;
+  CEntry* entries = (CEntry*)a.entries;
+  bool palette = false;
+  for (int g = 0; g < a.n_groups; ++g) {
+    CEntry& me = entries[g];
+    Params p;
+    {
+      CWord* src = (CWord*)me.desc;
+      uint32_t* dst = reinterpret_cast<uint32_t*>(&p);
+#pragma unroll
+      for (int i = 0; i < (int)(sizeof(Params) / 4); ++i) dst[i] = src[i];
+    }
+    if (!p.fb || p.ys >= p.Y) continue;
+    uint32_t* list = p.fb + (size_t)me.slot * (p.B + 2);
+    const int n = (int)min(list[0], (uint32_t)p.B);
+    if (n == 0) continue;
+    if (!palette) {
+      load_palette(s_pal, s_flat, threadIdx.x);
+      palette = true;
+    }
+    p.actions = me.actions;
+    p.look = me.look;
+    p.dones = me.dones;
+    p.action_kind = me.action_kind;
+    p.slot = me.slot;
+    p.ys = p.Y;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+      __syncthreads();  // the previous env is done with LDS
+      run_item<NT, kItemStep, LIFT>(p, me.obs_mode, (int)list[2 + i], NT / kWave, s_lds, s_pal,
+                                    s_flat, s_frames, s_u, s_v, s_upper);
+    }
+    if (threadIdx.x == 0) {
+      __threadfence();
+      if (atomicAdd(&list[1], 1u) == gridDim.x - 1) {
+        list[0] = 0u;
+        list[1] = 0u;
+        __threadfence();
+      }
     }
   }
 }
