@@ -317,7 +317,11 @@ def main():
         out["float32_obs_value"] = B * min(K, 200) / (f_ms / 1e3)
     if not args.no_serve:
         out["served"] = run_serving(args.serve_seconds)
-        out["served"]["closed_loop_frac_of_plain"] = out["served"]["closed_loop"]["env_steps_per_s"] / value
+        closed = out["served"]["closed_loop"]["env_steps_per_s"]
+        out["served"]["closed_loop_frac_of_plain"] = closed / value
+        # The serving target: the mixed workload holds 80% of the 49.6M
+        # env-steps/s plain-step figure, with p99 under 2 ms.
+        out["served"]["closed_loop_frac_of_49_6M"] = closed / 49.6e6
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args.obs)
     print(json.dumps(out))
