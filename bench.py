@@ -163,11 +163,13 @@ def serving_clients(seed=4, n_clients=64):
     return out
 
 
-def run_serving(seconds=60.0, rate=200.0, seed=4, budget_gb=64, closed_steps=200):
+def run_serving(seconds=60.0, rate=200.0, seed=4, budget_gb=64, closed_steps=200, n_streams=2):
     """Open loop: Poisson step requests per client at `rate`/s for `seconds`
     through the continuous-batching server; per-request latency = completion
-    seen by the server loop minus submit.  Closed loop: every client steps
-    back-to-back (`closed_steps` each) to show capacity."""
+    seen by the server loop minus submit.  Then an open loop at the serving
+    target's load (offered env-steps/s = 80% of 49.6M) for up to 10 s, with
+    the arrivals released by the server's loop thread.  Closed loop: every
+    client steps back-to-back (`closed_steps` each) to show capacity."""
     import numpy as np
     import torch
     import minecu
@@ -184,7 +186,7 @@ def run_serving(seconds=60.0, rate=200.0, seed=4, budget_gb=64, closed_steps=200
         envs.append(e)
         acts.append(torch.randint(0, 11, (256, c["batch"]), generator=g, device=dev))
     torch.cuda.synchronize()
-    srv = StepServer(hbm_budget_bytes=budget_gb << 30, n_streams=2)
+    srv = StepServer(hbm_budget_bytes=budget_gb << 30, n_streams=n_streams)
     gids = [srv.add_group(e) for e in envs]
     srv.autotune(8)
     torch.cuda.synchronize()
@@ -226,6 +228,44 @@ def run_serving(seconds=60.0, rate=200.0, seed=4, budget_gb=64, closed_steps=200
         "p99_ms": float(np.percentile(lat_ms, 99)), "max_ms": float(lat_ms.max()),
         "launches": st["launches"], "requests_per_launch": st["requests"] / max(1, st["launches"]),
         "wall_s": wall}
+    # Open loop at the serving target's load: Poisson arrivals whose offered
+    # env-steps/s are 80% of the 49.6M plain-step figure, released by the
+    # server's own loop thread at their planned times (latency counted from
+    # the planned arrival).
+    rate80 = 0.8 * 49.6e6 / float(batch_of.sum())
+    seconds80 = min(seconds, 10.0)
+    t80, w80 = [], []
+    for i in range(len(clients)):
+        n = rng.poisson(rate80 * seconds80 * 1.2) + 16
+        t = np.cumsum(rng.exponential(1.0 / rate80, n))
+        t = t[t < seconds80]
+        t80.append(t)
+        w80.append(np.full(len(t), i))
+    t80 = np.concatenate(t80)
+    w80 = np.concatenate(w80)
+    o80 = np.argsort(t80, kind="stable")
+    t80, w80 = t80[o80], w80[o80]
+    a80 = []
+    for i in w80.tolist():
+        a80.append(acts[i][count[i] % 256])
+        count[i] += 1
+    torch.cuda.synchronize()
+    st0 = srv.stats()
+    srv.schedule([gids[i] for i in w80.tolist()], a80, t80)
+    srv.drain()
+    rec80 = srv.records()
+    st1 = srv.stats()
+    lat80 = (rec80["t_done_ns"] - rec80["t_submit_ns"]) / 1e6
+    span80 = (rec80["t_done_ns"].max() - rec80["t_submit_ns"].min()) / 1e9
+    n_launch80 = st1["launches"] - st0["launches"]
+    open_80 = {
+        "requests": int(len(rec80)), "rate_per_client": rate80, "seconds": seconds80,
+        "offered_env_steps_per_s": float(batch_of[w80].sum() / seconds80),
+        "served_env_steps_per_s": float(batch_of[rec80["group"]].sum() / span80),
+        "p50_ms": float(np.percentile(lat80, 50)), "p99_ms": float(np.percentile(lat80, 99)),
+        "max_ms": float(lat80.max()), "launches": n_launch80,
+        "requests_per_launch": (st1["requests"] - st0["requests"]) / max(1, n_launch80)}
+    del a80
     # Closed loop (capacity): replay every client back to back.
     reps = [acts[i][:min(256, closed_steps)].repeat((closed_steps + 255) // 256, 1)[:closed_steps]
             .contiguous() for i in range(len(clients))]
@@ -242,8 +282,8 @@ def run_serving(seconds=60.0, rate=200.0, seed=4, budget_gb=64, closed_steps=200
               "steps_per_client": closed_steps}
     srv.close()
     return {"clients": len(clients), "total_batch": int(batch_of.sum()), "rate_per_client": rate,
-            "seconds": seconds, "budget_gb": budget_gb, "open_loop": open_loop,
-            "closed_loop": closed,
+            "seconds": seconds, "budget_gb": budget_gb, "streams": n_streams, "open_loop": open_loop,
+            "open_loop_at_80pct_of_49_6M": open_80, "closed_loop": closed,
             "workload": "64 clients, batch log-uniform [16,4096], world 16^3/32^3, obs 32^2/64^2 "
                         "uint8, Poisson 200/s per client, seed 4"}
 
@@ -259,6 +299,7 @@ def main():
     ap.add_argument("--no-f32", action="store_true", help="skip the float32-obs side measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-streams", type=int, default=2)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -316,7 +357,7 @@ def main():
         f_ms, _, f_launch = run_gpu(w, "float32", min(K, 200), args.warmup)
         out["float32_obs_value"] = B * min(K, 200) / (f_ms / 1e3)
     if not args.no_serve:
-        out["served"] = run_serving(args.serve_seconds)
+        out["served"] = run_serving(args.serve_seconds, n_streams=args.serve_streams)
         closed = out["served"]["closed_loop"]["env_steps_per_s"]
         out["served"]["closed_loop_frac_of_plain"] = closed / value
         # The serving target: the mixed workload holds 80% of the 49.6M

@@ -428,7 +428,14 @@ int mc_server_set_replay(mc_server* s, int gid, const void* actions_base, int64_
                          int64_t count, int action_kind, const float* look_base,
                          int64_t look_stride_bytes, uint8_t* dones);
 int mc_server_start_replay(mc_server* s, int gid);
-/* Blocks until no request is pending, in flight or left to replay. */
+/* Open loop at planned times: request k of group gids[k] (actions[k], of
+ * action_kind, no look) arrives t_offset_ns[k] after this call; offsets must
+ * be non-decreasing.  The loop thread queues each one at its arrival time and
+ * records that time as its submit time, so a late release counts against its
+ * latency.  Adds to any schedule still running. */
+int mc_server_schedule(mc_server* s, const int* gids, const void* const* actions,
+                       int action_kind, const int64_t* t_offset_ns, int64_t n);
+/* Blocks until no request is pending, in flight, scheduled or left to replay. */
 int mc_server_drain(mc_server* s);
 int mc_server_take_records(mc_server* s, mc_server_record* out, int max, int* n);
 int mc_server_stats(mc_server* s, int64_t* launches, int64_t* env_steps, int64_t* requests,

@@ -526,6 +526,15 @@ __device__ __forceinline__ float recip_core(float d) {
   const float e = fmaf(-d, r, 1.0f);
   return fmaf(e, r, r);
 }
+// Correctly rounded 1 / d: the Newton-refined estimate above, then one
+// remainder correction (Markstein's reciprocal step, exact remainder by FMA).
+// The same bits as 1.0f / d for every |d| in [2^-90, 2^90] (exhaustive GPU
+// test).
+__device__ __forceinline__ float rcp_cr(float d) {
+  const float r = recip_core(d);
+  const float e = fmaf(-d, r, 1.0f);
+  return fmaf(e, r, r);
+}
 __device__ __forceinline__ float div_core(float n, float d, float r) {
   float q = n * r;
   float rem = fmaf(-d, q, n);
@@ -557,9 +566,9 @@ __device__ __forceinline__ void ray_dir(float& x, float& y, float& z, float& ix,
       y = div_core(y, len, rl);
       z = div_core(z, len, rl);
     }
-    ix = (x != 0.0f) ? div_core(1.0f, x, recip_core(x)) : big;
-    iy = (y != 0.0f) ? div_core(1.0f, y, recip_core(y)) : big;
-    iz = (z != 0.0f) ? div_core(1.0f, z, recip_core(z)) : big;
+    ix = (x != 0.0f) ? rcp_cr(x) : big;
+    iy = (y != 0.0f) ? rcp_cr(y) : big;
+    iz = (z != 0.0f) ? rcp_cr(z) : big;
   } else {
     const float len = sqrtf(len2);
     if (RSQRT) {
@@ -591,9 +600,11 @@ struct OctPos {
   __device__ __forceinline__ uint32_t pick(bool bx, bool by, bool bz) const {
     return v[(bx ? 1 : 0) | (by ? 2 : 0) | (bz ? 4 : 0)];
   }
-  // The same word read again from LDS (not kept in a register meanwhile).
+  // The same word read again from LDS (not kept in a register meanwhile);
+  // through an LDS-space pointer, so it stays a ds_read.
   __device__ __forceinline__ uint32_t reread(bool bx, bool by, bool bz) const {
-    return reinterpret_cast<const volatile uint32_t*>(v)[(bx ? 1 : 0) | (by ? 2 : 0) | (bz ? 4 : 0)];
+    typedef __attribute__((address_space(3))) const volatile uint32_t lds_u32;
+    return ((lds_u32*)v)[(bx ? 1 : 0) | (by ? 2 : 0) | (bz ? 4 : 0)];
   }
 };
 
@@ -634,7 +645,7 @@ __device__ __forceinline__ uint32_t oct_pos(const Params& p, const Frame& f, int
 // result is sky.  Those bounds tighten the per-axis limits of the packed
 // position, so the same test ends the ray there.
 // ---------------------------------------------------------------------------
-template <bool LIFT>
+template <bool LIFT, bool VDF>
 __device__ __forceinline__ void trace(const Params& p, const Frame& f, const OctPos& op, float u,
                                       float v, const int8_t* __restrict__ world,
                                       const float* __restrict__ pal, float rgb[3]) {
@@ -676,10 +687,13 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
     // branch-free, and the next voxel's LDS read is issued before the test.
     int step = 0;
     while (!(lim | out | (blk >= 0))) {
-      const bool cx = (tm_x < tm_y) & (tm_x < tm_z);
-      const bool cy = !cx & (tm_y < tm_z);
-      const bool cz = !cx & !cy;
+      // The reference's order (x if tm_x < tm_y and tm_x < tm_z, else y if
+      // tm_y < tm_z, else z) from the minimum t: z exactly when tm_z is a
+      // minimum, else y exactly when tm_y is one (no t_max is NaN).
       t = fminf(fminf(tm_x, tm_y), tm_z);
+      const bool cz = tm_z == t;
+      const bool cy = !cz & (tm_y == t);
+      const bool cx = !cz & !cy;
       tm_x = cx ? tm_x + td_x : tm_x;
       tm_y = cy ? tm_y + td_y : tm_y;
       tm_z = cz ? tm_z + td_z : tm_z;
@@ -689,7 +703,7 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
       out = (pos & kGuard) != 0u;
       blk = (int)*vp;  // one step past the edge at most: a margin byte
       ++step;
-      lim = (step >= max_steps) | (!p.vd_far & (t >= vd));
+      lim = (step >= max_steps) | (!VDF & (t >= vd));
     }
     // The walk's results are read back from vector registers (pos, blk, t,
     // pinc; the step count is the sum of pos's fields' advances, the start
@@ -697,7 +711,7 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
     // be merged into scalar lane masks on every iteration.
     const uint32_t moved = pos - op.reread(step_x > 0, step_y > 0, step_z > 0);
     const int steps = (int)((moved & 0x3ffu) + ((moved >> 10) & 0x3ffu) + (moved >> 20));
-    const bool lim_end = (steps >= max_steps) | ((steps == 0 | !p.vd_far) & (t >= vd));
+    const bool lim_end = (steps >= max_steps) | (((steps == 0) | !VDF) & (t >= vd));
     asm volatile("" : "+v"(blk));  // a fresh test, not the loop's lane mask
     if (!lim_end & ((pos & kGuard) == 0u) & (blk >= 0)) {
       hit_block = blk;
@@ -994,7 +1008,7 @@ __device__ __forceinline__ void render_env_flat(const Params& p, int b, const in
   render_rgb8(p, b, uvm, tid, nt, su, sv, color);
 }
 
-template <int OBS, bool LIFT>
+template <int OBS, bool LIFT, bool VDF>
 __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t* world,
                                            const float* pal, const Frame& f, const OctPos& op,
                                            int tid, int nt, const float* su, const float* sv) {
@@ -1017,7 +1031,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       if (f.fill_block >= 0) {
         for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
       } else {
-        trace<LIFT>(p, f, op, u, v, world, pal, rgb);
+        trace<LIFT, VDF>(p, f, op, u, v, world, pal, rgb);
       }
       return to_u8(rgb[0]) | (to_u8(rgb[1]) << 8) | (to_u8(rgb[2]) << 16);
     };
@@ -1030,7 +1044,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       if (f.fill_block >= 0) {
         for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
       } else {
-        trace<LIFT>(p, f, op, ucol(px), vrow(py), world, pal, rgb);
+        trace<LIFT, VDF>(p, f, op, ucol(px), vrow(py), world, pal, rgb);
       }
       gf32* o = (gf32*)(out + ((int64_t)py * W + px) * 3);
       o[0] = rgb[0];
@@ -1461,8 +1475,11 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
       render_env_flat<LIFT, _Float16>(p, b, world, flat, f, op, uvm, et, ent, su, sv);
     else if (uvm != kUvShaded)
       render_env_flat<LIFT, float>(p, b, world, flat, f, op, uvm, et, ent, su, sv);
-    else if (obs_mode == kObsF32) render_env<1, LIFT>(p, b, world, pal, f, op, et, ent, su, sv);
-    else render_env<0, LIFT>(p, b, world, pal, f, op, et, ent, su, sv);
+    else if (obs_mode == kObsF32 && p.vd_far)
+      render_env<1, LIFT, true>(p, b, world, pal, f, op, et, ent, su, sv);
+    else if (obs_mode == kObsF32) render_env<1, LIFT, false>(p, b, world, pal, f, op, et, ent, su, sv);
+    else if (p.vd_far) render_env<0, LIFT, true>(p, b, world, pal, f, op, et, ent, su, sv);
+    else render_env<0, LIFT, false>(p, b, world, pal, f, op, et, ent, su, sv);
     if (KIND == kItemStep && any_done) store_fresh_world(p, gworld, nullptr, nvox, et, ent);
   }
 }
@@ -1823,7 +1840,7 @@ __global__ void k_debug_fast_math(int mode, uint32_t begin, uint32_t count, uint
       const float x = __uint_as_float(bits);
       const float ax = fabsf(x);
       if (!(ax >= 0x1p-90f && ax <= 0x1p90f)) continue;
-      const float f = div_core(1.0f, x, recip_core(x));
+      const float f = rcp_cr(x);
       const float g = 1.0f / x;
       ++checked;
       if (__float_as_uint(f) != __float_as_uint(g)) { ++bad; bad_in = bits; }

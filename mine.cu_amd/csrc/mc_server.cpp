@@ -18,6 +18,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <iterator>
 #include <mutex>
 #include <new>
 #include <string>
@@ -94,6 +95,10 @@ struct mc_server {
   std::vector<Group> groups;
   std::deque<int> waiting;
   std::deque<Req> pending;
+  // Planned arrivals (mc_server_schedule), in time order: released into
+  // `pending` by the loop thread once their time has come.
+  std::vector<Req> sched;
+  size_t sched_next = 0;
   std::deque<Launch> inflight;
   std::vector<hipStream_t> streams;
   std::vector<char> stream_busy;
@@ -143,6 +148,25 @@ struct mc_server {
     }
   }
 
+  // Queue the planned arrivals whose time has come (t_submit = arrival).
+  void release_scheduled_locked(int64_t t) {
+    while (sched_next < sched.size() && sched[sched_next].t_submit <= t) {
+      const Req& r = sched[sched_next++];
+      if (groups[r.gid].state != kAdmitted) {
+        failed++;
+        last_err = "scheduled request for a group that is no longer admitted";
+        done.push_back(mc_server_record{r.id, r.gid, r.t_submit, -1, -1});
+        continue;
+      }
+      pending.push_back(r);
+      groups[r.gid].pending++;
+    }
+    if (sched_next == sched.size() && !sched.empty()) {
+      sched.clear();
+      sched_next = 0;
+    }
+  }
+
   void loop() {
     (void)hipSetDevice(device);
     std::vector<Req> take;
@@ -157,7 +181,9 @@ struct mc_server {
       take.clear();
       {
         std::unique_lock<std::mutex> lk(mu);
-        retire_locked(now_ns());
+        const int64_t t_now = now_ns();
+        retire_locked(t_now);
+        if (running) release_scheduled_locked(t_now);
         if (!running && pending.empty() && inflight.empty()) break;
         for (size_t i = 0; i < streams.size(); ++i)
           if (!stream_busy[i]) {
@@ -190,11 +216,16 @@ struct mc_server {
           }
         }
         if (take.empty()) {
-          if (!inflight.empty()) {
+          // Sleep until a submit, or until shortly before the next planned
+          // arrival (then poll, as while launches are in flight).
+          int64_t wait_ns = 2000000;
+          if (sched_next < sched.size())
+            wait_ns = std::min<int64_t>(wait_ns, sched[sched_next].t_submit - now_ns() - 150000);
+          if (!inflight.empty() || wait_ns <= 0) {
             lk.unlock();
             std::this_thread::yield();
           } else {
-            cv.wait_for(lk, std::chrono::milliseconds(2));
+            cv.wait_for(lk, std::chrono::nanoseconds(wait_ns));
           }
           continue;
         }
@@ -403,6 +434,43 @@ int mc_server_start_replay(mc_server* s, int gid) {
   return MC_OK;
 }
 
+int mc_server_schedule(mc_server* s, const int* gids, const void* const* actions,
+                       int action_kind, const int64_t* t_offset_ns, int64_t n) {
+  if (!s || n < 0 || (n > 0 && (!gids || !actions || !t_offset_ns)))
+    return sfail(MC_E_INVALID, "bad argument");
+  if (action_kind != MC_ACT_DISCRETE && action_kind != MC_ACT_BUTTONS)
+    return sfail(MC_E_INVALID, "bad action kind %d", action_kind);
+  if (action_kind == MC_ACT_BUTTONS) return sfail(MC_E_INVALID, "scheduled requests carry no look");
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (int64_t k = 0; k < n; ++k) {
+      if (gids[k] < 0 || gids[k] >= (int)s->groups.size() || s->groups[gids[k]].state != kAdmitted)
+        return sfail(MC_E_INVALID, "request %lld: group %d is not admitted", (long long)k, gids[k]);
+      if (!actions[k]) return sfail(MC_E_INVALID, "request %lld: actions is NULL", (long long)k);
+      if (t_offset_ns[k] < 0 || (k > 0 && t_offset_ns[k] < t_offset_ns[k - 1]))
+        return sfail(MC_E_INVALID, "arrival offsets must be non-negative and non-decreasing");
+    }
+    // Merge with what is left of a running schedule, keeping time order.
+    std::vector<Req> left(s->sched.begin() + (int64_t)s->sched_next, s->sched.end());
+    const int64_t t0 = now_ns();
+    std::vector<Req> add;
+    add.reserve((size_t)n);
+    for (int64_t k = 0; k < n; ++k)
+      add.push_back(Req{s->next_id++, gids[k], actions[k], action_kind, nullptr, nullptr,
+                        t0 + t_offset_ns[k]});
+    s->sched.clear();
+    s->sched.reserve(left.size() + add.size());
+    // Completion records are appended under the lock: reserve now, so no
+    // reallocation of a long record list stalls the loop mid-schedule.
+    s->done.reserve(s->done.size() + left.size() + add.size());
+    std::merge(left.begin(), left.end(), add.begin(), add.end(), std::back_inserter(s->sched),
+               [](const Req& a, const Req& b) { return a.t_submit < b.t_submit; });
+    s->sched_next = 0;
+  }
+  s->cv.notify_one();
+  return MC_OK;
+}
+
 int mc_server_drain(mc_server* s) {
   if (!s) return sfail(MC_E_INVALID, "server is NULL");
   for (;;) {
@@ -410,7 +478,9 @@ int mc_server_drain(mc_server* s) {
       std::lock_guard<std::mutex> lk(s->mu);
       bool replay_left = false;
       for (const Group& g : s->groups) replay_left |= (g.rep_base && g.rep_next < g.rep_count);
-      if (s->pending.empty() && s->inflight.empty() && !replay_left) break;
+      if (s->pending.empty() && s->inflight.empty() && !replay_left &&
+          s->sched_next >= s->sched.size())
+        break;
     }
     std::this_thread::sleep_for(std::chrono::microseconds(100));
   }

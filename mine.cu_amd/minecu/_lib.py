@@ -96,6 +96,7 @@ SIGNATURES = {
     "mc_server_submit": ([_P, _I, _P, _I, _P, _P, _P], _I),
     "mc_server_set_replay": ([_P, _I, _P, ctypes.c_int64, ctypes.c_int64, _I, _P, ctypes.c_int64, _P], _I),
     "mc_server_start_replay": ([_P, _I], _I),
+    "mc_server_schedule": ([_P, _P, _P, _I, _P, ctypes.c_int64], _I),
     "mc_server_drain": ([_P], _I),
     "mc_server_take_records": ([_P, _P, _I, _P], _I),
     "mc_server_stats": ([_P, _P, _P, _P, _P], _I),

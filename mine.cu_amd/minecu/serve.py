@@ -107,6 +107,28 @@ class StepServer:
                                            None, ctypes.c_int64(0), None), "set_replay")
         self._check(L.mc_server_start_replay(self._h, gid), "start_replay")
 
+    def schedule(self, gids, actions, t_offsets_s) -> None:
+        """Open loop at planned times: request k steps group gids[k] with
+        actions[k] (a [B] int64 device tensor, kept alive by the caller until
+        drain) t_offsets_s[k] seconds from now (non-decreasing).  Each request
+        is queued by the server's loop thread at its arrival time, which is
+        also its recorded submit time."""
+        n = len(gids)
+        if len(actions) != n or len(t_offsets_s) != n:
+            raise ValueError("gids, actions and t_offsets_s must have the same length")
+        g = np.ascontiguousarray(np.asarray(gids, dtype=np.int32))
+        ptr = np.empty(n, dtype=np.uint64)
+        for k, (gid, a) in enumerate(zip(g.tolist(), actions)):
+            env = self._envs[gid]
+            if a.dtype != torch.int64 or a.device != env.device or a.numel() != env.batch_size \
+                    or not a.is_contiguous():
+                raise ValueError("scheduled actions must be contiguous int64 [B] on the env's device")
+            ptr[k] = a.data_ptr()
+        t = np.ascontiguousarray(np.round(np.asarray(t_offsets_s, dtype=np.float64) * 1e9).astype(np.int64))
+        self._check(_lib.load().mc_server_schedule(
+            self._h, g.ctypes.data_as(ctypes.c_void_p), ptr.ctypes.data_as(ctypes.c_void_p),
+            _lib.MC_ACT_DISCRETE, t.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(n)), "schedule")
+
     def drain(self) -> None:
         self._check(_lib.load().mc_server_drain(self._h), "drain")
 

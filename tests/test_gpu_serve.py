@@ -114,3 +114,54 @@ def test_admission_by_hbm_budget():
     srv.drain()
     assert len(srv.records()) == 2
     srv.close()
+
+
+def test_scheduled_open_loop_equals_alone():
+    """Planned Poisson arrivals released by the server's loop thread: every
+    group ends where stepping it alone ends, each request's recorded submit
+    time is its planned arrival, and malformed schedules are refused."""
+    from minecu import _lib
+    from minecu.serve import StepServer
+    T = 12
+    served = [_make(kw) for kw in SHAPES]
+    alone = [_make(kw) for kw in SHAPES]
+    acts = [_acts(T, kw["batch_size"], 50 + i) for i, kw in enumerate(SHAPES)]
+    torch.cuda.synchronize()
+    srv = StepServer(n_streams=2)
+    gids = [srv.add_group(e) for e in served]
+    rng = np.random.default_rng(7)
+    times, who = [], []
+    for i in range(len(SHAPES)):
+        t = np.cumsum(rng.exponential(1.0 / 2000.0, T))  # about 2000 requests/s per group
+        times.append(t)
+        who.append(np.full(T, i))
+    times, who = np.concatenate(times), np.concatenate(who)
+    order = np.argsort(times, kind="stable")
+    times, who = times[order], who[order]
+    step_of = np.zeros(len(who), dtype=np.int64)
+    seen = [0] * len(SHAPES)
+    for k, i in enumerate(who.tolist()):
+        step_of[k] = seen[i]
+        seen[i] += 1
+    a = [acts[i][s] for i, s in zip(who.tolist(), step_of.tolist())]
+    with pytest.raises(_lib.MineCuError):
+        srv.schedule([gids[0], gids[0]], [acts[0][0], acts[0][1]], [0.002, 0.001])  # decreasing
+    with pytest.raises(ValueError):
+        srv.schedule([gids[0]], [acts[1][0]], [0.0])  # another group's batch size
+    srv.schedule([gids[i] for i in who.tolist()], a, times)
+    srv.drain()
+    rec = srv.records()
+    assert len(rec) == len(who)
+    assert (rec["t_done_ns"] >= rec["t_launch_ns"]).all()
+    assert (rec["t_launch_ns"] >= rec["t_submit_ns"]).all()
+    # submit times are the planned arrivals: their gaps match the plan
+    sub = np.sort(rec["t_submit_ns"]) - rec["t_submit_ns"].min()
+    plan = np.round((times - times[0]) * 1e9).astype(np.int64)
+    assert np.abs(sub - plan).max() <= 1000
+    for i, e in enumerate(alone):
+        for t in range(T):
+            e.step(acts[i][t])
+    torch.cuda.synchronize()
+    for i in range(len(SHAPES)):
+        _same(_state(alone[i]), _state(served[i]), f"group {i}")
+    srv.close()
