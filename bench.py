@@ -264,6 +264,8 @@ def run_serving(seconds=60.0, rate=200.0, seed=4, budget_gb=64, closed_steps=200
         "served_env_steps_per_s": float(batch_of[rec80["group"]].sum() / span80),
         "p50_ms": float(np.percentile(lat80, 50)), "p99_ms": float(np.percentile(lat80, 99)),
         "max_ms": float(lat80.max()), "launches": n_launch80,
+        "p99_queue_ms": float(np.percentile((rec80["t_launch_ns"] - rec80["t_submit_ns"]) / 1e6, 99)),
+        "p99_launch_to_done_ms": float(np.percentile((rec80["t_done_ns"] - rec80["t_launch_ns"]) / 1e6, 99)),
         "requests_per_launch": (st1["requests"] - st0["requests"]) / max(1, n_launch80)}
     del a80
     # Closed loop (capacity): replay every client back to back.

@@ -441,30 +441,41 @@ int mc_server_schedule(mc_server* s, const int* gids, const void* const* actions
   if (action_kind != MC_ACT_DISCRETE && action_kind != MC_ACT_BUTTONS)
     return sfail(MC_E_INVALID, "bad action kind %d", action_kind);
   if (action_kind == MC_ACT_BUTTONS) return sfail(MC_E_INVALID, "scheduled requests carry no look");
+  for (int64_t k = 0; k < n; ++k) {
+    if (!actions[k]) return sfail(MC_E_INVALID, "request %lld: actions is NULL", (long long)k);
+    if (t_offset_ns[k] < 0 || (k > 0 && t_offset_ns[k] < t_offset_ns[k - 1]))
+      return sfail(MC_E_INVALID, "arrival offsets must be non-negative and non-decreasing");
+  }
+  // Built outside the lock with relative times; the clock starts once the
+  // list is ready, so building a long schedule delays no arrival.
+  std::vector<Req> add;
+  add.reserve((size_t)n);
+  for (int64_t k = 0; k < n; ++k)
+    add.push_back(Req{0, gids[k], actions[k], action_kind, nullptr, nullptr, t_offset_ns[k]});
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    for (int64_t k = 0; k < n; ++k) {
+    for (int64_t k = 0; k < n; ++k)
       if (gids[k] < 0 || gids[k] >= (int)s->groups.size() || s->groups[gids[k]].state != kAdmitted)
         return sfail(MC_E_INVALID, "request %lld: group %d is not admitted", (long long)k, gids[k]);
-      if (!actions[k]) return sfail(MC_E_INVALID, "request %lld: actions is NULL", (long long)k);
-      if (t_offset_ns[k] < 0 || (k > 0 && t_offset_ns[k] < t_offset_ns[k - 1]))
-        return sfail(MC_E_INVALID, "arrival offsets must be non-negative and non-decreasing");
-    }
-    // Merge with what is left of a running schedule, keeping time order.
-    std::vector<Req> left(s->sched.begin() + (int64_t)s->sched_next, s->sched.end());
-    const int64_t t0 = now_ns();
-    std::vector<Req> add;
-    add.reserve((size_t)n);
-    for (int64_t k = 0; k < n; ++k)
-      add.push_back(Req{s->next_id++, gids[k], actions[k], action_kind, nullptr, nullptr,
-                        t0 + t_offset_ns[k]});
-    s->sched.clear();
-    s->sched.reserve(left.size() + add.size());
     // Completion records are appended under the lock: reserve now, so no
     // reallocation of a long record list stalls the loop mid-schedule.
-    s->done.reserve(s->done.size() + left.size() + add.size());
-    std::merge(left.begin(), left.end(), add.begin(), add.end(), std::back_inserter(s->sched),
-               [](const Req& a, const Req& b) { return a.t_submit < b.t_submit; });
+    s->done.reserve(s->done.size() + (s->sched.size() - s->sched_next) + add.size());
+    const int64_t t0 = now_ns();
+    for (Req& r : add) {
+      r.id = s->next_id++;
+      r.t_submit += t0;
+    }
+    if (s->sched_next >= s->sched.size()) {
+      s->sched.swap(add);
+    } else {
+      // Merge with what is left of a running schedule, keeping time order.
+      std::vector<Req> merged;
+      merged.reserve(s->sched.size() - s->sched_next + add.size());
+      std::merge(s->sched.begin() + (int64_t)s->sched_next, s->sched.end(), add.begin(), add.end(),
+                 std::back_inserter(merged),
+                 [](const Req& a, const Req& b) { return a.t_submit < b.t_submit; });
+      s->sched.swap(merged);
+    }
     s->sched_next = 0;
   }
   s->cv.notify_one();
