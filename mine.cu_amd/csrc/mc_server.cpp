@@ -100,6 +100,9 @@ struct mc_server {
   std::vector<Req> sched;
   size_t sched_next = 0;
   std::deque<Launch> inflight;
+  // Requests taken from `pending` whose launch is being issued (outside the
+  // lock) and is not in `inflight` yet; drain waits for them too.
+  int launching = 0;
   std::vector<hipStream_t> streams;
   std::vector<char> stream_busy;
   std::vector<hipEvent_t> events;
@@ -215,6 +218,7 @@ struct mc_server {
             }
           }
         }
+        launching += (int)take.size();
         if (take.empty()) {
           // Sleep until a submit, or until shortly before the next planned
           // arrival (then poll, as while launches are in flight).
@@ -252,6 +256,7 @@ struct mc_server {
                              dns.data(), (int)take.size(), streams[stream]);
       if (st == MC_OK && hipEventRecord(ev, streams[stream]) != hipSuccess) st = MC_E_HIP;
       std::lock_guard<std::mutex> lk(mu);
+      launching -= (int)take.size();
       if (st != MC_OK) {
         last_err = mc_last_error();
         failed += (int64_t)take.size();
@@ -489,7 +494,7 @@ int mc_server_drain(mc_server* s) {
       std::lock_guard<std::mutex> lk(s->mu);
       bool replay_left = false;
       for (const Group& g : s->groups) replay_left |= (g.rep_base && g.rep_next < g.rep_count);
-      if (s->pending.empty() && s->inflight.empty() && !replay_left &&
+      if (s->pending.empty() && s->inflight.empty() && s->launching == 0 && !replay_left &&
           s->sched_next >= s->sched.size())
         break;
     }
