@@ -43,48 +43,45 @@ constexpr int8_t kAir = -1, kGrass = 0, kDirt = 1, kOakLog = 3, kLeaves = 4, kBr
                  kCobble = 9, kPlanks = 10, kBedrock = 12;
 
 // Shaded palette, 13 blocks x RGB then the sky (src/kernels.cu:27-44).  Copied
-// into LDS at kernel start so the per-pixel lookup is a ds_read.
-__device__ __forceinline__ float palette_entry(int i) {
-  switch (i) {
-    case 0: return 0.30f; case 1: return 0.65f; case 2: return 0.20f;    // grass
-    case 3: return 0.55f; case 4: return 0.35f; case 5: return 0.20f;    // dirt
-    case 6: return 0.50f; case 7: return 0.50f; case 8: return 0.50f;    // stone
-    case 9: return 0.55f; case 10: return 0.40f; case 11: return 0.25f;  // oak log
-    case 12: return 0.20f; case 13: return 0.55f; case 14: return 0.15f; // leaves
-    case 15: return 0.90f; case 16: return 0.85f; case 17: return 0.60f; // sand
-    case 18: return 0.20f; case 19: return 0.40f; case 20: return 0.80f; // water
-    case 21: return 0.85f; case 22: return 0.90f; case 23: return 0.95f; // glass
-    case 24: return 0.70f; case 25: return 0.35f; case 26: return 0.30f; // brick
-    case 27: return 0.40f; case 28: return 0.40f; case 29: return 0.40f; // cobblestone
-    case 30: return 0.75f; case 31: return 0.60f; case 32: return 0.40f; // planks
-    case 33: return 0.95f; case 34: return 0.95f; case 35: return 0.98f; // snow
-    case 36: return 0.15f; case 37: return 0.15f; case 38: return 0.15f; // bedrock
-    case 39: return 0.53f; case 40: return 0.81f; default: return 0.92f; // sky
-  }
-}
+// into LDS at kernel start (one constant-memory load per thread) so the
+// per-pixel lookup is a ds_read.
 constexpr int kPaletteLen = 42;
 constexpr int kSkyBase = 39;
+__device__ __constant__ float kPaletteTab[kPaletteLen] = {
+    0.30f, 0.65f, 0.20f,  // grass
+    0.55f, 0.35f, 0.20f,  // dirt
+    0.50f, 0.50f, 0.50f,  // stone
+    0.55f, 0.40f, 0.25f,  // oak log
+    0.20f, 0.55f, 0.15f,  // leaves
+    0.90f, 0.85f, 0.60f,  // sand
+    0.20f, 0.40f, 0.80f,  // water
+    0.85f, 0.90f, 0.95f,  // glass
+    0.70f, 0.35f, 0.30f,  // brick
+    0.40f, 0.40f, 0.40f,  // cobblestone
+    0.75f, 0.60f, 0.40f,  // planks
+    0.95f, 0.95f, 0.98f,  // snow
+    0.15f, 0.15f, 0.15f,  // bedrock
+    0.53f, 0.81f, 0.92f,  // sky
+};
 
 // Flat uint8 colours, 13 blocks then the sky (src/kernels.cu:479-495), packed
 // 0x00BBGGRR.
-__device__ __forceinline__ uint32_t flat_entry(int i) {
-  switch (i) {
-    case 0: return 76u | (166u << 8) | (51u << 16);     // grass
-    case 1: return 140u | (89u << 8) | (51u << 16);     // dirt
-    case 2: return 127u | (127u << 8) | (127u << 16);   // stone
-    case 3: return 140u | (102u << 8) | (64u << 16);    // oak log
-    case 4: return 51u | (140u << 8) | (38u << 16);     // leaves
-    case 5: return 229u | (217u << 8) | (153u << 16);   // sand
-    case 6: return 51u | (102u << 8) | (204u << 16);    // water
-    case 7: return 217u | (229u << 8) | (242u << 16);   // glass
-    case 8: return 178u | (89u << 8) | (76u << 16);     // brick
-    case 9: return 102u | (102u << 8) | (102u << 16);   // cobblestone
-    case 10: return 191u | (153u << 8) | (102u << 16);  // planks
-    case 11: return 242u | (242u << 8) | (250u << 16);  // snow
-    case 12: return 38u | (38u << 8) | (38u << 16);     // bedrock
-    default: return 135u | (206u << 8) | (235u << 16);  // sky
-  }
-}
+__device__ __constant__ uint32_t kFlatTab[14] = {
+    76u | (166u << 8) | (51u << 16),     // grass
+    140u | (89u << 8) | (51u << 16),     // dirt
+    127u | (127u << 8) | (127u << 16),   // stone
+    140u | (102u << 8) | (64u << 16),    // oak log
+    51u | (140u << 8) | (38u << 16),     // leaves
+    229u | (217u << 8) | (153u << 16),   // sand
+    51u | (102u << 8) | (204u << 16),    // water
+    217u | (229u << 8) | (242u << 16),   // glass
+    178u | (89u << 8) | (76u << 16),     // brick
+    102u | (102u << 8) | (102u << 16),   // cobblestone
+    191u | (153u << 8) | (102u << 16),   // planks
+    242u | (242u << 8) | (250u << 16),   // snow
+    38u | (38u << 8) | (38u << 16),      // bedrock
+    135u | (206u << 8) | (235u << 16),   // sky
+};
 constexpr int kFlatLen = 14;
 constexpr int kFlatSky = 13;
 
@@ -1288,8 +1285,8 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
 }
 
 __device__ __forceinline__ void load_palette(float* pal, uint32_t* flat, int tid) {
-  if (tid < kPaletteLen) pal[tid] = palette_entry(tid);
-  if (tid < kFlatLen) flat[tid] = flat_entry(tid);
+  if (tid < kPaletteLen) pal[tid] = kPaletteTab[tid];
+  if (tid < kFlatLen) flat[tid] = kFlatTab[tid];
 }
 
 // Wave-0 logic of MineEnv.reset for env b (minecu/__init__.py:270-302):
