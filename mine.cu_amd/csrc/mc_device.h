@@ -1080,13 +1080,13 @@ __device__ __forceinline__ EnvIn load_env_in(const Params& p, int b) {
   in.look_pitch = 0.0f;
   if (p.action_kind == 1) {
     // Discrete ids (docs/environment.md:3-17): 1..6 press one button.
-    const int64_t a = reinterpret_cast<const int64_t*>(p.actions)[b];
-    if (a >= 1 && a <= 4) in.bt_lo = 1u << (8 * (int)(a - 1));
-    else if (a >= 5 && a <= 6) in.bt_hi = 1u << (8 * (int)(a - 5));
-    else if (a == 7) in.look_yaw = p.look_delta;
-    else if (a == 8) in.look_yaw = -p.look_delta;
-    else if (a == 9) in.look_pitch = p.look_delta;
-    else if (a == 10) in.look_pitch = -p.look_delta;
+    // Selects rather than a branch chain; ids outside 1..10 do nothing.
+    const int64_t a64 = reinterpret_cast<const int64_t*>(p.actions)[b];
+    const int a = (a64 >= 1 && a64 <= 10) ? (int)a64 : 0;
+    in.bt_lo = (a >= 1 && a <= 4) ? (1u << ((8 * (a - 1)) & 31)) : 0u;
+    in.bt_hi = (a == 5 || a == 6) ? (1u << ((8 * (a - 5)) & 31)) : 0u;
+    in.look_yaw = (a == 7) ? p.look_delta : (a == 8) ? -p.look_delta : 0.0f;
+    in.look_pitch = (a == 9) ? p.look_delta : (a == 10) ? -p.look_delta : 0.0f;
   } else {
     const uint8_t* src = reinterpret_cast<const uint8_t*>(p.actions) + (int64_t)b * 8;
 #pragma unroll
@@ -1460,7 +1460,20 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
       solid_extent(p, world, &frames[e], (int)floorf(frames[e].ey), et, ent);
     __syncthreads();
   }
-  if (go) {
+  // An eye inside a solid block sees that block in every pixel: a constant
+  // uint8 image, written without the camera frame or the octant words.
+  const int npix = p.W * p.H;
+  const bool flat_img = (uvm != kUvShaded);
+  int fill = -1;
+  if (go && ((npix * 3) & 15) == 0 && (flat_img || obs_mode != kObsF32))
+    fill = __builtin_amdgcn_readfirstlane(flat_img ? frames[e].flat_fill : frames[e].fill_block);
+  if (fill >= 0) {
+    const uint32_t rgb = flat_img ? flat[fill < kFlatSky ? fill : kFlatSky]
+                                  : to_u8(pal[fill * 3]) | (to_u8(pal[fill * 3 + 1]) << 8) |
+                                        (to_u8(pal[fill * 3 + 2]) << 16);
+    fill_rgb8(reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3, npix, rgb, et, ent);
+    if (KIND == kItemStep && any_done) store_fresh_world(p, gworld, nullptr, nvox, et, ent);
+  } else if (go) {
     // Every wave of the env writes the same 8 octant words before it reads
     // them; a wave's LDS operations complete in order.
     if ((et & (kWave - 1)) < 8)
