@@ -92,6 +92,29 @@ def test_c2_shape_with_episode_regen():
     print(st.summary())
 
 
+def test_pockets_are_exact():
+    """16^3, 32x32 uint8: agents move and look around while falling, land
+    with the eye inside dirt, break the block at the eye (for most of them a
+    one-voxel air pocket whose six neighbours are solid, so every ray ends on
+    its first step), look around from inside it, then dig on (open pockets
+    and tunnels).  Every image, and all state, equals the oracle bit for
+    bit."""
+    B = 128
+    env, orc = _envs(batch_size=B, world_size=16, render_width=32, render_height=32,
+                     obs_dtype="uint8", episode_length=None)
+    g = torch.Generator().manual_seed(11)
+    moves = torch.tensor([1, 2, 3, 4, 7, 8, 9, 10])[torch.randint(0, 8, (16, B), generator=g)]
+    looks = torch.tensor([7, 8, 9, 10])[torch.randint(0, 4, (16, B), generator=g)]
+    dig = torch.where(torch.rand((12, B), generator=g) < 0.5, torch.full((12, B), 6),
+                      torch.tensor([7, 8, 9, 10])[torch.randint(0, 4, (12, B), generator=g)])
+    acts = torch.cat([moves, torch.full((1, B), 6), looks, dig]).numpy()
+    st = Stats()
+    env.reset()
+    orc.reset()
+    run_pair(env, orc, acts, st, "sealed")
+    assert st.exact, st.summary()
+
+
 def test_c3_shape_break_heavy():
     """32^3, 64x64 uint8, 50% break actions, episode_length 60."""
     env, orc = _envs(batch_size=64, world_size=32, render_width=64, render_height=64,
