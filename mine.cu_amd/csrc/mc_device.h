@@ -549,8 +549,12 @@ template <bool RSQRT>
 __device__ __forceinline__ void ray_dir(float& x, float& y, float& z, float& ix, float& iy,
                                         float& iz, float big) {
   const float len2 = x * x + y * y + z * z;
-  auto ok = [](float c) { return c == 0.0f || fabsf(c) >= 0x1p-60f; };
-  if (len2 >= 0x1p-60f && len2 <= 0x1p60f && ok(x) && ok(y) && ok(z)) {
+  // |c| is 0 or at least 2^-60 (0x21800000): its magnitude bits less one, as
+  // an unsigned, are then at least 0x217fffff.  Bitwise ANDs, not a chain of
+  // branches; a NaN or infinite component fails the len^2 range instead.
+  auto ok = [](float c) { return (__float_as_uint(c) & 0x7fffffffu) - 1u >= 0x217fffffu; };
+  const bool fast = (len2 >= 0x1p-60f) & (len2 <= 0x1p60f) & ok(x) & ok(y) & ok(z);
+  if (fast) {
     const float len = sqrt_core(len2);
     const float rl = recip_core(len);
     if (RSQRT) {
