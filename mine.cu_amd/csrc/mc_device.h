@@ -359,16 +359,16 @@ __device__ __forceinline__ void finish_frame(const Params& p, Frame& f, const in
   f.hx0 = f.hz0 = INT_MAX;
   f.hx1 = f.hz1 = f.hy1 = INT_MIN;
   f.lift = 0;
-  if (p.max_steps >= 1 && p.view_distance > 0.0f && f.ex > 0.0f && f.ex < (float)p.X &&
-      f.ey > 0.0f && f.ey < (float)p.Y && f.ez > 0.0f && f.ez < (float)p.Z) {
+  if ((p.max_steps >= 1) & (p.view_distance > 0.0f) & (f.ex > 0.0f) & (f.ex < (float)p.X) &
+      (f.ey > 0.0f) & (f.ey < (float)p.Y) & (f.ez > 0.0f) & (f.ez < (float)p.Z)) {
     const int vx = (int)floorf(f.ex), vy = (int)floorf(f.ey), vz = (int)floorf(f.ez);
     const int8_t blk = world[vy * p.X * p.Z + vx * p.Z + vz];
     if (blk >= 0 && blk < 13) f.fill_block = blk;
   }
   // floor(e) lies in [0, n) exactly when 0 <= e < n (false for NaN).
-  f.inside = f.ex >= 0.0f && f.ex < (float)p.X && f.ey >= 0.0f && f.ey < (float)p.Y &&
-             f.ez >= 0.0f && f.ez < (float)p.Z && p.X <= kMaxInsideDim &&
-             p.Y <= kMaxInsideDim && p.Z <= kMaxInsideDim;
+  f.inside = (f.ex >= 0.0f) & (f.ex < (float)p.X) & (f.ey >= 0.0f) & (f.ey < (float)p.Y) &
+             (f.ez >= 0.0f) & (f.ez < (float)p.Z) & (p.X <= kMaxInsideDim) &
+             (p.Y <= kMaxInsideDim) & (p.Z <= kMaxInsideDim);
   f.ivx = f.inside ? (int)floorf(f.ex) : 0;
   f.ivy = f.inside ? (int)floorf(f.ey) : 0;
   f.ivz = f.inside ? (int)floorf(f.ez) : 0;
@@ -716,7 +716,12 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
     asm volatile("" : "+v"(blk));  // a fresh test, not the loop's lane mask
     if (!lim_end & ((pos & kGuard) == 0u) & (blk >= 0)) {
       hit_block = blk;
-      hit_face = (pinc == 0u) ? -1 : (pinc == 1u ? 0 : (pinc == (1u << 10) ? 1 : 2));
+      // Face from the last step's axis (x 0, y 1, z 2; none -1), by
+      // independent selects rather than a nested chain.
+      int face = (pinc == 1u) ? 0 : -1;
+      face = (pinc == (1u << 10)) ? 1 : face;
+      face = (pinc == (1u << 20)) ? 2 : face;
+      hit_face = face;
     }
   } else {
     const float t1 = (0.0f - f.ex) * inv_x, t2 = ((float)X - f.ex) * inv_x;
@@ -764,11 +769,11 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
       }
     }
   }
-  if (hit_block >= 0 && hit_block < 13) {
-    const float shade = (hit_face == 0)                   ? 0.8f
-                        : (hit_face == 2)                 ? 0.9f
-                        : (hit_face == 1 && step_y < 0)   ? 0.6f
-                                                          : 1.0f;
+  if ((unsigned)hit_block < 13u) {
+    // Face shade by independent selects (no short-circuit branches).
+    float shade = (hit_face == 0) ? 0.8f : 1.0f;
+    shade = (hit_face == 2) ? 0.9f : shade;
+    shade = ((hit_face == 1) & (step_y < 0)) ? 0.6f : shade;
     // t / vd, or t * (1 / vd) when vd is a power of two (the same value).
     float fog = fminf(p.vd_pow2 ? t * p.inv_vd : t / vd, 1.0f);
     fog = fog * fog;
@@ -962,7 +967,7 @@ __device__ __forceinline__ void for_tiles(const Params& p, int tid, int nt, Pixe
   int tr = wv / tcols, tc = wv - tr * tcols;
   for (int t = wv; t < ntiles; t += nwv) {
     const int py = tr * 8 + ty, px = tc * 8 + tx;
-    if (py < p.H && px < p.W) fn(px, py);
+    if ((py < p.H) & (px < p.W)) fn(px, py);
     tr += dr;
     tc += dc;
     if (tc >= tcols) {
@@ -1000,7 +1005,7 @@ __device__ __forceinline__ void render_env_flat(const Params& p, int b, const in
   auto color = [&](float u, float v) -> uint32_t {
     if (per_env_scale) { u = u * f.su; v = v * f.sv; }
     const int h = (f.flat_fill >= 0) ? f.flat_fill : trace_flat<LIFT, T>(p, f, op, u, v, world);
-    return flat[(h >= 0 && h < kFlatSky) ? h : kFlatSky];
+    return flat[((unsigned)h < (unsigned)kFlatSky) ? h : kFlatSky];
   };
   if (f.flat_fill >= 0 && ((npix * 3) & 15) == 0) {
     fill_rgb8(out, npix, flat[f.flat_fill < kFlatSky ? f.flat_fill : kFlatSky], tid, nt);
@@ -1197,7 +1202,8 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
       const int vx = (int)floorf(ex + dx * t);
       const int vy = (int)floorf(ey + dy * t);
       const int vz = (int)floorf(ez + dz * t);
-      inb = vx >= 0 && vx < p.X && vy >= 0 && vy < p.Y && vz >= 0 && vz < p.Z;
+      inb = ((unsigned)vx < (unsigned)p.X) & ((unsigned)vy < (unsigned)p.Y) &
+            ((unsigned)vz < (unsigned)p.Z);
       if (inb) {
         idx = vy * p.X * p.Z + vx * p.Z + vz;
         blk = world[idx];
