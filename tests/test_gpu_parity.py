@@ -155,6 +155,21 @@ def test_buttons_and_look_continuous():
     print(st.summary())
 
 
+@pytest.mark.parametrize("max_steps", [8, 45, 46])
+def test_step_limit_boundaries(max_steps):
+    """16^3 (X + Y + Z - 3 = 45): with max_steps 46 no in-world ray can reach
+    the step limit, at 45 the longest in-world path just fits, and at 8 the
+    limit cuts the falling agents' long rays.  Bit-exact in every case."""
+    env, orc = _envs(batch_size=32, world_size=16, render_width=32, render_height=32,
+                     obs_dtype="uint8", episode_length=None, max_steps=max_steps)
+    st = Stats()
+    env.reset()
+    orc.reset()
+    compare_env(env, orc, st, f"reset max_steps={max_steps}")
+    run_pair(env, orc, _discrete(20, 32, 13), st, f"max_steps={max_steps}")
+    assert st.exact, st.summary()
+
+
 @pytest.mark.parametrize("dims,img", [((24, 16, 20), (30, 20)), ((9, 9, 9), (17, 11)),
                                       ((16, 16, 16), (20, 12))])
 def test_odd_shapes(dims, img):
