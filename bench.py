@@ -151,6 +151,79 @@ def run_gpu(w, obs, steps, warmup):
     return total_ms, wall, per_launch_ms
 
 
+def batched_groups(seed=3, n_groups=8):
+    """8 groups: batch log-uniform in [64, 16384], world 8^3, 16^3 or 32^3,
+    obs 16^2, 32^2 or 64^2 uint8."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_groups):
+        B = int(round(float(np.exp(rng.uniform(np.log(64), np.log(16384))))))
+        out.append(dict(batch=B, world=int(rng.choice([8, 16, 32])), res=int(rng.choice([16, 32, 64]))))
+    return out
+
+
+def run_batched(steps=200, warmup=20, seed=3):
+    """The uneven groups stepped together in one persistent launch per step
+    (autotuned shapes), against the same groups stepped one launch each."""
+    import torch
+    import minecu
+    from minecu.batch import EnvBatch
+    dev = torch.device("cuda", 0)
+    groups = batched_groups(seed)
+    g = torch.Generator(device=dev).manual_seed(seed)
+
+    def make():
+        envs = [minecu.MineEnv(batch_size=c["batch"], world_size=c["world"], render_width=c["res"],
+                               render_height=c["res"], device=str(dev), obs_dtype="uint8",
+                               episode_length=60) for c in groups]
+        for e in envs:
+            e.reset()
+        return envs
+
+    acts = [torch.randint(0, 11, (warmup + steps, c["batch"]), generator=g, device=dev)
+            for c in groups]
+    total = sum(c["batch"] for c in groups)
+    stream = torch.cuda.current_stream(dev)
+    # One launch per step for all groups.
+    envs = make()
+    batch = EnvBatch(persistent=True)
+    batch.autotune(envs)
+    for t in range(warmup):
+        batch.step(envs, [a[t] for a in acts])
+    s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s0.record(stream)
+    for t in range(warmup, warmup + steps):
+        batch.step(envs, [a[t] for a in acts])
+    s1.record(stream)
+    torch.cuda.synchronize()
+    batched_ms = s0.elapsed_time(s1)
+    batch.close()
+    for e in envs:
+        e.close()
+    # The same groups, one launch per group per step, default shapes.
+    envs = make()
+    for t in range(warmup):
+        for e, a in zip(envs, acts):
+            e.step(a[t])
+    torch.cuda.synchronize()
+    s0.record(stream)
+    for t in range(warmup, warmup + steps):
+        for e, a in zip(envs, acts):
+            e.step(a[t])
+    s1.record(stream)
+    torch.cuda.synchronize()
+    alone_ms = s0.elapsed_time(s1)
+    for e in envs:
+        e.close()
+    return {"groups": groups, "total_batch": total, "steps": steps,
+            "env_steps_per_s": total * steps / (batched_ms / 1e3),
+            "one_launch_per_group_env_steps_per_s": total * steps / (alone_ms / 1e3),
+            "workload": "8 groups, batch log-uniform [64,16384], world 8^3/16^3/32^3, obs "
+                        "16^2/32^2/64^2 uint8, uniform discrete actions, episode_length 60, seed 3"}
+
+
 def serving_clients(seed=4, n_clients=64):
     """64 clients: batch log-uniform in [16, 4096], world 16^3 or 32^3, obs
     32x32 or 64x64 uint8."""
@@ -300,6 +373,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32", action="store_true", help="skip the float32-obs side measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-groups workload")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-streams", type=int, default=2)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -358,6 +432,8 @@ def main():
     if args.obs == "uint8" and not args.no_f32:
         f_ms, _, f_launch = run_gpu(w, "float32", min(K, 200), args.warmup)
         out["float32_obs_value"] = B * min(K, 200) / (f_ms / 1e3)
+    if not args.no_batch:
+        out["batched"] = run_batched()
     if not args.no_serve:
         out["served"] = run_serving(args.serve_seconds, n_streams=args.serve_streams)
         closed = out["served"]["closed_loop"]["env_steps_per_s"]
