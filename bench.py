@@ -121,13 +121,13 @@ def cpu_baseline(w, obs, seconds_budget=15.0):
                       f"{done_steps} steps from reset of the same workload ({dt:.1f} s)"}
 
 
-def run_gpu(w, obs, steps, warmup):
+def run_gpu(w, obs, steps, warmup, layout="hwc"):
     import torch
     import minecu
     dev = torch.device("cuda", 0)
     env = minecu.MineEnv(batch_size=w["batch"], world_size=w["world"], render_width=w["W"],
                          render_height=w["H"], device=str(dev), obs_dtype=obs,
-                         episode_length=w["episode_length"])
+                         episode_length=w["episode_length"], obs_layout=layout)
     acts = make_actions(w, warmup + steps, dev)
     env.reset()
     for t in range(warmup):
@@ -375,6 +375,8 @@ def main():
     ap.add_argument("--obs", default="uint8", choices=["uint8", "float32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32", action="store_true", help="skip the float32-obs side measurement")
+    ap.add_argument("--no-chw", action="store_true",
+                    help="skip the planar-layout (obs_layout='chw') side measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-groups workload")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
@@ -441,6 +443,10 @@ def main():
         out["float32_obs_value"] = B * min(K, 200) / (f_ms / 1e3)
         if args.workload == "c2":
             out["float32_obs_vs_baseline"] = out["float32_obs_value"] / REFERENCE_C2
+    if args.obs == "uint8" and not args.no_chw:
+        # The same workload with planar [B, 3, H, W] uint8 observations.
+        c_ms, _, _ = run_gpu(w, "uint8", min(K, 200), args.warmup, layout="chw")
+        out["uint8_chw_obs_value"] = B * min(K, 200) / (c_ms / 1e3)
     if not args.no_batch:
         out["batched"] = run_batched()
     if not args.no_serve:

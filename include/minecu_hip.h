@@ -19,7 +19,7 @@
  *   step_count  int32  [B]
  *   do_reset    bool   [B]
  *   cameras     f32    [B, 5]         x, y, z, yaw, pitch
- *   obs         uint8 or f32 [B, H, W, 3]
+ *   obs         uint8 or f32 [B, H, W, 3]  ([B, 3, H, W] in the MC_OBS_*_CHW modes)
  *   rewards     f32    [B]
  *   buttons     int8   [B, 8]   look f32 [B, 2]   (multi-hot + radians)
  *   actions     int64  [B]      discrete ids 0..10 (docs/environment.md:3-17)
@@ -52,8 +52,13 @@ enum {
                              camera basis (render_kernel_uint8_prebasis, kernels.cu:568-700; the
                              smem kernel, kernels.cu:835-982, renders the same image); steps
                              write st->basis when it is non-NULL */
-  MC_OBS_U8_FP16 = 4      /* "uint8_fp16": the minimal image plane with a binary16 DDA
+  MC_OBS_U8_FP16 = 4,     /* "uint8_fp16": the minimal image plane with a binary16 DDA
                              (render_kernel_uint8_fp16, kernels.cu:1127-1262) */
+  /* No counterpart in the reference: the MC_OBS_U8 / MC_OBS_F32 images, same values, in planar
+   * [B, 3, H, W] layout -- the layout a convolutional policy reads, which the reference's
+   * training example produces with a permute of every observation (examples/train_wood.py:53). */
+  MC_OBS_U8_CHW = 5,
+  MC_OBS_F32_CHW = 6
 };
 
 /* Action encodings accepted by the step. */

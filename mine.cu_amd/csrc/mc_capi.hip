@@ -87,7 +87,7 @@ int validate_config(const mc_config* c) {
   if (r) return r;
   r = check_image(c->batch_size, c->width, c->height);
   if (r) return r;
-  if (c->obs_mode < MC_OBS_U8 || c->obs_mode > MC_OBS_U8_FP16)
+  if (c->obs_mode < MC_OBS_U8 || c->obs_mode > MC_OBS_F32_CHW)
     return fail(MC_E_INVALID, "obs_mode must be one of MC_OBS_* (got %d)", c->obs_mode);
   return MC_OK;
 }
@@ -207,18 +207,32 @@ Shape fit_lds(Shape s, int X, int Y, int Z) {
   return s;
 }
 
+bool planar_mode(int obs_mode) {
+  return obs_mode == MC_OBS_U8_CHW || obs_mode == MC_OBS_F32_CHW;
+}
+
 template <int NT, int KIND, bool LIFT>
 int launch_kind(const mc::Params& p, int obs_mode, int wpe, hipStream_t st) {
   const int epw = (NT / mc::kWave) / wpe;
   const int grid = (p.B + epw - 1) / epw;
   const size_t lds = (size_t)mc::item_lds_bytes(epw, p.X, p.ys, p.Z);
-  hipLaunchKernelGGL((mc::k_env<NT, KIND, LIFT>), dim3(grid), dim3(NT), lds, st, p, obs_mode,
-                     wpe);
+  // The planar modes run kernels of their own: the per-pixel store form is a
+  // template constant, so the interleaved kernels keep their code unchanged.
+  if (planar_mode(obs_mode))
+    hipLaunchKernelGGL((mc::k_env<NT, KIND, LIFT, true>), dim3(grid), dim3(NT), lds, st, p,
+                       obs_mode, wpe);
+  else
+    hipLaunchKernelGGL((mc::k_env<NT, KIND, LIFT>), dim3(grid), dim3(NT), lds, st, p, obs_mode,
+                       wpe);
   return check_launch("env kernel launch");
 }
 
 template <int NT, bool LIFT>
 int launch_nt(int kind, const mc::Params& p, int obs_mode, int wpe, hipStream_t st) {
+  // Planar images come from MineEnv's own step, reset and render only.
+  if (planar_mode(obs_mode) && kind != mc::kItemStep && kind != mc::kItemResetRender &&
+      kind != mc::kItemRenderState)
+    return fail(MC_E_UNSUPPORTED, "the MC_OBS_*_CHW modes render from MineEnv state only");
   switch (kind) {
     case mc::kItemStep: return launch_kind<NT, mc::kItemStep, LIFT>(p, obs_mode, wpe, st);
     case mc::kItemResetRender:
@@ -260,7 +274,15 @@ int launch_deferred(const mc::Params& q, int obs_mode, int lift, hipStream_t st)
   const int full = mc::item_lds_bytes(1, p.X, p.Y, p.Z);
   if (full > lds_budget()) return fail(MC_E_UNSUPPORTED, "world does not fit the LDS budget");
   const dim3 grid(std::min(p.B, 64));
-  if (full > 80 * 1024) {
+  if (planar_mode(obs_mode)) {
+    if (full > 80 * 1024) {
+      if (lift) hipLaunchKernelGGL((mc::k_env_fb<512, true, true>), grid, dim3(512), full, st, p, obs_mode);
+      else hipLaunchKernelGGL((mc::k_env_fb<512, false, true>), grid, dim3(512), full, st, p, obs_mode);
+    } else {
+      if (lift) hipLaunchKernelGGL((mc::k_env_fb<256, true, true>), grid, dim3(256), full, st, p, obs_mode);
+      else hipLaunchKernelGGL((mc::k_env_fb<256, false, true>), grid, dim3(256), full, st, p, obs_mode);
+    }
+  } else if (full > 80 * 1024) {
     if (lift) hipLaunchKernelGGL((mc::k_env_fb<512, true>), grid, dim3(512), full, st, p, obs_mode);
     else hipLaunchKernelGGL((mc::k_env_fb<512, false>), grid, dim3(512), full, st, p, obs_mode);
   } else {
@@ -1063,6 +1085,9 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
       if (envs[j] == envs[i]) return fail(MC_E_INVALID, "group %d listed twice", i);
     if (envs[i]->graph_mode)
       return fail(MC_E_UNSUPPORTED, "group %d is in graph mode (its slot is device-resident)", i);
+    if (planar_mode(envs[i]->cfg.obs_mode))
+      return fail(MC_E_UNSUPPORTED, "group %d: the MC_OBS_*_CHW modes step alone, not in a batch",
+                  i);
     if (envs[i]->shape.wpe == 8) nt = 512;
     lift = lift || envs[i]->shape.lift;
   }
@@ -1198,6 +1223,9 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
     if (!envs[i]) return fail(MC_E_INVALID, "env %d is NULL", i);
     for (int j = 0; j < i; ++j)
       if (envs[j] == envs[i]) return fail(MC_E_INVALID, "group %d listed twice", i);
+    if (planar_mode(envs[i]->cfg.obs_mode))
+      return fail(MC_E_UNSUPPORTED, "group %d: the MC_OBS_*_CHW modes step alone, not in a batch",
+                  i);
     const mc_config& c = envs[i]->cfg;
     const BucketKey k = bucket_of(c);
     const int j = (int)(std::find(keys.begin(), keys.end(), k) - keys.begin());

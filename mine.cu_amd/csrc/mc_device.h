@@ -936,6 +936,18 @@ __device__ __forceinline__ void fill_rgb8(uint8_t* out, int npix, uint32_t rgb, 
   }
 }
 
+// Constant-colour fill of env b's planar uint8 image [3, H, W] (packed
+// 0x00BBGGRR): each plane is one byte repeated.  Needs H * W % 16 == 0.
+__device__ __forceinline__ void fill_planes8(uint8_t* out, int npix, uint32_t rgb, int tid, int nt) {
+  guint4* o4 = (guint4*)out;
+  const int n16 = npix >> 4;
+  for (int c = tid; c < 3 * n16; c += nt) {
+    const int ch = (c >= n16) + (c >= 2 * n16);
+    const uint32_t w = ((rgb >> (8 * ch)) & 0xffu) * 0x01010101u;
+    o4[c] = v4u{w, w, w, w};
+  }
+}
+
 // Pixels are traced in 8x8 tiles, one pixel per lane of a wave per tile, and
 // a wave's loop over a tile ends when its longest ray ends: neighbouring
 // pixels have rays of similar length, so square tiles waste fewer lane-steps
@@ -981,12 +993,22 @@ __device__ __forceinline__ void for_tiles(const Params& p, int tid, int nt, Pixe
 // 0x00BBGGRR.
 template <typename ColorFn>
 __device__ __forceinline__ void render_rgb8(const Params& p, int b, int uvm, int tid, int nt,
-                                            const float* su, const float* sv, ColorFn color) {
+                                            const float* su, const float* sv, ColorFn color,
+                                            bool planar = false) {
   uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * p.W * p.H * 3;
   const bool tab = uv_tabled(p);
   for_tiles(p, tid, nt, [&](int px, int py) {
     const uint32_t c = color(tab ? su[px] : u_of(p, uvm, px), tab ? sv[py] : v_of(p, uvm, py));
-    store_rgb8(out, py * p.W + px, c);
+    if (planar) {
+      typedef MC_GLOBAL uint8_t gu8;
+      const int npix = p.W * p.H;
+      gu8* o = (gu8*)(out + py * p.W + px);
+      o[0] = (uint8_t)c;
+      o[npix] = (uint8_t)(c >> 8);
+      o[2 * npix] = (uint8_t)(c >> 16);
+    } else {
+      store_rgb8(out, py * p.W + px, c);
+    }
   });
 }
 
@@ -1014,17 +1036,20 @@ __device__ __forceinline__ void render_env_flat(const Params& p, int b, const in
   render_rgb8(p, b, uvm, tid, nt, su, sv, color);
 }
 
+// planar: the image is written [3, H, W] (the MC_OBS_*_CHW modes).  Callers
+// pass a compile-time constant, so each kernel holds one store form only.
 template <int OBS, bool LIFT, bool VDF>
 __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t* world,
                                            const float* pal, const Frame& f, const OctPos& op,
-                                           int tid, int nt, const float* su, const float* sv) {
+                                           int tid, int nt, const float* su, const float* sv,
+                                           bool planar) {
   const int W = p.W, npix = W * p.H;
   const bool tab = uv_tabled(p);
   auto ucol = [&](int px) { return tab ? su[px] : u_of(p, kUvShaded, px); };
   auto vrow = [&](int py) { return tab ? sv[py] : v_of(p, kUvShaded, py); };
   if (OBS == 0) {
     uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
-    if (f.fill_block >= 0 && ((npix * 3) & 15) == 0) {
+    if (f.fill_block >= 0 && ((npix * 3) & 15) == 0 && !planar) {
       fill_rgb8(out, npix,
                 to_u8(pal[f.fill_block * 3]) | (to_u8(pal[f.fill_block * 3 + 1]) << 8) |
                     (to_u8(pal[f.fill_block * 3 + 2]) << 16),
@@ -1041,7 +1066,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       }
       return to_u8(rgb[0]) | (to_u8(rgb[1]) << 8) | (to_u8(rgb[2]) << 16);
     };
-    render_rgb8(p, b, kUvShaded, tid, nt, su, sv, color);
+    render_rgb8(p, b, kUvShaded, tid, nt, su, sv, color, planar);
   } else {
     float* out = reinterpret_cast<float*>(p.obs) + (int64_t)b * npix * 3;
     typedef MC_GLOBAL float gf32;
@@ -1052,10 +1077,17 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       } else {
         trace<LIFT, VDF>(p, f, op, ucol(px), vrow(py), world, pal, rgb);
       }
-      gf32* o = (gf32*)(out + ((int64_t)py * W + px) * 3);
-      o[0] = rgb[0];
-      o[1] = rgb[1];
-      o[2] = rgb[2];
+      if (planar) {
+        gf32* o = (gf32*)(out + (int64_t)py * W + px);
+        o[0] = rgb[0];
+        o[npix] = rgb[1];
+        o[2 * npix] = rgb[2];
+      } else {
+        gf32* o = (gf32*)(out + ((int64_t)py * W + px) * 3);
+        o[0] = rgb[0];
+        o[1] = rgb[1];
+        o[2] = rgb[2];
+      }
     });
   }
 }
@@ -1386,7 +1418,10 @@ enum ItemKind {
 // variants with the minimal kernel's and the prebasis kernel's image plane.
 // Steps render the variant; the reset and state renders of the flat modes
 // are shaded uint8, as the reference's _render() is (minecu/__init__.py:446-466).
-enum ObsMode { kObsU8 = 0, kObsF32 = 1, kObsU8Minimal = 2, kObsU8Prebasis = 3, kObsU8Fp16 = 4 };
+enum ObsMode {
+  kObsU8 = 0, kObsF32 = 1, kObsU8Minimal = 2, kObsU8Prebasis = 3, kObsU8Fp16 = 4,
+  kObsU8Chw = 5, kObsF32Chw = 6  // the shaded images in planar [3, H, W] layout
+};
 
 __host__ __device__ __forceinline__ int world_stride_bytes(int X, int Y, int Z) {
   return (X * Y * Z + 15) & ~15;
@@ -1402,7 +1437,7 @@ __host__ __device__ __forceinline__ int item_lds_bytes(int epw, int X, int Y, in
 }
 
 // upper: one word per wave of the workgroup (staged-height check).
-template <int NT, int KIND, bool LIFT>
+template <int NT, int KIND, bool LIFT, bool CHW = false>
 __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, int wpe,
                                          int8_t* lds, const float* pal, const uint32_t* flat,
                                          Frame* frames, float* su, float* sv, int* upper) {
@@ -1475,13 +1510,18 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   const int npix = p.W * p.H;
   const bool flat_img = (uvm != kUvShaded);
   int fill = -1;
-  if (go && ((npix * 3) & 15) == 0 && (flat_img || obs_mode != kObsF32))
+  // CHW kernels: only the planar uint8 image has a block fill (whole planes).
+  const bool can_fill = CHW ? (obs_mode == kObsU8Chw) & ((npix & 15) == 0)
+                            : (flat_img || obs_mode != kObsF32);
+  if (go && ((npix * 3) & 15) == 0 && can_fill)
     fill = __builtin_amdgcn_readfirstlane(flat_img ? frames[e].flat_fill : frames[e].fill_block);
   if (fill >= 0) {
     const uint32_t rgb = flat_img ? flat[fill < kFlatSky ? fill : kFlatSky]
                                   : to_u8(pal[fill * 3]) | (to_u8(pal[fill * 3 + 1]) << 8) |
                                         (to_u8(pal[fill * 3 + 2]) << 16);
-    fill_rgb8(reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3, npix, rgb, et, ent);
+    uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
+    if (CHW) fill_planes8(out, npix, rgb, et, ent);
+    else fill_rgb8(out, npix, rgb, et, ent);
     if (KIND == kItemStep && any_done) store_fresh_world(p, gworld, nullptr, nvox, et, ent);
   } else if (go) {
     // Every wave of the env writes the same 8 octant words before it reads
@@ -1491,15 +1531,15 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     __builtin_amdgcn_wave_barrier();
     const Frame f = uniform_frame(&frames[e]);
     const OctPos op{frames[e].pos0};
+    const bool f32 = obs_mode == (CHW ? kObsF32Chw : kObsF32);
     if (uvm != kUvShaded && obs_mode == kObsU8Fp16)
       render_env_flat<LIFT, _Float16>(p, b, world, flat, f, op, uvm, et, ent, su, sv);
     else if (uvm != kUvShaded)
       render_env_flat<LIFT, float>(p, b, world, flat, f, op, uvm, et, ent, su, sv);
-    else if (obs_mode == kObsF32 && p.vd_far)
-      render_env<1, LIFT, true>(p, b, world, pal, f, op, et, ent, su, sv);
-    else if (obs_mode == kObsF32) render_env<1, LIFT, false>(p, b, world, pal, f, op, et, ent, su, sv);
-    else if (p.vd_far) render_env<0, LIFT, true>(p, b, world, pal, f, op, et, ent, su, sv);
-    else render_env<0, LIFT, false>(p, b, world, pal, f, op, et, ent, su, sv);
+    else if (f32 && p.vd_far) render_env<1, LIFT, true>(p, b, world, pal, f, op, et, ent, su, sv, CHW);
+    else if (f32) render_env<1, LIFT, false>(p, b, world, pal, f, op, et, ent, su, sv, CHW);
+    else if (p.vd_far) render_env<0, LIFT, true>(p, b, world, pal, f, op, et, ent, su, sv, CHW);
+    else render_env<0, LIFT, false>(p, b, world, pal, f, op, et, ent, su, sv, CHW);
     if (KIND == kItemStep && any_done) store_fresh_world(p, gworld, nullptr, nvox, et, ent);
   }
 }
@@ -1508,8 +1548,9 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
 // Kernels
 // ---------------------------------------------------------------------------
 
-// One group, one item per workgroup: grid = ceil(B / epw).
-template <int NT, int KIND, bool LIFT>
+// One group, one item per workgroup: grid = ceil(B / epw).  CHW: the planar
+// image modes, in kernels of their own.
+template <int NT, int KIND, bool LIFT, bool CHW = false>
 __global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
@@ -1519,8 +1560,8 @@ __global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
   __shared__ int s_upper[NT / kWave];
   load_palette(s_pal, s_flat, threadIdx.x);
   const int epw = (NT / kWave) / wpe;
-  run_item<NT, KIND, LIFT>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_flat, s_frames,
-                           s_u, s_v, s_upper);
+  run_item<NT, KIND, LIFT, CHW>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_flat,
+                                s_frames, s_u, s_v, s_upper);
 }
 
 // Full-height pass after a staged-height step launch (Params::ys): the envs
@@ -1528,7 +1569,7 @@ __global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
 // (all NT / 64 waves on it).  Launched with p.ys = Y; the list is complete
 // because the step launch precedes this one on the stream.  The last
 // workgroup to finish empties the list for the next launch on this slot.
-template <int NT, bool LIFT>
+template <int NT, bool LIFT, bool CHW = false>
 __global__ __launch_bounds__(NT) void k_env_fb(Params p, int obs_mode) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
@@ -1543,8 +1584,8 @@ __global__ __launch_bounds__(NT) void k_env_fb(Params p, int obs_mode) {
   load_palette(s_pal, s_flat, threadIdx.x);
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     __syncthreads();  // the previous env is done with LDS
-    run_item<NT, kItemStep, LIFT>(p, obs_mode, (int)list[2 + i], NT / kWave, s_lds, s_pal, s_flat,
-                                  s_frames, s_u, s_v, s_upper);
+    run_item<NT, kItemStep, LIFT, CHW>(p, obs_mode, (int)list[2 + i], NT / kWave, s_lds, s_pal,
+                                       s_flat, s_frames, s_u, s_v, s_upper);
   }
   if (threadIdx.x == 0) {
     __threadfence();

@@ -146,10 +146,19 @@ class MineEnv:
         seed: int = 42,
         obs_dtype: str = "float32",
         look_delta: float = 0.1,
+        obs_layout: str = "hwc",
     ):
         assert "cuda" in str(device), "MineEnv requires CUDA device"
         assert obs_dtype in _REFERENCE_OBS, \
             "obs_dtype must be one of " + ", ".join(repr(o) for o in _REFERENCE_OBS)
+        # "chw": the shaded uint8 / float32 observations written planar,
+        # [B, 3, H, W] -- what a convolutional policy reads (the reference's
+        # training example permutes every observation, examples/train_wood.py:53).
+        if obs_layout not in ("hwc", "chw"):
+            raise ValueError("obs_layout must be 'hwc' or 'chw'")
+        if obs_layout == "chw" and obs_dtype not in ("uint8", "float32"):
+            raise ValueError("obs_layout 'chw' is available for obs_dtype 'uint8' and 'float32'")
+        self.obs_layout = obs_layout
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -209,7 +218,8 @@ class MineEnv:
         self.pitches = torch.zeros((B,), dtype=torch.float32, device=dev)
         self.on_ground = torch.ones((B,), dtype=torch.bool, device=dev)
         obs_t = torch.uint8 if self.obs_dtype != "float32" else torch.float32
-        self.obs_buffer = torch.zeros((B, H, W, 3), dtype=obs_t, device=dev)
+        shape = (B, 3, H, W) if self.obs_layout == "chw" else (B, H, W, 3)
+        self.obs_buffer = torch.zeros(shape, dtype=obs_t, device=dev)
         self.cameras = torch.zeros((B, 5), dtype=torch.float32, device=dev)
         # Camera basis of the prebasis variants (minecu/__init__.py:237-241).
         self.basis = (torch.zeros((B, 14), dtype=torch.float32, device=dev)
@@ -241,6 +251,8 @@ class MineEnv:
             c.episode_length = -1
         c.look_delta = self.look_delta
         c.obs_mode = _OBS_MODES[self.obs_dtype]
+        if self.obs_layout == "chw":
+            c.obs_mode = _lib.MC_OBS_U8_CHW if self.obs_dtype == "uint8" else _lib.MC_OBS_F32_CHW
         s = _lib.McState()
         for name, t in (("voxels", self.voxels), ("positions", self.positions),
                         ("velocities", self.velocities), ("yaws", self.yaws),
@@ -366,6 +378,8 @@ class MineEnv:
 
     @property
     def observation_shape(self) -> Tuple[int, int, int]:
+        if self.obs_layout == "chw":
+            return (3, self.render_height, self.render_width)
         return (self.render_height, self.render_width, 3)
 
     # -- checkpoint / resume ---------------------------------------------------

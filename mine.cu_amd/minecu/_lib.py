@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libminecu_hip.so")
 
 MC_OK, MC_E_INVALID, MC_E_HIP, MC_E_UNSUPPORTED = 0, -1, -2, -3
 MC_OBS_U8, MC_OBS_F32, MC_OBS_U8_MINIMAL, MC_OBS_U8_PREBASIS, MC_OBS_U8_FP16 = 0, 1, 2, 3, 4
+MC_OBS_U8_CHW, MC_OBS_F32_CHW = 5, 6
 MC_ACT_BUTTONS, MC_ACT_DISCRETE = 0, 1
 MC_WORKSPACE_BYTES = 256
 

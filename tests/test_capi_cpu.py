@@ -61,8 +61,10 @@ def test_null_and_range_errors():
     assert "non-NULL" in _err(L) and not out.value
     assert L.mc_env_step(None, v, 0, v, None, None) == _l.MC_E_INVALID
     assert L.mc_debug_sincosf(None, None, None, 4, None) == _l.MC_E_INVALID
-    # flat variants: obs modes 2..4 are valid, 5 is not; NULL basis is refused
-    assert L.mc_render(v, v, v, 5, 4, 8, 8, 8, 8, 8, 64, 32.0, 70.0, None) == _l.MC_E_INVALID
+    # mc_render takes obs modes 0..4 (flat variants 2..4); the planar modes 5..6 render
+    # from MineEnv state only, 7 is no mode; NULL basis is refused
+    for mode in (5, 6, 7):
+        assert L.mc_render(v, v, v, mode, 4, 8, 8, 8, 8, 8, 64, 32.0, 70.0, None) == _l.MC_E_INVALID
     assert L.mc_launch_precompute_camera_basis(None, v, 4, 8, 8, 70.0, None) == _l.MC_E_INVALID
     assert L.mc_launch_render_uint8_prebasis(v, None, v, 4, 8, 8, 8, 64, None) == _l.MC_E_INVALID
     args = [v] * 21
