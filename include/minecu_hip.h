@@ -61,6 +61,12 @@ enum {
   MC_OBS_F32_CHW = 6
 };
 
+/* World generators (mc_config.terrain).  No counterpart in the reference, whose generator ignores
+ * its seed (kernels.cu:1800-1828): MC_TERRAIN_NOISE gives each env a seeded height map, column
+ * heights ground_height + d with d in [-terrain_amplitude, terrain_amplitude] (integer value noise
+ * on a 4-column lattice), with the tree and the house standing on it. */
+enum { MC_TERRAIN_FLAT = 0, MC_TERRAIN_NOISE = 1 };
+
 /* Action encodings accepted by the step. */
 enum {
   MC_ACT_BUTTONS = 0,  /* int8 [B,8] multi-hot + f32 [B,2] look (minecu/__init__.py:304-331) */
@@ -82,11 +88,14 @@ typedef struct mc_config {
   float reward_value;
   float spawn_x, spawn_y, spawn_z;
   int ground_height;
-  unsigned int seed;       /* accepted and unused, as in the reference (kernels.cu:1800) */
+  unsigned int seed;       /* unused by the flat world, as in the reference (kernels.cu:1800);
+                              the MC_TERRAIN_NOISE height maps are drawn from it */
   int tree_x, tree_z, house_x, house_z;
   int episode_length;      /* < 0: no episode limit (the reference's None) */
   float look_delta;        /* radians per discrete look action */
   int obs_mode;            /* MC_OBS_* */
+  int terrain;             /* MC_TERRAIN_* (0: the reference's flat world) */
+  int terrain_amplitude;   /* MC_TERRAIN_NOISE: 0..8, and ground_height - amplitude >= 2 */
 } mc_config;
 
 /* Device pointers of one env group's state (caller-owned, mutated in place). */

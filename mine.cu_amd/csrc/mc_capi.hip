@@ -81,6 +81,20 @@ int check_image(int B, int W, int H) {
   return MC_OK;
 }
 
+int check_terrain(const mc_config* c) {
+  if (c->terrain == MC_TERRAIN_FLAT) return MC_OK;
+  if (c->terrain != MC_TERRAIN_NOISE)
+    return fail(MC_E_INVALID, "terrain must be one of MC_TERRAIN_* (got %d)", c->terrain);
+  if (c->terrain_amplitude < 0 || c->terrain_amplitude > 8)
+    return fail(MC_E_INVALID, "terrain_amplitude must be in [0, 8] (got %d)", c->terrain_amplitude);
+  if (c->ground_height - c->terrain_amplitude < 2)
+    return fail(MC_E_INVALID, "ground_height - terrain_amplitude must be >= 2 (got %d - %d)",
+                c->ground_height, c->terrain_amplitude);
+  if (c->obs_mode == MC_OBS_U8_CHW || c->obs_mode == MC_OBS_F32_CHW)
+    return fail(MC_E_UNSUPPORTED, "MC_TERRAIN_NOISE renders the interleaved obs modes only");
+  return MC_OK;
+}
+
 int validate_config(const mc_config* c) {
   if (!c) return fail(MC_E_INVALID, "config is NULL");
   int r = check_dims(c->batch_size, c->world_x, c->world_y, c->world_z);
@@ -89,7 +103,7 @@ int validate_config(const mc_config* c) {
   if (r) return r;
   if (c->obs_mode < MC_OBS_U8 || c->obs_mode > MC_OBS_F32_CHW)
     return fail(MC_E_INVALID, "obs_mode must be one of MC_OBS_* (got %d)", c->obs_mode);
-  return MC_OK;
+  return check_terrain(c);
 }
 
 int validate_state(const mc_state* s) {
@@ -139,6 +153,9 @@ mc::Params make_params(const mc_config* c, const mc_state* s) {
   p.spawn_y = c->spawn_y;
   p.spawn_z = c->spawn_z;
   p.ground_height = c->ground_height;
+  p.terrain = c->terrain == MC_TERRAIN_NOISE ? 1 : 0;
+  p.terrain_seed = c->seed;
+  p.terrain_amp = p.terrain ? c->terrain_amplitude : 0;
   p.tree_x = c->tree_x;
   p.tree_z = c->tree_z;
   p.house_x = c->house_x;
@@ -216,9 +233,12 @@ int launch_kind(const mc::Params& p, int obs_mode, int wpe, hipStream_t st) {
   const int epw = (NT / mc::kWave) / wpe;
   const int grid = (p.B + epw - 1) / epw;
   const size_t lds = (size_t)mc::item_lds_bytes(epw, p.X, p.ys, p.Z);
-  // The planar modes run kernels of their own: the per-pixel store form is a
-  // template constant, so the interleaved kernels keep their code unchanged.
-  if (planar_mode(obs_mode))
+  // The planar modes and the seeded terrain run kernels of their own (template
+  // constants), so the flat-world interleaved kernels keep their code unchanged.
+  if (p.terrain)
+    hipLaunchKernelGGL((mc::k_env_terrain<NT, KIND, LIFT>), dim3(grid), dim3(NT), lds, st, p,
+                       obs_mode, wpe);
+  else if (planar_mode(obs_mode))
     hipLaunchKernelGGL((mc::k_env<NT, KIND, LIFT, true>), dim3(grid), dim3(NT), lds, st, p,
                        obs_mode, wpe);
   else
@@ -274,7 +294,15 @@ int launch_deferred(const mc::Params& q, int obs_mode, int lift, hipStream_t st)
   const int full = mc::item_lds_bytes(1, p.X, p.Y, p.Z);
   if (full > lds_budget()) return fail(MC_E_UNSUPPORTED, "world does not fit the LDS budget");
   const dim3 grid(std::min(p.B, 64));
-  if (planar_mode(obs_mode)) {
+  if (p.terrain) {
+    if (full > 80 * 1024) {
+      if (lift) hipLaunchKernelGGL((mc::k_env_fb_terrain<512, true>), grid, dim3(512), full, st, p, obs_mode);
+      else hipLaunchKernelGGL((mc::k_env_fb_terrain<512, false>), grid, dim3(512), full, st, p, obs_mode);
+    } else {
+      if (lift) hipLaunchKernelGGL((mc::k_env_fb_terrain<256, true>), grid, dim3(256), full, st, p, obs_mode);
+      else hipLaunchKernelGGL((mc::k_env_fb_terrain<256, false>), grid, dim3(256), full, st, p, obs_mode);
+    }
+  } else if (planar_mode(obs_mode)) {
     if (full > 80 * 1024) {
       if (lift) hipLaunchKernelGGL((mc::k_env_fb<512, true, true>), grid, dim3(512), full, st, p, obs_mode);
       else hipLaunchKernelGGL((mc::k_env_fb<512, false, true>), grid, dim3(512), full, st, p, obs_mode);
@@ -293,7 +321,8 @@ int launch_deferred(const mc::Params& q, int obs_mode, int lift, hipStream_t st)
 }
 
 // Layers a MineEnv step stages in LDS: the fresh world's solids end at
-// ground_height + 4 (the house roof); the agent's eye stays below
+// ground_height + 4 (the house roof; + terrain_amplitude on the seeded
+// terrain); the agent's eye stays below
 // max(spawn_y, Y/4 + jump apex) + 1.6 (the physics has no block collision:
 // only gravity, jumps and the ground clamp at Y/4), and the raycast reaches
 // 4 above the eye.  Envs outside these bounds are deferred to the
@@ -305,7 +334,8 @@ int staged_layers(const mc_config& c) {
   if (!(c.gravity < 0.0f) || !(c.jump_vel >= 0.0f) || !(c.spawn_y < (float)Y)) return Y;
   const float apex = c.jump_vel * c.jump_vel / (-2.0f * c.gravity);
   const float eye = std::max(c.spawn_y, (float)(Y / 4) + apex) + 1.6f;
-  const float need = std::max((float)(c.ground_height + 5), std::ceil(eye + 4.5f) + 1.0f);
+  const int top = c.ground_height + (c.terrain == MC_TERRAIN_NOISE ? c.terrain_amplitude : 0);
+  const float need = std::max((float)(top + 5), std::ceil(eye + 4.5f) + 1.0f);
   if (!(need < (float)Y)) return Y;
   return std::max(1, (int)need);
 }
@@ -366,12 +396,15 @@ int mc_place_house(int8_t* voxels, int B, int X, int Y, int Z, int house_x, int 
 
 int mc_regen_worlds(const mc_config* cfg, int8_t* voxels, const uint8_t* mask, void* stream) {
   if (!cfg || !voxels) return fail(MC_E_INVALID, "config/voxels is NULL");
-  const int r = check_dims(cfg->batch_size, cfg->world_x, cfg->world_y, cfg->world_z);
+  int r = check_dims(cfg->batch_size, cfg->world_x, cfg->world_y, cfg->world_z);
+  if (r) return r;
+  r = check_terrain(cfg);
   if (r) return r;
   mc::Params p = make_params(cfg, nullptr);
   p.voxels = voxels;
   p.mask = mask;
-  hipLaunchKernelGGL(mc::k_regen, dim3(p.B), dim3(256), 0, S(stream), p);
+  if (p.terrain) hipLaunchKernelGGL(mc::k_regen_terrain, dim3(p.B), dim3(256), 0, S(stream), p);
+  else hipLaunchKernelGGL(mc::k_regen, dim3(p.B), dim3(256), 0, S(stream), p);
   return check_launch("regen_worlds");
 }
 
@@ -1085,8 +1118,9 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
       if (envs[j] == envs[i]) return fail(MC_E_INVALID, "group %d listed twice", i);
     if (envs[i]->graph_mode)
       return fail(MC_E_UNSUPPORTED, "group %d is in graph mode (its slot is device-resident)", i);
-    if (planar_mode(envs[i]->cfg.obs_mode))
-      return fail(MC_E_UNSUPPORTED, "group %d: the MC_OBS_*_CHW modes step alone, not in a batch",
+    if (planar_mode(envs[i]->cfg.obs_mode) || envs[i]->cfg.terrain != MC_TERRAIN_FLAT)
+      return fail(MC_E_UNSUPPORTED,
+                  "group %d: the MC_OBS_*_CHW modes and MC_TERRAIN_NOISE step alone, not in a batch",
                   i);
     if (envs[i]->shape.wpe == 8) nt = 512;
     lift = lift || envs[i]->shape.lift;
@@ -1223,8 +1257,9 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
     if (!envs[i]) return fail(MC_E_INVALID, "env %d is NULL", i);
     for (int j = 0; j < i; ++j)
       if (envs[j] == envs[i]) return fail(MC_E_INVALID, "group %d listed twice", i);
-    if (planar_mode(envs[i]->cfg.obs_mode))
-      return fail(MC_E_UNSUPPORTED, "group %d: the MC_OBS_*_CHW modes step alone, not in a batch",
+    if (planar_mode(envs[i]->cfg.obs_mode) || envs[i]->cfg.terrain != MC_TERRAIN_FLAT)
+      return fail(MC_E_UNSUPPORTED,
+                  "group %d: the MC_OBS_*_CHW modes and MC_TERRAIN_NOISE step alone, not in a batch",
                   i);
     const mc_config& c = envs[i]->cfg;
     const BucketKey k = bucket_of(c);

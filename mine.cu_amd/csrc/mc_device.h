@@ -160,6 +160,11 @@ struct Params {
   // full-height pass (k_env_fb), which empties the list when it ends.
   int ys;
   uint32_t* fb;
+  // Seeded terrain (the k_*_terrain kernels only): column heights
+  // ground_height + d, d in [-terrain_amp, terrain_amp], from terrain_seed.
+  int terrain;                 // host side: 1 launches the k_*_terrain kernels
+  uint32_t terrain_seed;
+  int terrain_amp;
 };
 
 // Camera frame of one env, published by wave 0 through LDS.
@@ -245,6 +250,61 @@ __device__ __forceinline__ int8_t fresh_voxel(const Params& p, int y, int x, int
   return v;
 }
 
+// ---------------------------------------------------------------------------
+// Seeded terrain (MineEnv(terrain="noise")).  No counterpart in the reference,
+// whose world generator takes a seed and ignores it (src/kernels.cu:1800-1828).
+// Env b's column (x, z) is solid up to height ground_height + d:
+//   corners every 4 columns carry c = mix(seed, b, x / 4, z / 4) mod (2A + 1);
+//   d = round(bilinear mean of the 4 corners around (x, z)) - A, in exact
+//   integers: s = sum of corner * (4 - fx or fx) * (4 - fz or fz) over 16,
+//   d = (s + 8) / 16 - A, with fx = x % 4, fz = z % 4.
+// mix is MurmurHash3's 32-bit finaliser (fmix32, Austin Appleby, public
+// domain) chained over the inputs.  The tree and the house stand on the
+// surface of their own (x, z) corner column, with the flat world's shapes;
+// house cells it does not write (interior, door) keep the terrain.
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+__device__ __forceinline__ int terrain_corner(const Params& p, int b, int cx, int cz) {
+  const uint32_t h = fmix32(p.terrain_seed ^ fmix32((uint32_t)b * 0x9e3779b9u ^
+                                                    fmix32((uint32_t)cx * 0x632be5abu +
+                                                           (uint32_t)cz)));
+  return (int)(h % (uint32_t)(2 * p.terrain_amp + 1));
+}
+
+// Solid height of env b's column (x, z) (ground_height in the flat world).
+__device__ __forceinline__ int terrain_height(const Params& p, int b, int x, int z) {
+  const int cx = x >> 2, cz = z >> 2, fx = x & 3, fz = z & 3;
+  const int s = (terrain_corner(p, b, cx, cz) * (4 - fx) + terrain_corner(p, b, cx + 1, cz) * fx) *
+                    (4 - fz) +
+                (terrain_corner(p, b, cx, cz + 1) * (4 - fx) +
+                 terrain_corner(p, b, cx + 1, cz + 1) * fx) * fz;
+  return p.ground_height - p.terrain_amp + ((s + 8) >> 4);
+}
+
+__device__ __forceinline__ int8_t terrain_voxel(const Params& p, int b, int y, int x, int z,
+                                                int tree_h, int house_h) {
+  int8_t v = layer_block(y, terrain_height(p, b, x, z));
+  if (x == p.tree_x && z == p.tree_z) {
+    const int dy = y - tree_h;
+    if (dy >= 0 && dy < 3) v = kOakLog;
+    else if (dy == 3) v = kLeaves;
+  }
+  const int hx = x - p.house_x, hy = y - house_h, hz = z - p.house_z;
+  if (hx >= 0 && hx < 5 && hz >= 0 && hz < 4 && hy >= 0 && hy <= 4) {
+    const int8_t h = house_block(hy, hx, hz);
+    if (h != kNoWrite) v = h;
+  }
+  return v;
+}
+
 __device__ __forceinline__ uint32_t pack4(int8_t a, int8_t b, int8_t c, int8_t d) {
   return (uint32_t)(uint8_t)a | ((uint32_t)(uint8_t)b << 8) | ((uint32_t)(uint8_t)c << 16) |
          ((uint32_t)(uint8_t)d << 24);
@@ -288,9 +348,41 @@ __device__ __forceinline__ v4u fresh_chunk(const Params& p, int i0) {
   return v4u{w[0], w[1], w[2], w[3]};
 }
 
-// Write env b's fresh world to HBM (and to LDS when lds != nullptr).
-__device__ __forceinline__ void store_fresh_world(const Params& p, int8_t* gworld, int8_t* lds,
-                                                  int nvox, int tid, int nt) {
+// Write env b's fresh world to HBM (and to LDS when lds != nullptr).  TER:
+// the seeded terrain world.
+template <bool TER = false>
+__device__ __forceinline__ void store_fresh_world(const Params& p, int b, int8_t* gworld,
+                                                  int8_t* lds, int nvox, int tid, int nt) {
+  if (TER) {
+    const int XZ = p.X * p.Z;
+    const int tree_h = terrain_height(p, b, p.tree_x, p.tree_z);
+    const int house_h = terrain_height(p, b, p.house_x, p.house_z);
+    if ((nvox & 3) == 0) {
+      // 4 voxels per 32-bit store.
+      typedef MC_GLOBAL uint32_t gu32;
+      gu32* g = (gu32*)gworld;
+      uint32_t* l = reinterpret_cast<uint32_t*>(lds);
+      for (int c = tid; c < (nvox >> 2); c += nt) {
+        uint32_t w = 0u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int i = (c << 2) + k;
+          const int y = i / XZ, rem = i - y * XZ, x = rem / p.Z, z = rem - x * p.Z;
+          w |= (uint32_t)(uint8_t)terrain_voxel(p, b, y, x, z, tree_h, house_h) << (8 * k);
+        }
+        g[c] = w;
+        if (lds) l[c] = w;
+      }
+    } else {
+      for (int i = tid; i < nvox; i += nt) {
+        const int y = i / XZ, rem = i - y * XZ, x = rem / p.Z, z = rem - x * p.Z;
+        const int8_t v = terrain_voxel(p, b, y, x, z, tree_h, house_h);
+        gworld[i] = v;
+        if (lds) lds[i] = v;
+      }
+    }
+    return;
+  }
   if ((nvox & 15) == 0) {
     guint4* g = (guint4*)gworld;
     v4u* l = reinterpret_cast<v4u*>(lds);
@@ -1437,7 +1529,7 @@ __host__ __device__ __forceinline__ int item_lds_bytes(int epw, int X, int Y, in
 }
 
 // upper: one word per wave of the workgroup (staged-height check).
-template <int NT, int KIND, bool LIFT, bool CHW = false>
+template <int NT, int KIND, bool LIFT, bool CHW = false, bool TER = false>
 __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, int wpe,
                                          int8_t* lds, const float* pal, const uint32_t* flat,
                                          Frame* frames, float* su, float* sv, int* upper) {
@@ -1474,7 +1566,7 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
                                                               : kUvShaded;
   fill_uv(p, uvm, su, sv, tid, NT);
   if (active) {
-    if (regen) store_fresh_world(p, gworld, world, nvox, et, ent);
+    if (regen) store_fresh_world<TER>(p, b, gworld, world, nvox, et, ent);
     else stage_world(world, gworld, nstage, et, ent);
   }
   if (staged) {
@@ -1522,7 +1614,7 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
     if (CHW) fill_planes8(out, npix, rgb, et, ent);
     else fill_rgb8(out, npix, rgb, et, ent);
-    if (KIND == kItemStep && any_done) store_fresh_world(p, gworld, nullptr, nvox, et, ent);
+    if (KIND == kItemStep && any_done) store_fresh_world<TER>(p, b, gworld, nullptr, nvox, et, ent);
   } else if (go) {
     // Every wave of the env writes the same 8 octant words before it reads
     // them; a wave's LDS operations complete in order.
@@ -1540,7 +1632,7 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     else if (f32) render_env<1, LIFT, false>(p, b, world, pal, f, op, et, ent, su, sv, CHW);
     else if (p.vd_far) render_env<0, LIFT, true>(p, b, world, pal, f, op, et, ent, su, sv, CHW);
     else render_env<0, LIFT, false>(p, b, world, pal, f, op, et, ent, su, sv, CHW);
-    if (KIND == kItemStep && any_done) store_fresh_world(p, gworld, nullptr, nvox, et, ent);
+    if (KIND == kItemStep && any_done) store_fresh_world<TER>(p, b, gworld, nullptr, nvox, et, ent);
   }
 }
 
@@ -1586,6 +1678,50 @@ __global__ __launch_bounds__(NT) void k_env_fb(Params p, int obs_mode) {
     __syncthreads();  // the previous env is done with LDS
     run_item<NT, kItemStep, LIFT, CHW>(p, obs_mode, (int)list[2 + i], NT / kWave, s_lds, s_pal,
                                        s_flat, s_frames, s_u, s_v, s_upper);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&list[1], 1u) == gridDim.x - 1) {
+      list[0] = 0u;
+      list[1] = 0u;
+      __threadfence();
+    }
+  }
+}
+
+// k_env and k_env_fb for the seeded terrain world (interleaved images only):
+// kernels of their own, so the flat-world kernels keep their code.
+template <int NT, int KIND, bool LIFT>
+__global__ __launch_bounds__(NT) void k_env_terrain(Params p, int obs_mode, int wpe) {
+  extern __shared__ __align__(16) int8_t s_lds[];
+  __shared__ float s_pal[kPaletteLen];
+  __shared__ uint32_t s_flat[kFlatLen];
+  __shared__ Frame s_frames[NT / kWave];
+  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
+  __shared__ int s_upper[NT / kWave];
+  load_palette(s_pal, s_flat, threadIdx.x);
+  const int epw = (NT / kWave) / wpe;
+  run_item<NT, KIND, LIFT, false, true>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_flat,
+                                        s_frames, s_u, s_v, s_upper);
+}
+
+template <int NT, bool LIFT>
+__global__ __launch_bounds__(NT) void k_env_fb_terrain(Params p, int obs_mode) {
+  extern __shared__ __align__(16) int8_t s_lds[];
+  __shared__ float s_pal[kPaletteLen];
+  __shared__ uint32_t s_flat[kFlatLen];
+  __shared__ Frame s_frames[NT / kWave];
+  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
+  __shared__ int s_upper[NT / kWave];
+  const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
+  uint32_t* list = p.fb + (size_t)slot * (p.B + 2);
+  const int n = (int)min(list[0], (uint32_t)p.B);
+  if (n == 0) return;
+  load_palette(s_pal, s_flat, threadIdx.x);
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    __syncthreads();
+    run_item<NT, kItemStep, LIFT, false, true>(p, obs_mode, (int)list[2 + i], NT / kWave, s_lds,
+                                               s_pal, s_flat, s_frames, s_u, s_v, s_upper);
   }
   if (threadIdx.x == 0) {
     __threadfence();
@@ -1784,7 +1920,15 @@ __global__ __launch_bounds__(256) void k_regen(Params p) {
   const int b = blockIdx.x;
   if (p.mask && !p.mask[b]) return;
   const int nvox = p.X * p.Y * p.Z;
-  store_fresh_world(p, p.voxels + (int64_t)b * nvox, nullptr, nvox, threadIdx.x, 256);
+  store_fresh_world(p, b, p.voxels + (int64_t)b * nvox, nullptr, nvox, threadIdx.x, 256);
+}
+
+// k_regen for the seeded terrain world.
+__global__ __launch_bounds__(256) void k_regen_terrain(Params p) {
+  const int b = blockIdx.x;
+  if (p.mask && !p.mask[b]) return;
+  const int nvox = p.X * p.Y * p.Z;
+  store_fresh_world<true>(p, b, p.voxels + (int64_t)b * nvox, nullptr, nvox, threadIdx.x, 256);
 }
 
 // generate_world_kernel (src/kernels.cu:1793-1828), 16 voxels per thread.

@@ -147,6 +147,8 @@ class MineEnv:
         obs_dtype: str = "float32",
         look_delta: float = 0.1,
         obs_layout: str = "hwc",
+        terrain: str = "flat",
+        terrain_amplitude: int = 3,
     ):
         assert "cuda" in str(device), "MineEnv requires CUDA device"
         assert obs_dtype in _REFERENCE_OBS, \
@@ -159,6 +161,14 @@ class MineEnv:
         if obs_layout == "chw" and obs_dtype not in ("uint8", "float32"):
             raise ValueError("obs_layout 'chw' is available for obs_dtype 'uint8' and 'float32'")
         self.obs_layout = obs_layout
+        # "noise": each env gets a seeded height map (column heights
+        # ground_height + [-terrain_amplitude, terrain_amplitude]) with the tree
+        # and house on it; "flat" is the reference's world, whose seed is unused.
+        if terrain not in ("flat", "noise"):
+            raise ValueError("terrain must be 'flat' or 'noise'")
+        if terrain == "noise" and obs_layout == "chw":
+            raise ValueError("terrain 'noise' renders the 'hwc' layout only")
+        self.terrain, self.terrain_amplitude = terrain, int(terrain_amplitude)
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -253,6 +263,8 @@ class MineEnv:
         c.obs_mode = _OBS_MODES[self.obs_dtype]
         if self.obs_layout == "chw":
             c.obs_mode = _lib.MC_OBS_U8_CHW if self.obs_dtype == "uint8" else _lib.MC_OBS_F32_CHW
+        c.terrain = _lib.MC_TERRAIN_NOISE if self.terrain == "noise" else _lib.MC_TERRAIN_FLAT
+        c.terrain_amplitude = self.terrain_amplitude
         s = _lib.McState()
         for name, t in (("voxels", self.voxels), ("positions", self.positions),
                         ("velocities", self.velocities), ("yaws", self.yaws),

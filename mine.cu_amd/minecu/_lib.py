@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libminecu_hip.so")
 MC_OK, MC_E_INVALID, MC_E_HIP, MC_E_UNSUPPORTED = 0, -1, -2, -3
 MC_OBS_U8, MC_OBS_F32, MC_OBS_U8_MINIMAL, MC_OBS_U8_PREBASIS, MC_OBS_U8_FP16 = 0, 1, 2, 3, 4
 MC_OBS_U8_CHW, MC_OBS_F32_CHW = 5, 6
+MC_TERRAIN_FLAT, MC_TERRAIN_NOISE = 0, 1
 MC_ACT_BUTTONS, MC_ACT_DISCRETE = 0, 1
 MC_WORKSPACE_BYTES = 256
 
@@ -37,6 +38,7 @@ class McConfig(ctypes.Structure):
         ("ground_height", _I), ("seed", _U),
         ("tree_x", _I), ("tree_z", _I), ("house_x", _I), ("house_z", _I),
         ("episode_length", _I), ("look_delta", _F), ("obs_mode", _I),
+        ("terrain", _I), ("terrain_amplitude", _I),
     ]
 
 

@@ -58,6 +58,11 @@ def lib():
         L.orc_generate_world.argtypes = [P, i, i, i, i, i, u]
         L.orc_place_tree.argtypes = [P, i, i, i, i, i, i, i]
         L.orc_place_house.argtypes = [P, i, i, i, i, i, i, i]
+        L.orc_fmix32.argtypes = [u]
+        L.orc_fmix32.restype = u
+        L.orc_terrain_column.argtypes = [i, i, i, i, i, u]
+        L.orc_terrain_heights.argtypes = [P, i, i, i, i, i, u]
+        L.orc_terrain_world.argtypes = [P, i, i, i, i, i, i, u, i, i, i, i]
         L.orc_reset.argtypes = [P, P, P, P, P, P, i, f, f, f]
         L.orc_discrete_to_buttons.argtypes = [P, i, f, P, P]
         L.orc_decode_actions.argtypes = [P, P, P, P, P, P, P, P, P, P, i, f]
@@ -101,15 +106,36 @@ def half_fov_tan(fov_degrees: float) -> float:
     return float(lib().orc_half_fov_tan(fov_degrees))
 
 
-def fresh_world(B, X, Y, Z, ground_height, tree_xz, house_xz, seed=42) -> np.ndarray:
+def fresh_world(B, X, Y, Z, ground_height, tree_xz, house_xz, seed=42,
+                terrain_amplitude=None) -> np.ndarray:
     """generate_world + place_tree + place_house, the reference's regen order
-    (minecu/__init__.py:258-268)."""
+    (minecu/__init__.py:258-268).  terrain_amplitude: the seeded noise terrain
+    of that amplitude instead (None: the reference's flat world)."""
     v = np.empty((B, Y, X, Z), dtype=np.int8)
     L = lib()
+    if terrain_amplitude is not None:
+        L.orc_terrain_world(_p(v), B, X, Y, Z, ground_height, terrain_amplitude,
+                            seed & 0xFFFFFFFF, tree_xz[0], tree_xz[1], house_xz[0], house_xz[1])
+        return v
     L.orc_generate_world(_p(v), B, X, Y, Z, ground_height, seed)
     L.orc_place_tree(_p(v), B, X, Y, Z, tree_xz[0], tree_xz[1], ground_height)
     L.orc_place_house(_p(v), B, X, Y, Z, house_xz[0], house_xz[1], ground_height)
     return v
+
+
+def terrain_heights(b, X, Z, ground_height, amplitude, seed) -> np.ndarray:
+    """[X, Z] solid heights of env b's seeded terrain (before tree and house)."""
+    h = np.empty((X, Z), dtype=np.int32)
+    lib().orc_terrain_heights(_p(h), b, X, Z, ground_height, amplitude, seed & 0xFFFFFFFF)
+    return h
+
+
+def terrain_column(b, x, z, ground_height, amplitude, seed) -> int:
+    return int(lib().orc_terrain_column(b, x, z, ground_height, amplitude, seed & 0xFFFFFFFF))
+
+
+def fmix32(h: int) -> int:
+    return int(lib().orc_fmix32(h & 0xFFFFFFFF))
 
 
 def render(voxels, cameras, width, height, max_steps=64, view_distance=32.0,
@@ -198,7 +224,7 @@ class OracleEnv:
                  fov_degrees=70.0, dt=0.05, gravity=-20.0, walk_speed=4.0, sprint_mult=1.5,
                  jump_vel=8.0, ground_height=8, target_block=OAKLOG, reward_value=1.0,
                  episode_length=None, seed=42, obs_dtype="uint8", look_delta=0.1,
-                 nthreads=None):
+                 nthreads=None, terrain="flat", terrain_amplitude=3):
         assert obs_dtype in _FLAT, obs_dtype
         self.batch_size = B = batch_size
         self.world_x = world_x if world_x is not None else world_size
@@ -211,6 +237,7 @@ class OracleEnv:
         self.ground_height, self.target_block, self.reward_value = ground_height, target_block, reward_value
         self.episode_length, self.seed, self.obs_dtype = episode_length, seed, obs_dtype
         self.look_delta = look_delta
+        self.terrain_amplitude = terrain_amplitude if terrain == "noise" else None
         self.nthreads = nthreads or default_threads()
         # Spawn / tree / house (minecu/__init__.py:196-206).
         self.spawn_x = self.world_x / 2.0
@@ -239,7 +266,7 @@ class OracleEnv:
     def _fresh(self):
         return fresh_world(self.batch_size, self.world_x, self.world_y, self.world_z,
                            self.ground_height, (self.tree_x, self.tree_z),
-                           (self.house_x, self.house_z), self.seed)
+                           (self.house_x, self.house_z), self.seed, self.terrain_amplitude)
 
     def _regen(self):
         self.voxels[...] = self._fresh()

@@ -684,4 +684,86 @@ void orc_f16(const float* a, const float* b, uint16_t* ha, uint16_t* sum, uint8_
 // Largest sample length a caller may pass (documentation aid for ctypes users).
 int orc_version() { return 1; }
 
+// ---- seeded terrain ---------------------------------------------------------
+// No reference algorithm: the reference's generator ignores its seed
+// (src/kernels.cu:1800-1828) and lists noise terrain as a TODO (README.md:71).
+// This restates the library's MC_TERRAIN_NOISE definition from its statement:
+//   corner(b, i, j) = F(seed ^ F(b * 0x9e3779b9 ^ F(i * 0x632be5ab + j))) mod (2A + 1),
+//   F = MurmurHash3's fmix32 (shifts 16, 13, 16; multipliers 0x85ebca6b, 0xc2b2ae35);
+//   height(x, z) = ground - A + floor((bilinear weight sum over 16 + 8) / 16);
+// then generate_world's layers per column, the tree and the house on the
+// surface of their own corner column.  Built here as: the corner lattice of
+// each env first, then its height map, then columns, then the oracle's own
+// place_tree / place_house on that env's slab.
+uint32_t orc_fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+// One column (x, z), any integers: lattice cell by floor division.
+int orc_terrain_column(int b, int x, int z, int ground_height, int amp, unsigned int seed) {
+  auto fdiv4 = [](int a) { return (a - (((a % 4) + 4) % 4)) / 4; };
+  const int i = fdiv4(x), j = fdiv4(z), u = x - 4 * i, w = z - 4 * j;
+  auto corner = [&](int ci, int cj) {
+    const uint32_t inner = orc_fmix32((uint32_t)ci * 0x632be5abu + (uint32_t)cj);
+    return (int)(orc_fmix32(seed ^ orc_fmix32((uint32_t)b * 0x9e3779b9u ^ inner)) %
+                 (uint32_t)(2 * amp + 1));
+  };
+  const int sum = corner(i, j) * (4 - u) * (4 - w) + corner(i + 1, j) * u * (4 - w) +
+                  corner(i, j + 1) * (4 - u) * w + corner(i + 1, j + 1) * u * w;
+  return ground_height - amp + (sum + 8) / 16;
+}
+
+void orc_terrain_heights(int* heights, int b, int X, int Z, int ground_height, int amp,
+                         unsigned int seed) {
+  const int CX = X / 4 + 2, CZ = Z / 4 + 2;
+  std::vector<int> corner((size_t)CX * CZ);
+  for (int i = 0; i < CX; ++i)
+    for (int j = 0; j < CZ; ++j) {
+      const uint32_t inner = orc_fmix32((uint32_t)i * 0x632be5abu + (uint32_t)j);
+      const uint32_t mid = orc_fmix32((uint32_t)b * 0x9e3779b9u ^ inner);
+      corner[(size_t)i * CZ + j] = (int)(orc_fmix32(seed ^ mid) % (uint32_t)(2 * amp + 1));
+    }
+  for (int x = 0; x < X; ++x)
+    for (int z = 0; z < Z; ++z) {
+      const int i = x / 4, j = z / 4, u = x % 4, w = z % 4;
+      const int c00 = corner[(size_t)i * CZ + j], c10 = corner[(size_t)(i + 1) * CZ + j];
+      const int c01 = corner[(size_t)i * CZ + j + 1], c11 = corner[(size_t)(i + 1) * CZ + j + 1];
+      const int sum = c00 * (4 - u) * (4 - w) + c10 * u * (4 - w) + c01 * (4 - u) * w + c11 * u * w;
+      heights[x * Z + z] = ground_height - amp + (sum + 8) / 16;
+    }
+}
+
+void orc_terrain_world(int8_t* voxels, int B, int X, int Y, int Z, int ground_height, int amp,
+                       unsigned int seed, int tree_x, int tree_z, int house_x, int house_z) {
+  const int64_t wxyz = (int64_t)X * Y * Z;
+  std::vector<int> h((size_t)X * Z);
+  for (int b = 0; b < B; ++b) {
+    int8_t* v = voxels + b * wxyz;
+    orc_terrain_heights(h.data(), b, X, Z, ground_height, amp, seed);
+    for (int y = 0; y < Y; ++y)
+      for (int x = 0; x < X; ++x)
+        for (int z = 0; z < Z; ++z) {
+          const int g = h[x * Z + z];
+          int8_t block = AIR;
+          if (y == 0) block = BEDROCK;
+          else if (y < g - 1) block = DIRT;
+          else if (y == g - 1) block = GRASS;
+          v[(int64_t)y * X * Z + x * Z + z] = block;
+        }
+    // The tree and house corner columns may lie outside the world (small
+    // worlds): their surface follows the same definition, evaluated alone.
+    auto at = [&](int x, int z) {
+      if (x >= 0 && x < X && z >= 0 && z < Z) return h[x * Z + z];
+      return orc_terrain_column(b, x, z, ground_height, amp, seed);
+    };
+    orc_place_tree(v, 1, X, Y, Z, tree_x, tree_z, at(tree_x, tree_z));
+    orc_place_house(v, 1, X, Y, Z, house_x, house_z, at(house_x, house_z));
+  }
+}
+
 }  // extern "C"

@@ -1,0 +1,90 @@
+"""Pins of the oracle's seeded terrain (MineEnv(terrain="noise")), which has no
+reference algorithm (the reference's generator ignores its seed,
+src/kernels.cu:1800-1828):
+- the hash is MurmurHash3's fmix32: the finaliser of MurmurHash3_x86_32 on an
+  empty input, whose published values are 0 (seed 0), 0x514E28B7 (seed 1) and
+  0x81F16F39 (seed 0xFFFFFFFF);
+- amplitude 0 reduces to the reference's flat world, bit for bit;
+- heights stay in [ground - A, ground + A] and equal ground - A + corner at the
+  lattice points (a plain-Python corner below);
+- the lattice-first height map and the one-column evaluation agree, also for
+  negative and out-of-world columns;
+- worlds depend on seed and env, and the tree and house stand on the surface."""
+import numpy as np
+import pytest
+
+import oracle
+
+M = 0xFFFFFFFF
+
+
+def test_fmix32_known_answers():
+    assert oracle.fmix32(0) == 0
+    assert oracle.fmix32(1) == 0x514E28B7
+    assert oracle.fmix32(0xFFFFFFFF) == 0x81F16F39
+
+
+def _py_fmix(h):
+    h ^= h >> 16
+    h = (h * 0x85EBCA6B) & M
+    h ^= h >> 13
+    h = (h * 0xC2B2AE35) & M
+    return h ^ (h >> 16)
+
+
+def _py_corner(seed, b, i, j, A):
+    inner = _py_fmix((i * 0x632BE5AB + j) & M)
+    return _py_fmix((seed ^ _py_fmix(((b * 0x9E3779B9) & M) ^ inner)) & M) % (2 * A + 1)
+
+
+@pytest.mark.parametrize("seed,A,gh", [(42, 3, 8), (7, 1, 4), (123456789, 8, 10)])
+def test_heights_range_and_lattice_points(seed, A, gh):
+    for b in (0, 5):
+        h = oracle.terrain_heights(b, 16, 16, gh, A, seed)
+        assert h.min() >= gh - A and h.max() <= gh + A
+        for i in range(4):
+            for j in range(4):
+                assert h[4 * i, 4 * j] == gh - A + _py_corner(seed, b, i, j, A)
+        # between two lattice points of a row the height moves monotonically
+        for j in range(4):
+            for i in range(3):
+                seg = h[4 * i:4 * i + 5, 4 * j]
+                d = np.diff(seg)
+                assert (d >= 0).all() or (d <= 0).all()
+
+
+def test_column_form_matches_lattice_form():
+    h = oracle.terrain_heights(3, 24, 20, 9, 4, 99)
+    for x in range(24):
+        for z in range(20):
+            assert oracle.terrain_column(3, x, z, 9, 4, 99) == h[x, z]
+    # any column (negative, beyond the world) is defined and in range
+    for x, z in [(-1, -1), (-5, 7), (30, 2), (100, 100)]:
+        assert 5 <= oracle.terrain_column(3, x, z, 9, 4, 99) <= 13
+
+
+@pytest.mark.parametrize("ws", [8, 16, 32])
+def test_amplitude_zero_is_the_flat_world(ws):
+    kw = dict(B=3, X=ws, Y=ws, Z=ws, ground_height=8)
+    t, h = (ws // 2, ws // 2 + 2), (ws // 2 + 6, ws // 2 - 2)
+    flat = oracle.fresh_world(kw["B"], ws, ws, ws, 8, t, h, seed=5)
+    for seed in (0, 5, 2 ** 31 + 3):
+        noise = oracle.fresh_world(kw["B"], ws, ws, ws, 8, t, h, seed=seed, terrain_amplitude=0)
+        assert np.array_equal(flat, noise)
+
+
+def test_worlds_depend_on_seed_and_env_and_props_stand_on_the_surface():
+    t, hs = (8, 10), (14, 6)
+    a = oracle.fresh_world(4, 16, 16, 16, 8, t, hs, seed=1, terrain_amplitude=3)
+    b = oracle.fresh_world(4, 16, 16, 16, 8, t, hs, seed=2, terrain_amplitude=3)
+    a2 = oracle.fresh_world(4, 16, 16, 16, 8, t, hs, seed=1, terrain_amplitude=3)
+    assert np.array_equal(a, a2) and not np.array_equal(a, b)
+    assert not np.array_equal(a[0], a[1])
+    for e in range(4):
+        ht = oracle.terrain_column(e, t[0], t[1], 8, 3, 1)
+        col = a[e, :, t[0], t[1]]
+        assert col[ht - 1] == oracle.GRASS and list(col[ht:ht + 3]) == [oracle.OAKLOG] * 3
+        assert col[ht + 3] == oracle.LEAVES
+        hh = oracle.terrain_column(e, hs[0], hs[1], 8, 3, 1)
+        assert a[e, hh, hs[0], hs[1]] == oracle.COBBLESTONE          # house floor corner
+        assert a[e, hh + 4, hs[0] + 1, hs[1] + 1] == oracle.BRICK      # roof
