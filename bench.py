@@ -121,14 +121,14 @@ def cpu_baseline(w, obs, seconds_budget=15.0):
                       f"{done_steps} steps from reset of the same workload ({dt:.1f} s)"}
 
 
-def run_gpu(w, obs, steps, warmup, layout="hwc", terrain="flat"):
+def run_gpu(w, obs, steps, warmup, layout="hwc", terrain="flat", collision=False):
     import torch
     import minecu
     dev = torch.device("cuda", 0)
     env = minecu.MineEnv(batch_size=w["batch"], world_size=w["world"], render_width=w["W"],
                          render_height=w["H"], device=str(dev), obs_dtype=obs,
                          episode_length=w["episode_length"], obs_layout=layout,
-                         terrain=terrain)
+                         terrain=terrain, collision=collision)
     acts = make_actions(w, warmup + steps, dev)
     env.reset()
     for t in range(warmup):
@@ -377,7 +377,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32", action="store_true", help="skip the float32-obs side measurement")
     ap.add_argument("--no-terrain", action="store_true",
-                    help="skip the noise-terrain (terrain='noise') side measurement")
+                    help="skip the noise-terrain (terrain='noise', with and without collision) "
+                         "side measurements")
     ap.add_argument("--no-chw", action="store_true",
                     help="skip the planar-layout (obs_layout='chw') side measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
@@ -455,6 +456,9 @@ def main():
         # world regenerated on done steps is a fresh height map.
         n_ms, _, _ = run_gpu(w, "uint8", min(K, 200), args.warmup, terrain="noise")
         out["noise_terrain_value"] = B * min(K, 200) / (n_ms / 1e3)
+        # ... and on it with voxel collision (agents stand on the surface).
+        c_ms, _, _ = run_gpu(w, "uint8", min(K, 200), args.warmup, terrain="noise", collision=True)
+        out["noise_terrain_collision_value"] = B * min(K, 200) / (c_ms / 1e3)
     if not args.no_batch:
         out["batched"] = run_batched()
     if not args.no_serve:

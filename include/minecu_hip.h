@@ -96,6 +96,8 @@ typedef struct mc_config {
   int obs_mode;            /* MC_OBS_* */
   int terrain;             /* MC_TERRAIN_* (0: the reference's flat world) */
   int terrain_amplitude;   /* MC_TERRAIN_NOISE: 0..8, and ground_height - amplitude >= 2 */
+  int collision;           /* 1: voxel collision in place of the reference's ground plane at
+                              world_y / 4 (kernels.cu:1566); no counterpart in the reference */
 } mc_config;
 
 /* Device pointers of one env group's state (caller-owned, mutated in place). */

@@ -81,8 +81,16 @@ int check_image(int B, int W, int H) {
   return MC_OK;
 }
 
+int check_collision(const mc_config* c) {
+  if (c->collision != 0 && c->collision != 1)
+    return fail(MC_E_INVALID, "collision must be 0 or 1 (got %d)", c->collision);
+  if (c->collision && (c->obs_mode == MC_OBS_U8_CHW || c->obs_mode == MC_OBS_F32_CHW))
+    return fail(MC_E_UNSUPPORTED, "collision renders the interleaved obs modes only");
+  return MC_OK;
+}
+
 int check_terrain(const mc_config* c) {
-  if (c->terrain == MC_TERRAIN_FLAT) return MC_OK;
+  if (c->terrain == MC_TERRAIN_FLAT) return check_collision(c);
   if (c->terrain != MC_TERRAIN_NOISE)
     return fail(MC_E_INVALID, "terrain must be one of MC_TERRAIN_* (got %d)", c->terrain);
   if (c->terrain_amplitude < 0 || c->terrain_amplitude > 8)
@@ -92,8 +100,9 @@ int check_terrain(const mc_config* c) {
                 c->ground_height, c->terrain_amplitude);
   if (c->obs_mode == MC_OBS_U8_CHW || c->obs_mode == MC_OBS_F32_CHW)
     return fail(MC_E_UNSUPPORTED, "MC_TERRAIN_NOISE renders the interleaved obs modes only");
-  return MC_OK;
+  return check_collision(c);
 }
+
 
 int validate_config(const mc_config* c) {
   if (!c) return fail(MC_E_INVALID, "config is NULL");
@@ -156,6 +165,7 @@ mc::Params make_params(const mc_config* c, const mc_state* s) {
   p.terrain = c->terrain == MC_TERRAIN_NOISE ? 1 : 0;
   p.terrain_seed = c->seed;
   p.terrain_amp = p.terrain ? c->terrain_amplitude : 0;
+  p.collision = c->collision ? 1 : 0;
   p.tree_x = c->tree_x;
   p.tree_z = c->tree_z;
   p.house_x = c->house_x;
@@ -235,7 +245,13 @@ int launch_kind(const mc::Params& p, int obs_mode, int wpe, hipStream_t st) {
   const size_t lds = (size_t)mc::item_lds_bytes(epw, p.X, p.ys, p.Z);
   // The planar modes and the seeded terrain run kernels of their own (template
   // constants), so the flat-world interleaved kernels keep their code unchanged.
-  if (p.terrain)
+  if (KIND == mc::kItemStep && p.collision && p.terrain)
+    hipLaunchKernelGGL((mc::k_env_col<NT, LIFT, true>), dim3(grid), dim3(NT), lds, st, p,
+                       obs_mode, wpe);
+  else if (KIND == mc::kItemStep && p.collision)
+    hipLaunchKernelGGL((mc::k_env_col<NT, LIFT, false>), dim3(grid), dim3(NT), lds, st, p,
+                       obs_mode, wpe);
+  else if (p.terrain)
     hipLaunchKernelGGL((mc::k_env_terrain<NT, KIND, LIFT>), dim3(grid), dim3(NT), lds, st, p,
                        obs_mode, wpe);
   else if (planar_mode(obs_mode))
@@ -294,7 +310,21 @@ int launch_deferred(const mc::Params& q, int obs_mode, int lift, hipStream_t st)
   const int full = mc::item_lds_bytes(1, p.X, p.Y, p.Z);
   if (full > lds_budget()) return fail(MC_E_UNSUPPORTED, "world does not fit the LDS budget");
   const dim3 grid(std::min(p.B, 64));
-  if (p.terrain) {
+  if (p.collision) {
+    const bool big = full > 80 * 1024;
+    const int nt = big ? 512 : 256;
+    const int k = (big ? 4 : 0) + (lift ? 2 : 0) + (p.terrain ? 1 : 0);
+    switch (k) {
+      case 0: hipLaunchKernelGGL((mc::k_env_fb_col<256, false, false>), grid, dim3(nt), full, st, p, obs_mode); break;
+      case 1: hipLaunchKernelGGL((mc::k_env_fb_col<256, false, true>), grid, dim3(nt), full, st, p, obs_mode); break;
+      case 2: hipLaunchKernelGGL((mc::k_env_fb_col<256, true, false>), grid, dim3(nt), full, st, p, obs_mode); break;
+      case 3: hipLaunchKernelGGL((mc::k_env_fb_col<256, true, true>), grid, dim3(nt), full, st, p, obs_mode); break;
+      case 4: hipLaunchKernelGGL((mc::k_env_fb_col<512, false, false>), grid, dim3(nt), full, st, p, obs_mode); break;
+      case 5: hipLaunchKernelGGL((mc::k_env_fb_col<512, false, true>), grid, dim3(nt), full, st, p, obs_mode); break;
+      case 6: hipLaunchKernelGGL((mc::k_env_fb_col<512, true, false>), grid, dim3(nt), full, st, p, obs_mode); break;
+      default: hipLaunchKernelGGL((mc::k_env_fb_col<512, true, true>), grid, dim3(nt), full, st, p, obs_mode); break;
+    }
+  } else if (p.terrain) {
     if (full > 80 * 1024) {
       if (lift) hipLaunchKernelGGL((mc::k_env_fb_terrain<512, true>), grid, dim3(512), full, st, p, obs_mode);
       else hipLaunchKernelGGL((mc::k_env_fb_terrain<512, false>), grid, dim3(512), full, st, p, obs_mode);
@@ -335,7 +365,9 @@ int staged_layers(const mc_config& c) {
   const float apex = c.jump_vel * c.jump_vel / (-2.0f * c.gravity);
   const float eye = std::max(c.spawn_y, (float)(Y / 4) + apex) + 1.6f;
   const int top = c.ground_height + (c.terrain == MC_TERRAIN_NOISE ? c.terrain_amplitude : 0);
-  const float need = std::max((float)(top + 5), std::ceil(eye + 4.5f) + 1.0f);
+  // With collision the agent can stand on any solid: up to the house roof.
+  const float eye_c = c.collision ? std::max(eye, (float)(top + 5) + apex + 1.6f) : eye;
+  const float need = std::max((float)(top + 5), std::ceil(eye_c + 4.5f) + 1.0f);
   if (!(need < (float)Y)) return Y;
   return std::max(1, (int)need);
 }
@@ -1118,10 +1150,11 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
       if (envs[j] == envs[i]) return fail(MC_E_INVALID, "group %d listed twice", i);
     if (envs[i]->graph_mode)
       return fail(MC_E_UNSUPPORTED, "group %d is in graph mode (its slot is device-resident)", i);
-    if (planar_mode(envs[i]->cfg.obs_mode) || envs[i]->cfg.terrain != MC_TERRAIN_FLAT)
+    if (planar_mode(envs[i]->cfg.obs_mode) || envs[i]->cfg.terrain != MC_TERRAIN_FLAT ||
+        envs[i]->cfg.collision)
       return fail(MC_E_UNSUPPORTED,
-                  "group %d: the MC_OBS_*_CHW modes and MC_TERRAIN_NOISE step alone, not in a batch",
-                  i);
+                  "group %d: the MC_OBS_*_CHW modes, MC_TERRAIN_NOISE and collision step alone, "
+                  "not in a batch", i);
     if (envs[i]->shape.wpe == 8) nt = 512;
     lift = lift || envs[i]->shape.lift;
   }
@@ -1257,10 +1290,11 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
     if (!envs[i]) return fail(MC_E_INVALID, "env %d is NULL", i);
     for (int j = 0; j < i; ++j)
       if (envs[j] == envs[i]) return fail(MC_E_INVALID, "group %d listed twice", i);
-    if (planar_mode(envs[i]->cfg.obs_mode) || envs[i]->cfg.terrain != MC_TERRAIN_FLAT)
+    if (planar_mode(envs[i]->cfg.obs_mode) || envs[i]->cfg.terrain != MC_TERRAIN_FLAT ||
+        envs[i]->cfg.collision)
       return fail(MC_E_UNSUPPORTED,
-                  "group %d: the MC_OBS_*_CHW modes and MC_TERRAIN_NOISE step alone, not in a batch",
-                  i);
+                  "group %d: the MC_OBS_*_CHW modes, MC_TERRAIN_NOISE and collision step alone, "
+                  "not in a batch", i);
     const mc_config& c = envs[i]->cfg;
     const BucketKey k = bucket_of(c);
     const int j = (int)(std::find(keys.begin(), keys.end(), k) - keys.begin());

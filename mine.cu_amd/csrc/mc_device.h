@@ -163,6 +163,7 @@ struct Params {
   // Seeded terrain (the k_*_terrain kernels only): column heights
   // ground_height + d, d in [-terrain_amp, terrain_amp], from terrain_seed.
   int terrain;                 // host side: 1 launches the k_*_terrain kernels
+  int collision;               // host side: 1 launches the k_*_col step kernels
   uint32_t terrain_seed;
   int terrain_amp;
 };
@@ -1240,6 +1241,7 @@ __device__ __forceinline__ EnvIn load_env_in(const Params& p, int b) {
   return in;
 }
 
+template <bool COL = false>
 __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, int8_t* world,
                                             int8_t* gworld, bool any_done, const EnvIn& in,
                                             bool upper_solid) {
@@ -1276,6 +1278,52 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
   float nx = in.px + move_x;
   float ny = in.py + vel_y * p.dt;
   float nz = in.pz + move_z;
+  if (COL) {
+    // Voxel collision (MineEnv(collision=True); no counterpart in the
+    // reference, whose ground is the plane y = Y/4).  Solid = a block id >= 0
+    // inside the world (layers above the staged ones count as air: an env
+    // that could touch them is deferred to the full-height pass).
+    //  1. x, z clamp to [1, dim - 1] first;
+    //  2. the horizontal move is undone when the target column is solid at the
+    //     old feet layer floor(py) or the one above;
+    //  3. rising into a solid block at floor(ny + 1.8) stops: ny = py, vy = 0;
+    //  4. the ground is the top of the highest solid voxel of the column at
+    //     or below floor(py) (0 when there is none): ny <= ground lands there.
+    nx = fminf(fmaxf(nx, 1.0f), (float)p.X - 1.0f);
+    nz = fminf(fmaxf(nz, 1.0f), (float)p.Z - 1.0f);
+    const int ylim = p.ys < p.Y ? p.ys : p.Y;
+    auto solid = [&](int x, int y, int z) -> bool {
+      if (!(((unsigned)x < (unsigned)p.X) & ((unsigned)y < (unsigned)ylim) &
+            ((unsigned)z < (unsigned)p.Z)))
+        return false;
+      return world[(y * p.X + x) * p.Z + z] >= 0;
+    };
+    const int fy = (int)floorf(in.py);
+    int cx = (int)floorf(nx), cz = (int)floorf(nz);
+    if (solid(cx, fy, cz) || solid(cx, fy + 1, cz)) {
+      nx = in.px;
+      nz = in.pz;
+      cx = (int)floorf(nx);
+      cz = (int)floorf(nz);
+    }
+    if (vel_y > 0.0f && solid(cx, (int)floorf(ny + 1.8f), cz)) {
+      ny = in.py;
+      vel_y = 0.0f;
+    }
+    int g = 0;
+    for (int y = min(fy, ylim - 1); y >= 0; --y)
+      if (solid(cx, y, cz)) {
+        g = y + 1;
+        break;
+      }
+    if (ny <= (float)g) {
+      ny = (float)g;
+      vel_y = 0.0f;
+      og = true;
+    } else {
+      og = false;
+    }
+  } else {
   const float ground_y = (float)(p.Y / 4);
   if (ny <= ground_y) {
     ny = ground_y;
@@ -1286,6 +1334,7 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
   }
   nx = fminf(fmaxf(nx, 1.0f), (float)p.X - 1.0f);
   nz = fminf(fmaxf(nz, 1.0f), (float)p.Z - 1.0f);
+  }
 
   // Staged height (Params::ys): the env runs here only when its layers
   // y >= ys are air, the raycast's samples (at most 3.975 from the eye) stay
@@ -1529,7 +1578,7 @@ __host__ __device__ __forceinline__ int item_lds_bytes(int epw, int X, int Y, in
 }
 
 // upper: one word per wave of the workgroup (staged-height check).
-template <int NT, int KIND, bool LIFT, bool CHW = false, bool TER = false>
+template <int NT, int KIND, bool LIFT, bool CHW = false, bool TER = false, bool COL = false>
 __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, int wpe,
                                          int8_t* lds, const float* pal, const uint32_t* flat,
                                          Frame* frames, float* su, float* sv, int* upper) {
@@ -1581,7 +1630,7 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
       for (int k = 0; k < wpe; ++k) upper_solid |= upper[e * wpe + k] != 0;
     Frame f;
     if (KIND == kItemStep)
-      f = env_update(p, slot, b, world, gworld, any_done, in, upper_solid);
+      f = env_update<COL>(p, slot, b, world, gworld, any_done, in, upper_solid);
     else if (KIND == kItemResetRender) f = reset_update(p, slot, b, world);
     else if (KIND == kItemRenderBasis) f = frame_from_basis(p, b, world);
     else f = camera_update(p, b, world, KIND == kItemRenderState);
@@ -1722,6 +1771,50 @@ __global__ __launch_bounds__(NT) void k_env_fb_terrain(Params p, int obs_mode) {
     __syncthreads();
     run_item<NT, kItemStep, LIFT, false, true>(p, obs_mode, (int)list[2 + i], NT / kWave, s_lds,
                                                s_pal, s_flat, s_frames, s_u, s_v, s_upper);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&list[1], 1u) == gridDim.x - 1) {
+      list[0] = 0u;
+      list[1] = 0u;
+      __threadfence();
+    }
+  }
+}
+
+// Step kernels with voxel collision (flat or seeded terrain world, TER).
+template <int NT, bool LIFT, bool TER>
+__global__ __launch_bounds__(NT) void k_env_col(Params p, int obs_mode, int wpe) {
+  extern __shared__ __align__(16) int8_t s_lds[];
+  __shared__ float s_pal[kPaletteLen];
+  __shared__ uint32_t s_flat[kFlatLen];
+  __shared__ Frame s_frames[NT / kWave];
+  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
+  __shared__ int s_upper[NT / kWave];
+  load_palette(s_pal, s_flat, threadIdx.x);
+  const int epw = (NT / kWave) / wpe;
+  run_item<NT, kItemStep, LIFT, false, TER, true>(p, obs_mode, blockIdx.x * epw, wpe, s_lds,
+                                                  s_pal, s_flat, s_frames, s_u, s_v, s_upper);
+}
+
+template <int NT, bool LIFT, bool TER>
+__global__ __launch_bounds__(NT) void k_env_fb_col(Params p, int obs_mode) {
+  extern __shared__ __align__(16) int8_t s_lds[];
+  __shared__ float s_pal[kPaletteLen];
+  __shared__ uint32_t s_flat[kFlatLen];
+  __shared__ Frame s_frames[NT / kWave];
+  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
+  __shared__ int s_upper[NT / kWave];
+  const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
+  uint32_t* list = p.fb + (size_t)slot * (p.B + 2);
+  const int n = (int)min(list[0], (uint32_t)p.B);
+  if (n == 0) return;
+  load_palette(s_pal, s_flat, threadIdx.x);
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    __syncthreads();
+    run_item<NT, kItemStep, LIFT, false, TER, true>(p, obs_mode, (int)list[2 + i], NT / kWave,
+                                                    s_lds, s_pal, s_flat, s_frames, s_u, s_v,
+                                                    s_upper);
   }
   if (threadIdx.x == 0) {
     __threadfence();

@@ -149,6 +149,7 @@ class MineEnv:
         obs_layout: str = "hwc",
         terrain: str = "flat",
         terrain_amplitude: int = 3,
+        collision: bool = False,
     ):
         assert "cuda" in str(device), "MineEnv requires CUDA device"
         assert obs_dtype in _REFERENCE_OBS, \
@@ -169,6 +170,11 @@ class MineEnv:
         if terrain == "noise" and obs_layout == "chw":
             raise ValueError("terrain 'noise' renders the 'hwc' layout only")
         self.terrain, self.terrain_amplitude = terrain, int(terrain_amplitude)
+        # collision=True: the agent stands on, and is blocked by, solid voxels
+        # instead of the reference's ground plane at world_y / 4.
+        if collision and obs_layout == "chw":
+            raise ValueError("collision renders the 'hwc' layout only")
+        self.collision = bool(collision)
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -265,6 +271,7 @@ class MineEnv:
             c.obs_mode = _lib.MC_OBS_U8_CHW if self.obs_dtype == "uint8" else _lib.MC_OBS_F32_CHW
         c.terrain = _lib.MC_TERRAIN_NOISE if self.terrain == "noise" else _lib.MC_TERRAIN_FLAT
         c.terrain_amplitude = self.terrain_amplitude
+        c.collision = 1 if self.collision else 0
         s = _lib.McState()
         for name, t in (("voxels", self.voxels), ("positions", self.positions),
                         ("velocities", self.velocities), ("yaws", self.yaws),

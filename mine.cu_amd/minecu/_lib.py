@@ -38,7 +38,7 @@ class McConfig(ctypes.Structure):
         ("ground_height", _I), ("seed", _U),
         ("tree_x", _I), ("tree_z", _I), ("house_x", _I), ("house_z", _I),
         ("episode_length", _I), ("look_delta", _F), ("obs_mode", _I),
-        ("terrain", _I), ("terrain_amplitude", _I),
+        ("terrain", _I), ("terrain_amplitude", _I), ("collision", _I),
     ]
 
 

@@ -67,6 +67,8 @@ def lib():
         L.orc_discrete_to_buttons.argtypes = [P, i, f, P, P]
         L.orc_decode_actions.argtypes = [P, P, P, P, P, P, P, P, P, P, i, f]
         L.orc_physics.argtypes = [P, P, P, P, P, P, P, P, P, P, P, i, i, i, i, f, f, f, f]
+        L.orc_physics_collide.argtypes = [P, P, P, P, P, P, P, P, P, P, P, P, i, i, i, i, f, f,
+                                          f, f]
         L.orc_raycast_interact.argtypes = [P, P, P, P, P, P, P, i, i, i, i, i, f]
         L.orc_update_cameras.argtypes = [P, P, P, P, i]
         L.orc_render.argtypes = [P, P, P, i, i, i, i, i, i, i, i, f, f, P, i]
@@ -224,7 +226,7 @@ class OracleEnv:
                  fov_degrees=70.0, dt=0.05, gravity=-20.0, walk_speed=4.0, sprint_mult=1.5,
                  jump_vel=8.0, ground_height=8, target_block=OAKLOG, reward_value=1.0,
                  episode_length=None, seed=42, obs_dtype="uint8", look_delta=0.1,
-                 nthreads=None, terrain="flat", terrain_amplitude=3):
+                 nthreads=None, terrain="flat", terrain_amplitude=3, collision=False):
         assert obs_dtype in _FLAT, obs_dtype
         self.batch_size = B = batch_size
         self.world_x = world_x if world_x is not None else world_size
@@ -238,6 +240,7 @@ class OracleEnv:
         self.episode_length, self.seed, self.obs_dtype = episode_length, seed, obs_dtype
         self.look_delta = look_delta
         self.terrain_amplitude = terrain_amplitude if terrain == "noise" else None
+        self.collision = bool(collision)
         self.nthreads = nthreads or default_threads()
         # Spawn / tree / house (minecu/__init__.py:196-206).
         self.spawn_x = self.world_x / 2.0
@@ -321,9 +324,16 @@ class OracleEnv:
         L.orc_decode_actions(_p(buttons), _p(look), _p(fwd), _p(strafe), _p(dyaw), _p(dpitch),
                              _p(jump), _p(brk), _p(place), _p(speed), B, self.sprint_mult)
         X, Y, Z = self.world_x, self.world_y, self.world_z
-        L.orc_physics(_p(self.positions), _p(self.velocities), _p(self.yaws), _p(self.pitches),
-                      _p(self.on_ground), _p(fwd), _p(strafe), _p(dyaw), _p(dpitch), _p(jump),
-                      _p(speed), B, X, Y, Z, self.dt, self.gravity, self.walk_speed, self.jump_vel)
+        if self.collision:
+            L.orc_physics_collide(_p(self.positions), _p(self.velocities), _p(self.yaws),
+                                  _p(self.pitches), _p(self.on_ground), _p(fwd), _p(strafe),
+                                  _p(dyaw), _p(dpitch), _p(jump), _p(speed), _p(self.voxels), B,
+                                  X, Y, Z, self.dt, self.gravity, self.walk_speed, self.jump_vel)
+        else:
+            L.orc_physics(_p(self.positions), _p(self.velocities), _p(self.yaws),
+                          _p(self.pitches), _p(self.on_ground), _p(fwd), _p(strafe), _p(dyaw),
+                          _p(dpitch), _p(jump), _p(speed), B, X, Y, Z, self.dt, self.gravity,
+                          self.walk_speed, self.jump_vel)
         L.orc_raycast_interact(_p(self.voxels), _p(self.positions), _p(self.yaws),
                                _p(self.pitches), _p(brk), _p(place), _p(self.rewards), B, X, Y, Z,
                                self.target_block, self.reward_value)

@@ -533,6 +533,75 @@ void orc_physics(float* positions, float* velocities, float* yaws, float* pitche
   }
 }
 
+// Voxel collision (the library's MineEnv(collision=True); the reference's
+// only ground is the plane y = world_y / 4, src/kernels.cu:1566).  Same
+// decode and integration as orc_physics, then, on env i's world as it is
+// before this step's break/place:
+//   clamp x, z to [1, dim - 1];
+//   a move into a column with a solid block at layer floor(y) or floor(y) + 1
+//   (y: the position before the step) is undone in x and z together;
+//   rising with a solid block at layer floor(new_y + 1.8) keeps y, vy = 0;
+//   the ground is 1 + the highest solid layer <= floor(y) of the column (0
+//   when none); new_y <= ground lands on it.  Solid: a block id >= 0 inside
+//   the world.
+void orc_physics_collide(float* positions, float* velocities, float* yaws, float* pitches,
+                         uint8_t* on_ground, const float* forward_in, const float* strafe_in,
+                         const float* delta_yaw_in, const float* delta_pitch_in,
+                         const uint8_t* jump_in, const float* speed_mult, const int8_t* voxels,
+                         int B, int world_x, int world_y, int world_z, float dt, float gravity,
+                         float walk_speed, float jump_vel) {
+  const int64_t wxyz = (int64_t)world_x * world_y * world_z;
+  for (int i = 0; i < B; ++i) {
+    const int8_t* w = voxels + i * wxyz;
+    auto is_solid = [&](int x, int y, int z) {
+      if (x < 0 || x >= world_x || y < 0 || y >= world_y || z < 0 || z >= world_z) return false;
+      return w[((int64_t)y * world_x + x) * world_z + z] >= 0;
+    };
+    yaws[i] += delta_yaw_in[i];
+    float new_pitch = pitches[i] + delta_pitch_in[i];
+    pitches[i] = fminf(fmaxf(new_pitch, -1.56f), 1.56f);
+    float sin_yaw, cos_yaw;
+    mc_sincosf(yaws[i], &sin_yaw, &cos_yaw);
+    const float fwd = forward_in[i], str = strafe_in[i];
+    const float speed = walk_speed * speed_mult[i];
+    const float move_x = sin_yaw * fwd * speed * dt + cos_yaw * str * walk_speed * dt;
+    const float move_z = cos_yaw * fwd * speed * dt - sin_yaw * str * walk_speed * dt;
+    float vel_y = velocities[i * 3 + 1];
+    if (jump_in[i] && on_ground[i]) vel_y = jump_vel;
+    vel_y += gravity * dt;
+    const float old_x = positions[i * 3 + 0], old_y = positions[i * 3 + 1],
+                old_z = positions[i * 3 + 2];
+    float new_x = fminf(fmaxf(old_x + move_x, 1.0f), (float)world_x - 1.0f);
+    float new_y = old_y + vel_y * dt;
+    float new_z = fminf(fmaxf(old_z + move_z, 1.0f), (float)world_z - 1.0f);
+    const int feet = (int)std::floor(old_y);
+    if (is_solid((int)std::floor(new_x), feet, (int)std::floor(new_z)) ||
+        is_solid((int)std::floor(new_x), feet + 1, (int)std::floor(new_z))) {
+      new_x = old_x;
+      new_z = old_z;
+    }
+    const int col_x = (int)std::floor(new_x), col_z = (int)std::floor(new_z);
+    if (vel_y > 0.0f && is_solid(col_x, (int)std::floor(new_y + 1.8f), col_z)) {
+      new_y = old_y;
+      vel_y = 0.0f;
+    }
+    int ground = 0;
+    for (int y = std::min(feet, world_y - 1); y >= 0 && ground == 0; --y)
+      if (is_solid(col_x, y, col_z)) ground = y + 1;
+    if (new_y <= (float)ground) {
+      new_y = (float)ground;
+      vel_y = 0.0f;
+      on_ground[i] = 1;
+    } else {
+      on_ground[i] = 0;
+    }
+    positions[i * 3 + 0] = new_x;
+    positions[i * 3 + 1] = new_y;
+    positions[i * 3 + 2] = new_z;
+    velocities[i * 3 + 1] = vel_y;
+  }
+}
+
 // raycast_interact_kernel (src/kernels.cu:1591-1680): 32 fixed samples along
 // the view ray; first solid in-bounds voxel is broken (reward if it matches the
 // target) or, for place, DIRT goes into the previous in-bounds air sample.

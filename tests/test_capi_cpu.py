@@ -29,9 +29,9 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
     _l, _ = _lib()
-    # 30 four-byte fields (terrain and terrain_amplitude last), 12 pointers
-    assert ctypes.sizeof(_l.McConfig) == 30 * 4
-    assert [f[0] for f in _l.McConfig._fields_][-2:] == ["terrain", "terrain_amplitude"]
+    # 31 four-byte fields (terrain, terrain_amplitude, collision last), 12 pointers
+    assert ctypes.sizeof(_l.McConfig) == 31 * 4
+    assert [f[0] for f in _l.McConfig._fields_][-3:] == ["terrain", "terrain_amplitude", "collision"]
     assert ctypes.sizeof(_l.McState) == 12 * 8
     text = open(HEADER).read()
     state = text[text.index("typedef struct mc_state"):text.index("} mc_state;")]

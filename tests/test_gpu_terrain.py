@@ -1,4 +1,5 @@
-"""Seeded terrain (MineEnv(terrain="noise")) on the GPU: worlds, positions,
+"""Seeded terrain (MineEnv(terrain="noise")) and voxel collision
+(collision=True) on the GPU: worlds, positions,
 rewards, dones and images equal the oracle's bit for bit through reset,
 steps, done steps with regeneration, the staged-height fast path and the
 full-height pass; amplitude 0 steps exactly like the flat world; batches and
@@ -86,3 +87,39 @@ def test_seeds_give_different_worlds_and_batches_refuse_terrain():
         minecu.MineEnv(device="cuda", terrain="noise", obs_layout="chw", **kw)
     with pytest.raises(_lib.MineCuError):
         minecu.MineEnv(device="cuda", terrain="noise", terrain_amplitude=9, **kw)
+
+
+def _mixed_buttons(T, B, seed):
+    rng = np.random.default_rng(seed)
+    b = (rng.random((T, B, 8)) < np.array([0.6, 0.1, 0.15, 0.15, 0.3, 0.2, 0.1, 0.3])).astype(np.int8)
+    look = rng.normal(0.0, 0.3, (T, B, 2)).astype(np.float32)
+    return b, look
+
+
+@pytest.mark.parametrize("ws,terrain,amp,seed", [
+    (16, "flat", 3, 5),
+    (16, "noise", 3, 42),
+    (32, "noise", 5, 8),
+])
+def test_collision_matches_oracle(ws, terrain, amp, seed):
+    import minecu
+    B = 48
+    kw = dict(batch_size=B, world_size=ws, render_width=24, render_height=16, obs_dtype="uint8",
+              episode_length=20, seed=seed)
+    env = minecu.MineEnv(device="cuda", terrain=terrain, terrain_amplitude=amp, collision=True,
+                         **kw)
+    orc = oracle.OracleEnv(terrain=terrain, terrain_amplitude=amp, collision=True, **kw)
+    env.reset()
+    orc.reset()
+    bts, looks = _mixed_buttons(45, B, seed)
+    st = Stats()
+    for t in range(45):
+        _, r, d = env.step(torch.as_tensor(bts[t]).cuda(), torch.as_tensor(looks[t]).cuda())
+        _, ro, do = orc.step(bts[t], looks[t])
+        assert np.array_equal(r.cpu().numpy(), ro) and np.array_equal(d.cpu().numpy(), do), t
+        if t % 5 == 4:
+            compare_env(env, orc, st, f"step {t}")
+    assert st.exact, st.summary()
+    if terrain == "flat":
+        # the agents stand on the grass (y = 8), not on the plane y = Y / 4
+        assert float(env.positions[:, 1].min()) > ws // 4
