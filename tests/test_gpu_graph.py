@@ -18,12 +18,15 @@ def _same(a, b, where):
         assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), f"{where}: {k}"
 
 
-@pytest.mark.parametrize("obs_dtype,ws", [("uint8", 16), ("uint8_prebasis", 32)])
-def test_graph_replays_equal_eager_steps(obs_dtype, ws):
+@pytest.mark.parametrize("obs_dtype,ws,extra", [
+    ("uint8", 16, {}), ("uint8_prebasis", 32, {}),
+    ("uint8", 16, {"terrain": "noise", "collision": True}), ("float32", 32, {"terrain": "noise"}),
+])
+def test_graph_replays_equal_eager_steps(obs_dtype, ws, extra):
     import minecu
     B, T = 64, 26
     kw = dict(batch_size=B, world_size=ws, render_width=32, render_height=32,
-              obs_dtype=obs_dtype, episode_length=5, device="cuda")
+              obs_dtype=obs_dtype, episode_length=5, device="cuda", **extra)
     eager, graphed = minecu.MineEnv(**kw), minecu.MineEnv(**kw)
     eager.reset()
     graphed.reset()

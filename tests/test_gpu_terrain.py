@@ -26,6 +26,8 @@ def _acts(T, B, seed):
     (32, 32, 24, "uint8", 5, 7),
     (16, 24, 16, "float32", 2, 2 ** 31 + 11),
     (8, 16, 16, "uint8", 4, 3),
+    (16, 32, 32, "uint8_minimal", 3, 11),
+    (16, 32, 16, "uint8_prebasis", 2, 12),
 ])
 def test_terrain_matches_oracle(ws, W, H, dtype, amp, seed):
     import minecu
@@ -96,15 +98,17 @@ def _mixed_buttons(T, B, seed):
     return b, look
 
 
-@pytest.mark.parametrize("ws,terrain,amp,seed", [
-    (16, "flat", 3, 5),
-    (16, "noise", 3, 42),
-    (32, "noise", 5, 8),
+@pytest.mark.parametrize("ws,terrain,amp,seed,dtype", [
+    (16, "flat", 3, 5, "uint8"),
+    (16, "noise", 3, 42, "uint8"),
+    (32, "noise", 5, 8, "uint8"),
+    (16, "noise", 2, 13, "float32"),
+    (32, "flat", 3, 14, "uint8_minimal"),
 ])
-def test_collision_matches_oracle(ws, terrain, amp, seed):
+def test_collision_matches_oracle(ws, terrain, amp, seed, dtype):
     import minecu
     B = 48
-    kw = dict(batch_size=B, world_size=ws, render_width=24, render_height=16, obs_dtype="uint8",
+    kw = dict(batch_size=B, world_size=ws, render_width=24, render_height=16, obs_dtype=dtype,
               episode_length=20, seed=seed)
     env = minecu.MineEnv(device="cuda", terrain=terrain, terrain_amplitude=amp, collision=True,
                          **kw)
