@@ -108,3 +108,27 @@ def test_config_presets_match_reference():
     assert (FAST_ENV.batch_size, FAST_ENV.render_width, FAST_ENV.max_steps) == (32768, 32, 32)
     assert (QUALITY_ENV.render_height, QUALITY_ENV.view_distance) == (128, 64.0)
     assert DEFAULT_ENV.memory_bytes() == EnvConfig().memory_bytes()
+
+
+def test_terrain_and_collision_settings_are_validated_before_any_launch():
+    _l, L = _lib()
+    v = ctypes.c_void_p(16)  # never dereferenced: every case fails validation first
+
+    def cfg(**kw):
+        c = _l.McConfig()
+        c.batch_size, c.world_x, c.world_y, c.world_z = 2, 16, 16, 16
+        c.ground_height = 8
+        for k, val in kw.items():
+            setattr(c, k, val)
+        return c
+
+    for kw, code in [(dict(terrain=2), _l.MC_E_INVALID),
+                     (dict(terrain=1, terrain_amplitude=9), _l.MC_E_INVALID),
+                     (dict(terrain=1, terrain_amplitude=-1), _l.MC_E_INVALID),
+                     (dict(terrain=1, terrain_amplitude=7), _l.MC_E_INVALID),  # 8 - 7 < 2
+                     (dict(terrain=1, terrain_amplitude=3, obs_mode=5), _l.MC_E_UNSUPPORTED),
+                     (dict(collision=2), _l.MC_E_INVALID),
+                     (dict(collision=1, obs_mode=6), _l.MC_E_UNSUPPORTED)]:
+        c = cfg(**kw)
+        assert L.mc_regen_worlds(ctypes.byref(c), v, None, None) == code, kw
+        assert _err(L), kw
