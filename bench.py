@@ -376,6 +376,10 @@ def main():
     ap.add_argument("--obs", default="uint8", choices=["uint8", "float32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32", action="store_true", help="skip the float32-obs side measurement")
+    ap.add_argument("--terrain", default="flat", choices=["flat", "noise"],
+                    help="world of the main measurement (default: the reference's flat world)")
+    ap.add_argument("--collision", action="store_true",
+                    help="voxel collision in the main measurement (default: the reference's plane)")
     ap.add_argument("--no-terrain", action="store_true",
                     help="skip the noise-terrain (terrain='noise', with and without collision) "
                          "side measurements")
@@ -404,8 +408,11 @@ def main():
         return
     w = WORKLOADS[args.workload]
     obs_bytes = 1 if args.obs == "uint8" else 4
-    total_ms, wall, per_launch = run_gpu(w, args.obs, args.steps, args.warmup)
+    total_ms, wall, per_launch = run_gpu(w, args.obs, args.steps, args.warmup,
+                                         terrain=args.terrain, collision=args.collision)
     K, B = args.steps, w["batch"]
+    # The reference's configuration: the flat world and its ground plane.
+    ref_world = args.terrain == "flat" and not args.collision
     value = B * K / (total_ms / 1e3)
     per_launch.sort()
     avg_launch_ms = sum(per_launch) / len(per_launch)
@@ -425,18 +432,19 @@ def main():
         # 49.63M env-steps/s at B=32768, 16^3, 32x32 (docs/benchmarks.md:34),
         # measured with float32 observations; float32_obs_value below is the
         # like-for-like figure.
-        "vs_baseline": value / REFERENCE_C2 if args.workload == "c2" else None,
+        "vs_baseline": value / REFERENCE_C2 if args.workload == "c2" and ref_world else None,
         "dtype": "f32",
         "data": "synthetic (deterministic flat world + tree + house, seeded random actions)",
         "config": {"workload": describe(args.workload, w, args.obs), "batch": B,
                    "world": [w["world"]] * 3, "obs": [w["H"], w["W"], 3], "obs_dtype": args.obs,
-                   "episode_length": w["episode_length"], "max_steps": 64, "look_delta": 0.1},
+                   "episode_length": w["episode_length"], "max_steps": 64, "look_delta": 0.1,
+                   "terrain": args.terrain, "collision": bool(args.collision)},
         "wall_s": wall,
         "step_launch_ms": {"avg": avg_launch_ms, "p50": per_launch[len(per_launch) // 2],
                            "p99": per_launch[min(len(per_launch) - 1, int(0.99 * len(per_launch)))]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, args.obs),
+                     "traffic": measured_traffic(args.workload, args.obs) if ref_world else None,
                      "kernel": "k_step (fused step, one launch per env.step)",
                      "alg_bytes_per_env_step": alg},
         "reference_published": {"value": REFERENCE_C2, "hw": "RTX 3090", "obs": "float32",
