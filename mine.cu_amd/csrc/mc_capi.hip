@@ -15,7 +15,8 @@
 #include <tuple>
 #include <vector>
 
-#include "mc_device.h"
+#include "mc_launch.h"
+#include "mc_util_kernels.h"
 #include "minecu_hip.h"
 
 namespace {
@@ -238,49 +239,6 @@ bool planar_mode(int obs_mode) {
   return obs_mode == MC_OBS_U8_CHW || obs_mode == MC_OBS_F32_CHW;
 }
 
-template <int NT, int KIND, bool LIFT>
-int launch_kind(const mc::Params& p, int obs_mode, int wpe, hipStream_t st) {
-  const int epw = (NT / mc::kWave) / wpe;
-  const int grid = (p.B + epw - 1) / epw;
-  const size_t lds = (size_t)mc::item_lds_bytes(epw, p.X, p.ys, p.Z);
-  // The planar modes and the seeded terrain run kernels of their own (template
-  // constants), so the flat-world interleaved kernels keep their code unchanged.
-  if (KIND == mc::kItemStep && p.collision && p.terrain)
-    hipLaunchKernelGGL((mc::k_env_col<NT, LIFT, true>), dim3(grid), dim3(NT), lds, st, p,
-                       obs_mode, wpe);
-  else if (KIND == mc::kItemStep && p.collision)
-    hipLaunchKernelGGL((mc::k_env_col<NT, LIFT, false>), dim3(grid), dim3(NT), lds, st, p,
-                       obs_mode, wpe);
-  else if (p.terrain)
-    hipLaunchKernelGGL((mc::k_env_terrain<NT, KIND, LIFT>), dim3(grid), dim3(NT), lds, st, p,
-                       obs_mode, wpe);
-  else if (planar_mode(obs_mode))
-    hipLaunchKernelGGL((mc::k_env<NT, KIND, LIFT, true>), dim3(grid), dim3(NT), lds, st, p,
-                       obs_mode, wpe);
-  else
-    hipLaunchKernelGGL((mc::k_env<NT, KIND, LIFT>), dim3(grid), dim3(NT), lds, st, p, obs_mode,
-                       wpe);
-  return check_launch("env kernel launch");
-}
-
-template <int NT, bool LIFT>
-int launch_nt(int kind, const mc::Params& p, int obs_mode, int wpe, hipStream_t st) {
-  // Planar images come from MineEnv's own step, reset and render only.
-  if (planar_mode(obs_mode) && kind != mc::kItemStep && kind != mc::kItemResetRender &&
-      kind != mc::kItemRenderState)
-    return fail(MC_E_UNSUPPORTED, "the MC_OBS_*_CHW modes render from MineEnv state only");
-  switch (kind) {
-    case mc::kItemStep: return launch_kind<NT, mc::kItemStep, LIFT>(p, obs_mode, wpe, st);
-    case mc::kItemResetRender:
-      return launch_kind<NT, mc::kItemResetRender, LIFT>(p, obs_mode, wpe, st);
-    case mc::kItemRenderCameras:
-      return launch_kind<NT, mc::kItemRenderCameras, LIFT>(p, obs_mode, wpe, st);
-    case mc::kItemRenderBasis:
-      return launch_kind<NT, mc::kItemRenderBasis, LIFT>(p, obs_mode, wpe, st);
-    default: return launch_kind<NT, mc::kItemRenderState, LIFT>(p, obs_mode, wpe, st);
-  }
-}
-
 // Staged layers below Y (Params::ys) apply to step items with a deferred-env
 // list only; every other launch stages whole worlds.
 int launch_env(int kind, const mc::Params& q, int obs_mode, Shape s, hipStream_t st) {
@@ -292,14 +250,14 @@ int launch_env(int kind, const mc::Params& q, int obs_mode, Shape s, hipStream_t
   s = fit_lds(s, p.X, p.ys, p.Z);
   if (mc::item_lds_bytes(epw_of(s), p.X, p.ys, p.Z) > lds_budget())
     return fail(MC_E_UNSUPPORTED, "world does not fit the LDS budget");
-  switch (s.nt * 2 + s.lift) {
-    case 128: return launch_nt<64, false>(kind, p, obs_mode, s.wpe, st);
-    case 129: return launch_nt<64, true>(kind, p, obs_mode, s.wpe, st);
-    case 512: return launch_nt<256, false>(kind, p, obs_mode, s.wpe, st);
-    case 513: return launch_nt<256, true>(kind, p, obs_mode, s.wpe, st);
-    case 1024: return launch_nt<512, false>(kind, p, obs_mode, s.wpe, st);
-    default: return launch_nt<512, true>(kind, p, obs_mode, s.wpe, st);
-  }
+  // Planar images come from MineEnv's own step, reset and render only.
+  if (planar_mode(obs_mode) && kind != mc::kItemStep && kind != mc::kItemResetRender &&
+      kind != mc::kItemRenderState)
+    return fail(MC_E_UNSUPPORTED, "the MC_OBS_*_CHW modes render from MineEnv state only");
+  const size_t lds = (size_t)mc::item_lds_bytes(epw_of(s), p.X, p.ys, p.Z);
+  const hipError_t e = mcl::kernels(s.nt, s.lift != 0).env(kind, p, obs_mode, s.wpe, lds, st);
+  if (e != hipSuccess) return fail(MC_E_HIP, "env kernel launch: %s", hipGetErrorString(e));
+  return MC_OK;
 }
 
 // Full-height pass over the envs a staged-height step deferred (p.fb, the
@@ -309,45 +267,11 @@ int launch_deferred(const mc::Params& q, int obs_mode, int lift, hipStream_t st)
   p.ys = p.Y;
   const int full = mc::item_lds_bytes(1, p.X, p.Y, p.Z);
   if (full > lds_budget()) return fail(MC_E_UNSUPPORTED, "world does not fit the LDS budget");
-  const dim3 grid(std::min(p.B, 64));
-  if (p.collision) {
-    const bool big = full > 80 * 1024;
-    const int nt = big ? 512 : 256;
-    const int k = (big ? 4 : 0) + (lift ? 2 : 0) + (p.terrain ? 1 : 0);
-    switch (k) {
-      case 0: hipLaunchKernelGGL((mc::k_env_fb_col<256, false, false>), grid, dim3(nt), full, st, p, obs_mode); break;
-      case 1: hipLaunchKernelGGL((mc::k_env_fb_col<256, false, true>), grid, dim3(nt), full, st, p, obs_mode); break;
-      case 2: hipLaunchKernelGGL((mc::k_env_fb_col<256, true, false>), grid, dim3(nt), full, st, p, obs_mode); break;
-      case 3: hipLaunchKernelGGL((mc::k_env_fb_col<256, true, true>), grid, dim3(nt), full, st, p, obs_mode); break;
-      case 4: hipLaunchKernelGGL((mc::k_env_fb_col<512, false, false>), grid, dim3(nt), full, st, p, obs_mode); break;
-      case 5: hipLaunchKernelGGL((mc::k_env_fb_col<512, false, true>), grid, dim3(nt), full, st, p, obs_mode); break;
-      case 6: hipLaunchKernelGGL((mc::k_env_fb_col<512, true, false>), grid, dim3(nt), full, st, p, obs_mode); break;
-      default: hipLaunchKernelGGL((mc::k_env_fb_col<512, true, true>), grid, dim3(nt), full, st, p, obs_mode); break;
-    }
-  } else if (p.terrain) {
-    if (full > 80 * 1024) {
-      if (lift) hipLaunchKernelGGL((mc::k_env_fb_terrain<512, true>), grid, dim3(512), full, st, p, obs_mode);
-      else hipLaunchKernelGGL((mc::k_env_fb_terrain<512, false>), grid, dim3(512), full, st, p, obs_mode);
-    } else {
-      if (lift) hipLaunchKernelGGL((mc::k_env_fb_terrain<256, true>), grid, dim3(256), full, st, p, obs_mode);
-      else hipLaunchKernelGGL((mc::k_env_fb_terrain<256, false>), grid, dim3(256), full, st, p, obs_mode);
-    }
-  } else if (planar_mode(obs_mode)) {
-    if (full > 80 * 1024) {
-      if (lift) hipLaunchKernelGGL((mc::k_env_fb<512, true, true>), grid, dim3(512), full, st, p, obs_mode);
-      else hipLaunchKernelGGL((mc::k_env_fb<512, false, true>), grid, dim3(512), full, st, p, obs_mode);
-    } else {
-      if (lift) hipLaunchKernelGGL((mc::k_env_fb<256, true, true>), grid, dim3(256), full, st, p, obs_mode);
-      else hipLaunchKernelGGL((mc::k_env_fb<256, false, true>), grid, dim3(256), full, st, p, obs_mode);
-    }
-  } else if (full > 80 * 1024) {
-    if (lift) hipLaunchKernelGGL((mc::k_env_fb<512, true>), grid, dim3(512), full, st, p, obs_mode);
-    else hipLaunchKernelGGL((mc::k_env_fb<512, false>), grid, dim3(512), full, st, p, obs_mode);
-  } else {
-    if (lift) hipLaunchKernelGGL((mc::k_env_fb<256, true>), grid, dim3(256), full, st, p, obs_mode);
-    else hipLaunchKernelGGL((mc::k_env_fb<256, false>), grid, dim3(256), full, st, p, obs_mode);
-  }
-  return check_launch("deferred-env launch");
+  const int nt = full > 80 * 1024 ? 512 : 256;
+  const hipError_t e = mcl::kernels(nt, lift != 0).fb(p, obs_mode, dim3(std::min(p.B, 64)),
+                                                      (size_t)full, st);
+  if (e != hipSuccess) return fail(MC_E_HIP, "deferred-env launch: %s", hipGetErrorString(e));
+  return MC_OK;
 }
 
 // Layers a MineEnv step stages in LDS: the fresh world's solids end at
@@ -1059,28 +983,21 @@ struct mc_batch {
 };
 
 namespace {
-template <int NT, bool LIFT>
-int occupancy_blocks(mc_batch* b, int lds) {
+int occupancy_blocks(mc_batch* b, int nt, bool lift, int lds) {
   for (const auto& o : b->occ)
-    if (o.nt == NT && o.lift == (int)LIFT && o.lds == lds) return o.blocks;
-  int nb = 0;
-  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &nb, mc::k_multi<NT, LIFT, true>, NT, (size_t)lds);
-  if (e != hipSuccess || nb < 1) nb = 1;
-  b->occ.push_back({NT, (int)LIFT, lds, nb});
+    if (o.nt == nt && o.lift == (int)lift && o.lds == lds) return o.blocks;
+  const int nb = mcl::kernels(nt, lift).multi_blocks((size_t)lds);
+  b->occ.push_back({nt, (int)lift, lds, nb});
   return nb;
 }
 
-template <int NT, bool LIFT>
-int launch_multi(mc_batch* b, const mc::MultiArgs& a, int lds, hipStream_t st) {
-  if (b->persistent) {
-    const int grid = std::max(1, std::min(a.total_items, occupancy_blocks<NT, LIFT>(b, lds) * b->cus));
-    hipLaunchKernelGGL((mc::k_multi<NT, LIFT, true>), dim3(grid), dim3(NT), (size_t)lds, st, a);
-  } else {
-    hipLaunchKernelGGL((mc::k_multi<NT, LIFT, false>), dim3(std::max(1, a.total_items)), dim3(NT),
-                       (size_t)lds, st, a);
-  }
-  return check_launch("batched step launch");
+int launch_multi(mc_batch* b, int nt, bool lift, const mc::MultiArgs& a, int lds, hipStream_t st) {
+  const int grid = b->persistent
+                       ? std::max(1, std::min(a.total_items, occupancy_blocks(b, nt, lift, lds) * b->cus))
+                       : std::max(1, a.total_items);
+  const hipError_t e = mcl::kernels(nt, lift).multi(a, b->persistent != 0, grid, (size_t)lds, st);
+  if (e != hipSuccess) return fail(MC_E_HIP, "batched step launch: %s", hipGetErrorString(e));
+  return MC_OK;
 }
 }  // namespace
 
@@ -1218,23 +1135,14 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
   a.total_items = items;
   a.counters = b->counters + 2 * slot;
   a.entries = de;
-  const int r = (nt == 512) ? (lift ? launch_multi<512, true>(b, a, lds, S(stream))
-                                   : launch_multi<512, false>(b, a, lds, S(stream)))
-                            : (lift ? launch_multi<256, true>(b, a, lds, S(stream))
-                                    : launch_multi<256, false>(b, a, lds, S(stream)));
+  const int r = launch_multi(b, nt, lift, a, lds, S(stream));
   if (r) return r;
   if (lds_full > 0) {
     // Full-height pass over the envs the staged groups deferred.
-    const dim3 grid(64);
-    if (lds_full > 80 * 1024) {
-      if (lift) hipLaunchKernelGGL((mc::k_multi_fb<512, true>), grid, dim3(512), lds_full, S(stream), a);
-      else hipLaunchKernelGGL((mc::k_multi_fb<512, false>), grid, dim3(512), lds_full, S(stream), a);
-    } else {
-      if (lift) hipLaunchKernelGGL((mc::k_multi_fb<256, true>), grid, dim3(256), lds_full, S(stream), a);
-      else hipLaunchKernelGGL((mc::k_multi_fb<256, false>), grid, dim3(256), lds_full, S(stream), a);
-    }
-    const int rf = check_launch("batched deferred-env launch");
-    if (rf) return rf;
+    const hipError_t ef = mcl::kernels(lds_full > 80 * 1024 ? 512 : 256, lift)
+                              .multi_fb(a, dim3(64), (size_t)lds_full, S(stream));
+    if (ef != hipSuccess)
+      return fail(MC_E_HIP, "batched deferred-env launch: %s", hipGetErrorString(ef));
   }
   e = hipEventRecord(b->ev[slot], S(stream));
   if (e != hipSuccess) return fail(MC_E_HIP, "batch event: %s", hipGetErrorString(e));

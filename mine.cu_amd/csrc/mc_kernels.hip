@@ -1,0 +1,107 @@
+// mc_kernels.hip - launch wrappers of the step and render kernels for one
+// workgroup width MC_NT (64, 256 or 512) and one setting MC_LIFT (0/1) of
+// the upward-ray exit.  The Makefile compiles this file once per pair; each
+// object exports the KernelSet kKernels<NT>_<LIFT> (mc_launch.h).
+#include "mc_launch.h"
+
+#ifndef MC_NT
+#error "compile with -DMC_NT=64|256|512 -DMC_LIFT=0|1"
+#endif
+
+#define MC_CAT3(a, b, c) a##b##_##c
+#define MC_SET_NAME(nt, lift) MC_CAT3(kKernels, nt, lift)
+
+namespace mcl {
+namespace {
+
+constexpr int NT = MC_NT;
+constexpr bool LIFT = MC_LIFT != 0;
+
+bool planar(int obs_mode) { return obs_mode == mc::kObsU8Chw || obs_mode == mc::kObsF32Chw; }
+
+template <int KIND>
+void launch_kind(const mc::Params& p, int obs_mode, int wpe, size_t lds, hipStream_t st) {
+  const int epw = (NT / mc::kWave) / wpe;
+  const dim3 grid((p.B + epw - 1) / epw);
+  // The planar modes, the seeded terrain and voxel collision run kernels of
+  // their own (template constants), so the flat-world interleaved kernels keep
+  // their code unchanged.
+  if (KIND == mc::kItemStep && p.collision && p.terrain)
+    hipLaunchKernelGGL((mc::k_env_col<NT, LIFT, true>), grid, dim3(NT), lds, st, p, obs_mode, wpe);
+  else if (KIND == mc::kItemStep && p.collision)
+    hipLaunchKernelGGL((mc::k_env_col<NT, LIFT, false>), grid, dim3(NT), lds, st, p, obs_mode, wpe);
+  else if (p.terrain)
+    hipLaunchKernelGGL((mc::k_env_terrain<NT, KIND, LIFT>), grid, dim3(NT), lds, st, p, obs_mode,
+                       wpe);
+  else if (KIND != mc::kItemRenderCameras && KIND != mc::kItemRenderBasis && planar(obs_mode))
+    hipLaunchKernelGGL((mc::k_env<NT, KIND, LIFT, true>), grid, dim3(NT), lds, st, p, obs_mode,
+                       wpe);
+  else
+    hipLaunchKernelGGL((mc::k_env<NT, KIND, LIFT>), grid, dim3(NT), lds, st, p, obs_mode, wpe);
+}
+
+hipError_t env(int kind, const mc::Params& p, int obs_mode, int wpe, size_t lds, hipStream_t st) {
+  switch (kind) {
+    case mc::kItemStep: launch_kind<mc::kItemStep>(p, obs_mode, wpe, lds, st); break;
+    case mc::kItemResetRender: launch_kind<mc::kItemResetRender>(p, obs_mode, wpe, lds, st); break;
+    case mc::kItemRenderCameras:
+      launch_kind<mc::kItemRenderCameras>(p, obs_mode, wpe, lds, st);
+      break;
+    case mc::kItemRenderBasis: launch_kind<mc::kItemRenderBasis>(p, obs_mode, wpe, lds, st); break;
+    default: launch_kind<mc::kItemRenderState>(p, obs_mode, wpe, lds, st); break;
+  }
+  return hipGetLastError();
+}
+
+#if MC_NT >= 256
+hipError_t fb(const mc::Params& p, int obs_mode, dim3 grid, size_t lds, hipStream_t st) {
+  if (p.collision && p.terrain)
+    hipLaunchKernelGGL((mc::k_env_fb_col<NT, LIFT, true>), grid, dim3(NT), lds, st, p, obs_mode);
+  else if (p.collision)
+    hipLaunchKernelGGL((mc::k_env_fb_col<NT, LIFT, false>), grid, dim3(NT), lds, st, p, obs_mode);
+  else if (p.terrain)
+    hipLaunchKernelGGL((mc::k_env_fb_terrain<NT, LIFT>), grid, dim3(NT), lds, st, p, obs_mode);
+  else if (planar(obs_mode))
+    hipLaunchKernelGGL((mc::k_env_fb<NT, LIFT, true>), grid, dim3(NT), lds, st, p, obs_mode);
+  else
+    hipLaunchKernelGGL((mc::k_env_fb<NT, LIFT>), grid, dim3(NT), lds, st, p, obs_mode);
+  return hipGetLastError();
+}
+
+hipError_t multi(const mc::MultiArgs& a, bool persist, int grid, size_t lds, hipStream_t st) {
+  if (persist)
+    hipLaunchKernelGGL((mc::k_multi<NT, LIFT, true>), dim3(grid), dim3(NT), lds, st, a);
+  else
+    hipLaunchKernelGGL((mc::k_multi<NT, LIFT, false>), dim3(grid), dim3(NT), lds, st, a);
+  return hipGetLastError();
+}
+
+hipError_t multi_fb(const mc::MultiArgs& a, dim3 grid, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((mc::k_multi_fb<NT, LIFT>), grid, dim3(NT), lds, st, a);
+  return hipGetLastError();
+}
+
+int multi_blocks(size_t lds) {
+  int nb = 0;
+  const hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mc::k_multi<NT, LIFT, true>, NT, lds);
+  if (e != hipSuccess || nb < 1) {
+    (void)hipGetLastError();
+    nb = 1;
+  }
+  return nb;
+}
+#endif
+
+}  // namespace
+
+const KernelSet& MC_SET_NAME(MC_NT, MC_LIFT)() {
+#if MC_NT >= 256
+  static const KernelSet set = {env, fb, multi, multi_fb, multi_blocks};
+#else
+  static const KernelSet set = {env, nullptr, nullptr, nullptr, nullptr};
+#endif
+  return set;
+}
+
+}  // namespace mcl

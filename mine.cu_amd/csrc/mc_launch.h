@@ -1,0 +1,46 @@
+// mc_launch.h - internal interface between the host logic (mc_capi.hip) and
+// the kernel launch wrappers of mc_kernels.hip.
+//
+// mc_kernels.hip is compiled once per workgroup width (64, 256, 512 threads)
+// and setting of the upward-ray exit (lift 0/1); each object holds the step
+// and render kernels of that width and exports one KernelSet.  The six
+// objects compile in parallel, and every kernel instantiation lives in
+// exactly one of them.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mc_device.h"
+
+namespace mcl {
+
+struct KernelSet {
+  // One item per workgroup over a group: grid = ceil(B / epw) with epw = NT /
+  // (64 wpe).  Dispatches on the item kind, the obs mode (interleaved or
+  // planar) and the world (flat or seeded terrain, plane or voxel collision).
+  hipError_t (*env)(int kind, const mc::Params& p, int obs_mode, int wpe, size_t lds,
+                    hipStream_t st);
+  // Full-height pass over the envs a staged-height step deferred (NULL for
+  // the 64-thread set).
+  hipError_t (*fb)(const mc::Params& p, int obs_mode, dim3 grid, size_t lds, hipStream_t st);
+  // Several groups in one launch (NULL for the 64-thread set).
+  hipError_t (*multi)(const mc::MultiArgs& a, bool persist, int grid, size_t lds,
+                      hipStream_t st);
+  hipError_t (*multi_fb)(const mc::MultiArgs& a, dim3 grid, size_t lds, hipStream_t st);
+  // Resident workgroups per CU of the persistent multi kernel at `lds` bytes.
+  int (*multi_blocks)(size_t lds);
+};
+
+const KernelSet& kKernels64_0();
+const KernelSet& kKernels64_1();
+const KernelSet& kKernels256_0();
+const KernelSet& kKernels256_1();
+const KernelSet& kKernels512_0();
+const KernelSet& kKernels512_1();
+
+inline const KernelSet& kernels(int nt, bool lift) {
+  if (nt == 64) return lift ? kKernels64_1() : kKernels64_0();
+  if (nt == 256) return lift ? kKernels256_1() : kKernels256_0();
+  return lift ? kKernels512_1() : kKernels512_0();
+}
+
+}  // namespace mcl
