@@ -114,6 +114,19 @@ typedef struct mc_state {
   void* obs;
   float* rewards;
   float* basis;            /* f32 [B,14] camera basis (kernels.cu:503-510), or NULL */
+  /* The decoded controls of the last step, MineEnv's public decode buffers
+   * (minecu/__init__.py:243-251, written by decode_actions_kernel,
+   * kernels.cu:1459-1504): f32 [B] forward_in, strafe_in, delta_yaw_in,
+   * delta_pitch_in, speed_mult; bool [B] jump_in, do_break, do_place.  All
+   * NULL (not written) or all non-NULL. */
+  float* forward_in;
+  float* strafe_in;
+  float* delta_yaw_in;
+  float* delta_pitch_in;
+  uint8_t* jump_in;
+  uint8_t* do_break;
+  uint8_t* do_place;
+  float* speed_mult;
 } mc_state;
 
 /* Minimum workspace (device bytes) for the stateless mc_step / mc_env_reset_state. */

@@ -121,6 +121,10 @@ int validate_state(const mc_state* s) {
   if (!s->voxels || !s->positions || !s->velocities || !s->yaws || !s->pitches ||
       !s->on_ground || !s->step_count || !s->do_reset || !s->cameras || !s->obs || !s->rewards)
     return fail(MC_E_INVALID, "every state pointer must be non-NULL");
+  const int ctrl = !!s->forward_in + !!s->strafe_in + !!s->delta_yaw_in + !!s->delta_pitch_in +
+                   !!s->jump_in + !!s->do_break + !!s->do_place + !!s->speed_mult;
+  if (ctrl != 0 && ctrl != 8)
+    return fail(MC_E_INVALID, "the decode buffers must be all NULL or all non-NULL");
   return MC_OK;
 }
 
@@ -186,6 +190,14 @@ mc::Params make_params(const mc_config* c, const mc_state* s) {
     p.obs = s->obs;
     p.rewards = s->rewards;
     p.basis = s->basis;
+    p.forward_in = s->forward_in;
+    p.strafe_in = s->strafe_in;
+    p.delta_yaw_in = s->delta_yaw_in;
+    p.delta_pitch_in = s->delta_pitch_in;
+    p.jump_in = s->jump_in;
+    p.do_break = s->do_break;
+    p.do_place = s->do_place;
+    p.speed_mult_in = s->speed_mult;
   }
   p.ys = c->world_y;
   p.fb = nullptr;

@@ -221,8 +221,8 @@ class MineEnv:
 
     # -- state -------------------------------------------------------------
     def _allocate_state(self):
-        """minecu/__init__.py:214-256 (the reference's eight decode buffers are
-        internal to its kernel chain and do not exist here)."""
+        """minecu/__init__.py:214-256, with the same tensors, names, dtypes
+        and shapes, the eight decode buffers included."""
         B = self.batch_size
         X, Y, Z = self.world_x, self.world_y, self.world_z
         H, W = self.render_height, self.render_width
@@ -240,6 +240,16 @@ class MineEnv:
         # Camera basis of the prebasis variants (minecu/__init__.py:237-241).
         self.basis = (torch.zeros((B, 14), dtype=torch.float32, device=dev)
                       if self.obs_dtype in _BASIS_OBS else None)
+        # Action decode buffers (minecu/__init__.py:243-251): the step's
+        # decoded controls, written by the fused step kernel.
+        self.forward_in = torch.zeros((B,), dtype=torch.float32, device=dev)
+        self.strafe_in = torch.zeros((B,), dtype=torch.float32, device=dev)
+        self.delta_yaw_in = torch.zeros((B,), dtype=torch.float32, device=dev)
+        self.delta_pitch_in = torch.zeros((B,), dtype=torch.float32, device=dev)
+        self.jump_in = torch.zeros((B,), dtype=torch.bool, device=dev)
+        self.do_break = torch.zeros((B,), dtype=torch.bool, device=dev)
+        self.do_place = torch.zeros((B,), dtype=torch.bool, device=dev)
+        self.speed_mult = torch.ones((B,), dtype=torch.float32, device=dev)
         self.rewards = torch.zeros((B,), dtype=torch.float32, device=dev)
         self.step_count = torch.zeros((B,), dtype=torch.int32, device=dev)
         self.do_reset = torch.zeros((B,), dtype=torch.bool, device=dev)
@@ -278,7 +288,11 @@ class MineEnv:
                         ("pitches", self.pitches), ("on_ground", self.on_ground),
                         ("step_count", self.step_count), ("do_reset", self.do_reset),
                         ("cameras", self.cameras), ("obs", self.obs_buffer),
-                        ("rewards", self.rewards)):
+                        ("rewards", self.rewards), ("forward_in", self.forward_in),
+                        ("strafe_in", self.strafe_in), ("delta_yaw_in", self.delta_yaw_in),
+                        ("delta_pitch_in", self.delta_pitch_in), ("jump_in", self.jump_in),
+                        ("do_break", self.do_break), ("do_place", self.do_place),
+                        ("speed_mult", self.speed_mult)):
             setattr(s, name, t.data_ptr())
         s.basis = self.basis.data_ptr() if self.basis is not None else None
         return c, s

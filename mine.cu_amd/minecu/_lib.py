@@ -45,7 +45,8 @@ class McConfig(ctypes.Structure):
 class McState(ctypes.Structure):
     _fields_ = [(n, _P) for n in (
         "voxels", "positions", "velocities", "yaws", "pitches", "on_ground", "step_count",
-        "do_reset", "cameras", "obs", "rewards", "basis")]
+        "do_reset", "cameras", "obs", "rewards", "basis", "forward_in", "strafe_in",
+        "delta_yaw_in", "delta_pitch_in", "jump_in", "do_break", "do_place", "speed_mult")]
 
 
 # Exported symbols and their argument types.

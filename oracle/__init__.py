@@ -43,42 +43,64 @@ def build() -> str:
     return _LIB_PATH
 
 
+def _bind(L):
+    """Declare the oracle library's signatures."""
+    P = ctypes.c_void_p
+    i, f, u = ctypes.c_int, ctypes.c_float, ctypes.c_uint
+    L.orc_sincosf.argtypes = [P, P, P, i]
+    L.orc_half_fov_tan.argtypes = [f]
+    L.orc_half_fov_tan.restype = f
+    L.orc_generate_world.argtypes = [P, i, i, i, i, i, u]
+    L.orc_place_tree.argtypes = [P, i, i, i, i, i, i, i]
+    L.orc_place_house.argtypes = [P, i, i, i, i, i, i, i]
+    L.orc_fmix32.argtypes = [u]
+    L.orc_fmix32.restype = u
+    L.orc_terrain_column.argtypes = [i, i, i, i, i, u]
+    L.orc_terrain_heights.argtypes = [P, i, i, i, i, i, u]
+    L.orc_terrain_world.argtypes = [P, i, i, i, i, i, i, u, i, i, i, i]
+    L.orc_reset.argtypes = [P, P, P, P, P, P, i, f, f, f]
+    L.orc_discrete_to_buttons.argtypes = [P, i, f, P, P]
+    L.orc_decode_actions.argtypes = [P, P, P, P, P, P, P, P, P, P, i, f]
+    L.orc_physics.argtypes = [P, P, P, P, P, P, P, P, P, P, P, i, i, i, i, f, f, f, f]
+    L.orc_physics_collide.argtypes = [P, P, P, P, P, P, P, P, P, P, P, P, i, i, i, i, f, f,
+                                      f, f]
+    L.orc_raycast_interact.argtypes = [P, P, P, P, P, P, P, i, i, i, i, i, f]
+    L.orc_update_cameras.argtypes = [P, P, P, P, i]
+    L.orc_render.argtypes = [P, P, P, i, i, i, i, i, i, i, i, f, f, P, i]
+    L.orc_half_fov_tan_flat.argtypes = [f]
+    L.orc_half_fov_tan_flat.restype = f
+    L.orc_camera_basis.argtypes = [P, P, i, i, i, f]
+    L.orc_render_flat.argtypes = [P, P, P, i, i, i, i, i, i, i, i, f, P, i]
+    L.orc_f16.argtypes = [P, P, P, P, P, i]
+    return L
+
+
 def lib():
     """Load (building first if absent) the oracle shared library."""
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        P = ctypes.c_void_p
-        i, f, u = ctypes.c_int, ctypes.c_float, ctypes.c_uint
-        L.orc_sincosf.argtypes = [P, P, P, i]
-        L.orc_half_fov_tan.argtypes = [f]
-        L.orc_half_fov_tan.restype = f
-        L.orc_generate_world.argtypes = [P, i, i, i, i, i, u]
-        L.orc_place_tree.argtypes = [P, i, i, i, i, i, i, i]
-        L.orc_place_house.argtypes = [P, i, i, i, i, i, i, i]
-        L.orc_fmix32.argtypes = [u]
-        L.orc_fmix32.restype = u
-        L.orc_terrain_column.argtypes = [i, i, i, i, i, u]
-        L.orc_terrain_heights.argtypes = [P, i, i, i, i, i, u]
-        L.orc_terrain_world.argtypes = [P, i, i, i, i, i, i, u, i, i, i, i]
-        L.orc_reset.argtypes = [P, P, P, P, P, P, i, f, f, f]
-        L.orc_discrete_to_buttons.argtypes = [P, i, f, P, P]
-        L.orc_decode_actions.argtypes = [P, P, P, P, P, P, P, P, P, P, i, f]
-        L.orc_physics.argtypes = [P, P, P, P, P, P, P, P, P, P, P, i, i, i, i, f, f, f, f]
-        L.orc_physics_collide.argtypes = [P, P, P, P, P, P, P, P, P, P, P, P, i, i, i, i, f, f,
-                                          f, f]
-        L.orc_raycast_interact.argtypes = [P, P, P, P, P, P, P, i, i, i, i, i, f]
-        L.orc_update_cameras.argtypes = [P, P, P, P, i]
-        L.orc_render.argtypes = [P, P, P, i, i, i, i, i, i, i, i, f, f, P, i]
-        L.orc_half_fov_tan_flat.argtypes = [f]
-        L.orc_half_fov_tan_flat.restype = f
-        L.orc_camera_basis.argtypes = [P, P, i, i, i, f]
-        L.orc_render_flat.argtypes = [P, P, P, i, i, i, i, i, i, i, i, f, P, i]
-        L.orc_f16.argtypes = [P, P, P, P, P, i]
-        _lib = L
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+_mutants = {}
+
+
+def mutant_lib(k: int):
+    """The oracle compiled with planted slip number k in its shading (see
+    ORC_MUTANT in mc_oracle.cpp) -- only for the test that shows the render
+    pins catch such a slip."""
+    if k not in _mutants:
+        out = os.path.join(_HERE, "_build", f"libmc_oracle_mutant{k}.so")
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        subprocess.run(["g++", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-std=c++17",
+                        "-pthread", f"-DORC_MUTANT={k}",
+                        "-I" + os.path.join(_HERE, "..", "mine.cu_amd", "csrc"), "-shared", "-o", out,
+                        os.path.join(_HERE, "mc_oracle.cpp")], check=True)
+        _mutants[k] = _bind(ctypes.CDLL(out))
+    return _mutants[k]
 
 
 def _p(a: Optional[np.ndarray]):
@@ -141,11 +163,11 @@ def fmix32(h: int) -> int:
 
 
 def render(voxels, cameras, width, height, max_steps=64, view_distance=32.0,
-           fov_degrees=70.0, out_u8=True, want_hits=False, nthreads=None):
+           fov_degrees=70.0, out_u8=True, want_hits=False, nthreads=None, L=None):
     B, Y, X, Z = voxels.shape
     out = np.empty((B, height, width, 3), dtype=np.uint8 if out_u8 else np.float32)
     hits = np.empty((B, height, width), dtype=np.int32) if want_hits else None
-    lib().orc_render(_p(voxels), _p(cameras), _p(out), int(out_u8), B, X, Y, Z, width, height,
+    (L or lib()).orc_render(_p(voxels), _p(cameras), _p(out), int(out_u8), B, X, Y, Z, width, height,
                      max_steps, view_distance, fov_degrees, _p(hits),
                      nthreads or default_threads())
     return (out, hits) if want_hits else out
@@ -263,6 +285,10 @@ class OracleEnv:
         self.rewards = np.zeros(B, np.float32)
         self.step_count = np.zeros(B, np.int32)
         self.do_reset = np.zeros(B, np.uint8)
+        self.forward_in, self.strafe_in = np.zeros(B, np.float32), np.zeros(B, np.float32)
+        self.delta_yaw_in, self.delta_pitch_in = np.zeros(B, np.float32), np.zeros(B, np.float32)
+        self.jump_in, self.do_break = np.zeros(B, np.uint8), np.zeros(B, np.uint8)
+        self.do_place, self.speed_mult = np.zeros(B, np.uint8), np.ones(B, np.float32)
         self.voxels = self._fresh()
         self.hits = None
 
@@ -317,9 +343,14 @@ class OracleEnv:
             self.do_reset = (self.step_count >= self.episode_length).astype(np.uint8)
         else:
             self.do_reset[:] = 0
+        # The decode buffers are kept as attributes, as MineEnv's are
+        # (minecu/__init__.py:243-251).
         fwd, strafe, dyaw, dpitch = (np.empty(B, np.float32) for _ in range(4))
         jump, brk, place = (np.empty(B, np.uint8) for _ in range(3))
         speed = np.empty(B, np.float32)
+        (self.forward_in, self.strafe_in, self.delta_yaw_in, self.delta_pitch_in, self.jump_in,
+         self.do_break, self.do_place, self.speed_mult) = (fwd, strafe, dyaw, dpitch, jump, brk,
+                                                           place, speed)
         L = lib()
         L.orc_decode_actions(_p(buttons), _p(look), _p(fwd), _p(strafe), _p(dyaw), _p(dpitch),
                              _p(jump), _p(brk), _p(place), _p(speed), B, self.sprint_mult)

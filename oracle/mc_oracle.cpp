@@ -29,6 +29,13 @@
 
 #include "mc_math.h"
 
+// ORC_MUTANT (0 = none) compiles one planted slip into the shading, only for
+// the test that checks the fog pins catch it (tests/test_oracle_known_answers.py);
+// the oracle built by the Makefile has none.
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 namespace {
 
 // Block ids (src/kernels.cu:11-24).
@@ -168,7 +175,15 @@ PixelResult render_pixel(const int8_t* vox, int X, int Y, int Z, const float* ca
     else if (hit_face == 2) shade = 0.9f;
     else if (hit_face == 1 && step_y < 0) shade = 0.6f;
     float fog = fminf(t / view_distance, 1.0f);
+#if ORC_MUTANT == 1  // planted slip for the pin tests: fog not squared
+#elif ORC_MUTANT == 2  // planted slip: (1 - fog) and fog swapped
+    fog = 1.0f - fog * fog;
+#elif ORC_MUTANT == 3  // planted slip: face shade dropped
     fog = fog * fog;
+    shade = 1.0f;
+#else
+    fog = fog * fog;
+#endif
     res.rgb[0] = r * shade * (1.0f - fog) + SKY_COLOR[0] * fog;
     res.rgb[1] = g * shade * (1.0f - fog) + SKY_COLOR[1] * fog;
     res.rgb[2] = b * shade * (1.0f - fog) + SKY_COLOR[2] * fog;

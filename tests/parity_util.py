@@ -15,6 +15,8 @@ from __future__ import annotations
 import numpy as np
 
 POS_ULP = 1
+DECODE_BUFFERS = ("forward_in", "strafe_in", "delta_yaw_in", "delta_pitch_in", "jump_in",
+                  "do_break", "do_place", "speed_mult")
 OBS_DIFF_RATE = 1e-3
 OBS_MAX_LSB = 1
 
@@ -89,6 +91,14 @@ def compare_env(env, orc, stats: Stats, where: str, check_obs=True):
     compare_exact("step_count", env.step_count.cpu().numpy(), orc.step_count, where)
     for k in ("positions", "velocities", "yaws", "pitches", "cameras"):
         compare_float(k, getattr(env, k).cpu().numpy(), getattr(orc, k), stats, where)
+    # The decode buffers (minecu/__init__.py:243-251): bit-exact.
+    for k in DECODE_BUFFERS:
+        g = getattr(env, k).cpu().numpy()
+        o = getattr(orc, k)
+        if g.dtype == np.bool_:
+            compare_exact(k, g.astype(np.uint8), o.astype(np.uint8), where)
+        else:
+            compare_exact(k, g.view(np.int32), o.view(np.int32), where)
     if check_obs:
         compare_obs(env.obs_buffer.cpu().numpy(), orc.obs_buffer, stats, where)
 

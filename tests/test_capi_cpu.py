@@ -29,13 +29,15 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
     _l, _ = _lib()
-    # 31 four-byte fields (terrain, terrain_amplitude, collision last), 12 pointers
+    # 31 four-byte fields (terrain, terrain_amplitude, collision last), 20 pointers (the
+    # decode buffers last)
     assert ctypes.sizeof(_l.McConfig) == 31 * 4
     assert [f[0] for f in _l.McConfig._fields_][-3:] == ["terrain", "terrain_amplitude", "collision"]
-    assert ctypes.sizeof(_l.McState) == 12 * 8
+    assert ctypes.sizeof(_l.McState) == 20 * 8
+    assert [f[0] for f in _l.McState._fields_][-1] == "speed_mult"
     text = open(HEADER).read()
     state = text[text.index("typedef struct mc_state"):text.index("} mc_state;")]
-    assert len(re.findall(r"\*\s*\w+;", state)) == 12
+    assert len(re.findall(r"\*\s*\w+;", state)) == 20
 
 
 def _err(L):

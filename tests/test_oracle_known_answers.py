@@ -115,8 +115,9 @@ SHADE_TABLE = {  # (shade 1, 0.8, 0.9, 0.6) at fog 0, truncated to uint8
 }
 
 
-def _render1(vox, cam, W=1, H=1, vd=1e30, max_steps=64, fov=70.0):
-    return oracle.render(vox[None], np.asarray(cam, np.float32)[None], W, H, max_steps, vd, fov)[0]
+def _render1(vox, cam, W=1, H=1, vd=1e30, max_steps=64, fov=70.0, L=None):
+    return oracle.render(vox[None], np.asarray(cam, np.float32)[None], W, H, max_steps, vd, fov,
+                         L=L)[0]
 
 
 @pytest.mark.parametrize("block", sorted(SHADE_TABLE))
@@ -300,3 +301,177 @@ def test_batch_independence():
             one.step(acts[t, b:b + 1])
         assert np.array_equal(one.obs_buffer[0], big.obs_buffer[b])
         assert np.array_equal(one.positions[0], big.positions[b])
+
+
+# --- fog blend (src/kernels.cu:455-464 uint8, 281-291 float32) ----------------
+# colour = block * shade * (1 - fog) + sky * fog, fog = min(t / view_distance, 1)^2,
+# then uint8 truncates colour * 255.  Expected values here come from that
+# closed form in float64 at exactly known t; each case is chosen with every
+# channel at least 1e-3 away from an integer boundary, so float32 rounding
+# cannot move the truncated byte, while a fog that is not squared, a swapped
+# (1 - fog) / fog, or a missing shade moves at least one channel.
+PALETTE = {OAKLOG: (0.55, 0.40, 0.25), LEAVES: (0.20, 0.55, 0.15), GRASS: (0.30, 0.65, 0.20)}
+SKY = (0.53, 0.81, 0.92)
+
+
+def _fogged(block, shade, t, vd):
+    fog = min(t / vd, 1.0) ** 2
+    return [c * shade * (1.0 - fog) + s * fog for c, s in zip(PALETTE[block], SKY)]
+
+
+def _u8(rgb):
+    vals = [c * 255.0 for c in rgb]
+    for v in vals:
+        assert abs(v - round(v)) > 1e-3, "case too close to a rounding boundary"
+    return tuple(int(math.floor(v)) for v in vals)
+
+
+def _z_face_world(block):
+    """32^3 air world with one block at (y 4, x 4, z 14); an eye at z = 4.25
+    looking +z along the voxel row's centre hits its z face at t = 9.75."""
+    vox = np.full((32, 32, 32), AIR, np.int8)
+    vox[4, 4, 14] = block
+    return vox, [4.5, 4.5 - 1.6, 4.25, 0.0, 0.0]
+
+
+def test_fog_known_answer_oaklog_z_face():
+    vox, cam = _z_face_world(OAKLOG)
+    got = tuple(_render1(vox, cam, vd=32.0)[0, 0])
+    assert got == (127, 102, 73) == _u8(_fogged(OAKLOG, 0.9, 9.75, 32.0))
+
+
+@pytest.mark.parametrize("vd", [10.0, 12.5, 20.0, 64.0])
+def test_fog_known_answers_vs_closed_form(vd):
+    """The same z face at several view distances (fog from 0.023 to 0.95)."""
+    vox, cam = _z_face_world(OAKLOG)
+    assert tuple(_render1(vox, cam, vd=vd)[0, 0]) == _u8(_fogged(OAKLOG, 0.9, 9.75, vd))
+
+
+def _check_fog_known_answers(L=None):
+    """The z-face known answers at four view distances, uint8 and float32."""
+    vox, cam = _z_face_world(OAKLOG)
+    for vd in (32.0, 10.0, 12.5, 20.0):
+        assert tuple(_render1(vox, cam, vd=vd, L=L)[0, 0]) == _u8(_fogged(OAKLOG, 0.9, 9.75, vd))
+        out = oracle.render(vox[None], np.asarray(cam, np.float32)[None], 1, 1, 64, vd, 70.0,
+                            out_u8=False, L=L)[0, 0, 0]
+        assert np.max(np.abs(out.astype(np.float64) - _fogged(OAKLOG, 0.9, 9.75, vd))) < 1e-6
+
+
+def test_fog_known_answers_all_view_distances():
+    _check_fog_known_answers()
+
+
+@pytest.mark.parametrize("k,slip", [(1, "fog not squared"), (2, "(1 - fog) and fog swapped"),
+                                    (3, "face shade dropped")])
+def test_fog_pins_catch_planted_slips(k, slip):
+    """The oracle compiled with a planted slip in its shading fails the
+    known answers."""
+    with pytest.raises(AssertionError):
+        _check_fog_known_answers(oracle.mutant_lib(k))
+
+
+@pytest.mark.parametrize("vd", [9.75, 9.0, 2.0])
+def test_fog_at_or_beyond_view_distance_is_sky(vd):
+    """t >= view_distance ends the ray before the lookup (kernels.cu:413-417):
+    the pixel is the shaded sky (135, 206, 234), not a fully fogged block."""
+    vox, cam = _z_face_world(OAKLOG)
+    assert tuple(_render1(vox, cam, vd=vd)[0, 0]) == (135, 206, 234)
+
+
+@pytest.mark.parametrize("vd", [32.0, 12.5])
+def test_fog_float32_obs_closed_form(vd):
+    """float32 observations (render_kernel, kernels.cu:281-291) equal the
+    closed form to float32 rounding (|error| < 1e-6 per channel)."""
+    vox, cam = _z_face_world(OAKLOG)
+    out = oracle.render(vox[None], np.asarray(cam, np.float32)[None], 1, 1, 64, vd, 70.0,
+                        out_u8=False)[0, 0, 0]
+    exp = _fogged(OAKLOG, 0.9, 9.75, vd)
+    assert np.max(np.abs(out.astype(np.float64) - exp)) < 1e-6
+    # beyond the view distance: the sky colour itself, exactly
+    sky = oracle.render(vox[None], np.asarray(cam, np.float32)[None], 1, 1, 64, 9.0, 70.0,
+                        out_u8=False)[0, 0, 0]
+    assert np.array_equal(sky, np.float32(SKY))
+
+
+def test_fog_x_and_top_faces():
+    """Shades 0.8 (x face) and 0.6 (top face seen looking down) under fog."""
+    vox = np.full((32, 32, 32), AIR, np.int8)
+    vox[4, 20, 4] = GRASS
+    # eye x = 4.5 looking +x (yaw pi/2: forward = (1, 0, cos(pi/2)))
+    cam = [4.5, 4.5 - 1.6, 4.5, math.pi / 2, 0.0]
+    t = 20.0 - 4.5
+    assert tuple(_render1(vox, cam, vd=32.0)[0, 0]) == _u8(_fogged(GRASS, 0.8, t, 32.0))
+    vox = np.full((32, 32, 32), AIR, np.int8)
+    vox[4, 4, 4] = LEAVES
+    # eye 7.25 above the top face y = 5, looking straight down is pitch
+    # -pi/2, beyond the clamp; the kernel takes any pitch from the camera row.
+    cam = [4.5, 12.25 - 1.6, 4.5, 0.0, -math.pi / 2]
+    out = _render1(vox, cam, vd=16.0)[0, 0]
+    assert tuple(out) == _u8(_fogged(LEAVES, 0.6, 12.25 - 5.0, 16.0))
+
+
+# --- done-step order (minecu/__init__.py:435-442; SURVEY Q3) ------------------
+def _q3_env(obs_dtype):
+    return oracle.OracleEnv(batch_size=1, world_size=32, render_width=1, render_height=1,
+                            episode_length=12, obs_dtype=obs_dtype)
+
+
+def _check_done_step_order(env):
+    """Steps a fresh 32^3 one-env oracle (1x1 image) through the done step and
+    checks what it must render; see test_done_step_renders_before_regeneration."""
+    L = env.episode_length
+    env.reset()
+    assert env.voxels[0, 11, 16, 18] == LEAVES
+    _, r, d = env.step(np.array([6]))
+    assert r[0] == 0.0 and not d[0] and env.voxels[0, 11, 16, 18] == AIR
+    for _ in range(1, L):
+        _, _, d = env.step(np.array([0]))
+        assert not d[0]
+    assert env.positions[0, 1] == 8.0 and env.voxels[0, 9, 16, 18] == OAKLOG
+    obs, r, d = env.step(np.array([6]))
+    assert d[0] and r[0] == 1.0
+    leaves = _fogged(LEAVES, 0.9, 2.0, 32.0)
+    if env.obs_dtype == "uint8":
+        assert tuple(obs[0, 0, 0]) == (135, 206, 234)
+    else:
+        assert np.array_equal(obs[0, 0, 0], np.float32(SKY))
+    # spawn pose, regenerated world (leaves and log back)
+    assert env.positions[0].tolist() == [16.0, 10.0, 16.0]
+    assert np.array_equal(env.voxels, oracle.fresh_world(1, 32, 32, 32, 8, (16, 18), (22, 14)))
+    obs, r, d = env.step(np.array([0]))
+    assert not d[0]
+    if env.obs_dtype == "uint8":
+        assert tuple(obs[0, 0, 0]) == _u8(leaves) == (46, 126, 35)
+    else:
+        assert np.max(np.abs(obs[0, 0, 0].astype(np.float64) - leaves)) < 1e-6
+
+
+@pytest.mark.parametrize("obs_dtype", ["uint8", "float32"])
+def test_done_step_renders_before_regeneration(obs_dtype):
+    """32^3, one env, 1x1 image (the centre ray is the forward vector).
+    Step 0 breaks the leaves at (y 11, x 16, z 18) straight ahead of the
+    falling agent's eye.  The done step (step index L after reset) respawns
+    the agent facing +z at eye height 11.6; its image is rendered in the world
+    as it was before regeneration, so the centre ray passes where the leaves
+    were and leaves the world: sky.  The world is regenerated afterwards; the
+    next step's centre ray hits the leaves' z face at t = 2.  The done step's
+    own break (from the landed pose) still earns the log's reward."""
+    _check_done_step_order(_q3_env(obs_dtype))
+
+
+@pytest.mark.parametrize("obs_dtype", ["uint8", "float32"])
+def test_done_step_order_pin_catches_regenerate_first(obs_dtype):
+    """Planted on one oracle env: regenerate the worlds before the done
+    step's render (the render sees do_reset already set) -- the check above
+    fails."""
+    env = _q3_env(obs_dtype)
+    plain = env._render
+
+    def regen_then_render(want_hits=False):
+        if env.do_reset.any():
+            env._regen()
+        return plain(want_hits)
+
+    env._render = regen_then_render
+    with pytest.raises(AssertionError):
+        _check_done_step_order(env)
