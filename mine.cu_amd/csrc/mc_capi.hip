@@ -297,10 +297,14 @@ int staged_layers(const mc_config& c) {
   const int Y = c.world_y;
   if (c.world_x > mc::kMaxInsideDim || Y > mc::kMaxInsideDim || c.world_z > mc::kMaxInsideDim)
     return Y;
-  if (!(c.gravity < 0.0f) || !(c.jump_vel >= 0.0f) || !(c.spawn_y < (float)Y)) return Y;
+  // On the seeded terrain an env respawns up to terrain_amplitude above spawn_y
+  // (its spawn column's surface, spawn_of() in mc_device.h).
+  const int amp = c.terrain == MC_TERRAIN_NOISE ? c.terrain_amplitude : 0;
+  const float spawn_top = c.spawn_y + (float)amp;
+  if (!(c.gravity < 0.0f) || !(c.jump_vel >= 0.0f) || !(spawn_top < (float)Y)) return Y;
   const float apex = c.jump_vel * c.jump_vel / (-2.0f * c.gravity);
-  const float eye = std::max(c.spawn_y, (float)(Y / 4) + apex) + 1.6f;
-  const int top = c.ground_height + (c.terrain == MC_TERRAIN_NOISE ? c.terrain_amplitude : 0);
+  const float eye = std::max(spawn_top, (float)(Y / 4) + apex) + 1.6f;
+  const int top = c.ground_height + amp;
   // With collision the agent can stand on any solid: up to the house roof.
   const float eye_c = c.collision ? std::max(eye, (float)(top + 5) + apex + 1.6f) : eye;
   const float need = std::max((float)(top + 5), std::ceil(eye_c + 4.5f) + 1.0f);

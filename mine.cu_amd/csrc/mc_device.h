@@ -1185,6 +1185,18 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
   }
 }
 
+// Respawn height of env b: spawn_y on the flat world; on the seeded terrain
+// (TER) spawn_y raised or lowered by the height of env b's spawn column
+// (floor(spawn_x), floor(spawn_z)) relative to ground_height, so the agent
+// appears the same distance above its own surface (no counterpart in the
+// reference, whose world is flat).
+template <bool TER>
+__device__ __forceinline__ float spawn_of(const Params& p, int b) {
+  if (!TER) return p.spawn_y;
+  const int h = terrain_height(p, b, (int)floorf(p.spawn_x), (int)floorf(p.spawn_z));
+  return p.spawn_y + (float)(h - p.ground_height);
+}
+
 // ---------------------------------------------------------------------------
 // Per-env step logic, executed by the 64 lanes of wave 0 (values uniform
 // across lanes except in the raycast).  Order: episode check
@@ -1241,10 +1253,11 @@ __device__ __forceinline__ EnvIn load_env_in(const Params& p, int b) {
   return in;
 }
 
+// spawn_y: env b's respawn height (spawn_of()).
 template <bool COL = false>
 __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, int8_t* world,
                                             int8_t* gworld, bool any_done, const EnvIn& in,
-                                            bool upper_solid) {
+                                            bool upper_solid, float spawn_y) {
   const int lane = threadIdx.x & (kWave - 1);
   const int sc = in.sc;
   const bool done = in.done;
@@ -1343,7 +1356,7 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
   // the full-height pass.
   if (p.ys < p.Y) {
     const float rx = done ? p.spawn_x : nx, rz = done ? p.spawn_z : nz;
-    const float ry = (done ? p.spawn_y : ny) + 1.6f, ys = (float)p.ys;
+    const float ry = (done ? spawn_y : ny) + 1.6f, ys = (float)p.ys;
     const bool ok = !upper_solid && (ny + 1.6f) + 4.5f < ys && rx >= 0.0f && rx < (float)p.X &&
                     ry >= 0.0f && ry < ys && rz >= 0.0f && rz < (float)p.Z;
     if (!ok) {
@@ -1409,7 +1422,7 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
 
   // Masked respawn (after physics and raycast, as launch_step_uint8 orders it).
   if (done) {
-    nx = p.spawn_x; ny = p.spawn_y; nz = p.spawn_z;
+    nx = p.spawn_x; ny = spawn_y; nz = p.spawn_z;
     vel_y = 0.0f;
     yaw = 0.0f;
     pitch = 0.0f;
@@ -1476,13 +1489,13 @@ __device__ __forceinline__ void load_palette(float* pal, uint32_t* flat, int tid
 // respawn when masked (reset_kernel, kernels.cu:1775-1785), step_count = 0,
 // report "done on the next step" into the episode ring, refresh the camera.
 __device__ __forceinline__ Frame reset_update(const Params& p, int slot, int b,
-                                              const int8_t* world) {
+                                              const int8_t* world, float spawn_y) {
   const int lane = threadIdx.x & (kWave - 1);
   const bool r = p.mask ? (p.mask[b] != 0) : true;
   float x = p.positions[b * 3 + 0], y = p.positions[b * 3 + 1], z = p.positions[b * 3 + 2];
   float yaw = p.yaws[b], pitch = p.pitches[b];
   if (r) {
-    x = p.spawn_x; y = p.spawn_y; z = p.spawn_z;
+    x = p.spawn_x; y = spawn_y; z = p.spawn_z;
     yaw = 0.0f; pitch = 0.0f;
   }
   if (lane == 0) {
@@ -1630,8 +1643,8 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
       for (int k = 0; k < wpe; ++k) upper_solid |= upper[e * wpe + k] != 0;
     Frame f;
     if (KIND == kItemStep)
-      f = env_update<COL>(p, slot, b, world, gworld, any_done, in, upper_solid);
-    else if (KIND == kItemResetRender) f = reset_update(p, slot, b, world);
+      f = env_update<COL>(p, slot, b, world, gworld, any_done, in, upper_solid, spawn_of<TER>(p, b));
+    else if (KIND == kItemResetRender) f = reset_update(p, slot, b, world, spawn_of<TER>(p, b));
     else if (KIND == kItemRenderBasis) f = frame_from_basis(p, b, world);
     else f = camera_update(p, b, world, KIND == kItemRenderState);
     if (et == 0) {

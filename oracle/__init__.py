@@ -270,6 +270,16 @@ class OracleEnv:
         self.spawn_z = self.world_z / 2.0
         self.tree_x, self.tree_z = int(self.spawn_x), int(self.spawn_z) + 2
         self.house_x, self.house_z = int(self.spawn_x) + 6, int(self.spawn_z) - 2
+        # Per-env respawn height: on the seeded terrain, spawn_y shifted by the
+        # height of the env's spawn column (int(spawn_x), int(spawn_z)) above
+        # or below ground_height (the library's definition; the reference's
+        # world is flat).
+        self.spawn_ys = np.full(B, self.spawn_y, np.float32)
+        if self.terrain_amplitude is not None:
+            for b in range(B):
+                h = terrain_column(b, int(self.spawn_x), int(self.spawn_z), ground_height,
+                                   self.terrain_amplitude, seed)
+                self.spawn_ys[b] = np.float32(self.spawn_y) + np.float32(h - ground_height)
         # State (minecu/__init__.py:214-256).
         X, Y, Z = self.world_x, self.world_y, self.world_z
         self.positions = np.zeros((B, 3), np.float32)
@@ -291,6 +301,11 @@ class OracleEnv:
         self.do_place, self.speed_mult = np.zeros(B, np.uint8), np.ones(B, np.float32)
         self.voxels = self._fresh()
         self.hits = None
+
+    def _spawn_heights(self):
+        """Respawned envs stand spawn_ys[b] high (the flat world: spawn_y)."""
+        m = self.do_reset.astype(bool)
+        self.positions[m, 1] = self.spawn_ys[m]
 
     def _fresh(self):
         return fresh_world(self.batch_size, self.world_x, self.world_y, self.world_z,
@@ -322,6 +337,7 @@ class OracleEnv:
         lib().orc_reset(_p(self.positions), _p(self.velocities), _p(self.yaws), _p(self.pitches),
                         _p(self.on_ground), _p(self.do_reset), self.batch_size, self.spawn_x,
                         self.spawn_y, self.spawn_z)
+        self._spawn_heights()
         self.step_count[self.do_reset.astype(bool)] = 0
         if mask is None or np.asarray(mask).any():
             self._regen()
@@ -371,6 +387,7 @@ class OracleEnv:
         L.orc_reset(_p(self.positions), _p(self.velocities), _p(self.yaws), _p(self.pitches),
                     _p(self.on_ground), _p(self.do_reset), B, self.spawn_x, self.spawn_y,
                     self.spawn_z)
+        self._spawn_heights()
         if self.flat is None:
             self._render(want_hits)
         else:

@@ -50,6 +50,31 @@ def test_terrain_matches_oracle(ws, W, H, dtype, amp, seed):
     assert st.exact, st.summary()
 
 
+@pytest.mark.parametrize("collision", [False, True])
+def test_terrain_spawn_on_column_surface(collision):
+    """Amplitude 8 (spawn columns from 2 to 18 high, ground_height 10): every
+    env spawns and respawns on its own spawn column's surface; kernel and
+    oracle agree bit for bit through reset, falls, landings and a done step."""
+    import minecu
+    B = 48
+    kw = dict(batch_size=B, world_size=32, render_width=32, render_height=24, obs_dtype="uint8",
+              episode_length=10, seed=42, ground_height=10, terrain_amplitude=8,
+              collision=collision)
+    env = minecu.MineEnv(device="cuda", terrain="noise", **kw)
+    orc = oracle.OracleEnv(terrain="noise", **kw)
+    env.reset()
+    orc.reset()
+    st = Stats()
+    compare_env(env, orc, st, "reset")
+    acts = _acts(24, B, 5)
+    for t in range(len(acts)):
+        _, r, d = env.step(torch.as_tensor(acts[t]).cuda())
+        _, ro, do = orc.step(acts[t])
+        assert np.array_equal(r.cpu().numpy(), ro) and np.array_equal(d.cpu().numpy(), do), t
+        compare_env(env, orc, st, f"step {t}")
+    assert st.exact, st.summary()
+
+
 def test_amplitude_zero_steps_like_the_flat_world():
     import minecu
     kw = dict(batch_size=32, world_size=16, render_width=32, render_height=32, obs_dtype="uint8",

@@ -88,3 +88,40 @@ def test_worlds_depend_on_seed_and_env_and_props_stand_on_the_surface():
         hh = oracle.terrain_column(e, hs[0], hs[1], 8, 3, 1)
         assert a[e, hh, hs[0], hs[1]] == oracle.COBBLESTONE          # house floor corner
         assert a[e, hh + 4, hs[0] + 1, hs[1] + 1] == oracle.BRICK      # roof
+
+
+@pytest.mark.parametrize("collision", [False, True])
+def test_spawn_stands_on_the_spawn_column(collision):
+    """On the seeded terrain every env respawns 2 above its own spawn column
+    (ground_height + 2 on the flat world): at amplitude 8 no reset eye is inside
+    a solid, and with voxel collision an agent falls onto its surface instead
+    of climbing out of buried terrain."""
+    B, gh, A = 64, 10, 8
+    env = oracle.OracleEnv(batch_size=B, world_size=32, render_width=4, render_height=4,
+                           terrain="noise", terrain_amplitude=A, ground_height=gh, seed=42,
+                           collision=collision, episode_length=30)
+    env.reset()
+    sx, sz = int(env.spawn_x), int(env.spawn_z)
+    heights = np.array([oracle.terrain_column(b, sx, sz, gh, A, 42) for b in range(B)])
+    assert heights.min() < gh < heights.max()          # the columns differ
+    assert np.array_equal(env.positions[:, 1], (heights + 2).astype(np.float32))
+    for b in range(B):
+        eye = env.positions[b] + np.float32([0, 1.6, 0])
+        assert env.voxels[b, int(eye[1]), int(eye[0]), int(eye[2])] == -1
+        assert env.voxels[b, heights[b] - 1, sx, sz] >= 0   # the surface block below
+    for _ in range(4):
+        env.step(np.zeros(B, np.int64))
+    if collision:
+        # falling, never rising: at most 2 above the column, landed at its top
+        assert (env.positions[:, 1] <= heights + 2).all()
+    # a done step respawns on the same column surface
+    for _ in range(27):
+        env.step(np.zeros(B, np.int64))
+    assert env.do_reset.all()
+    assert np.array_equal(env.positions[:, 1], (heights + 2).astype(np.float32))
+
+
+def test_flat_world_spawn_is_spawn_y():
+    env = oracle.OracleEnv(batch_size=3, world_size=16, render_width=4, render_height=4)
+    env.reset()
+    assert (env.positions[:, 1] == 10.0).all()
