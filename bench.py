@@ -95,12 +95,14 @@ def algorithmic_bytes_per_env_step(w, obs_bytes):
     return w["W"] * w["H"] * 3 * obs_bytes + vox + 71 + regen
 
 
-def cpu_baseline(w, obs, seconds_budget=15.0):
-    """Time the scalar oracle on a bounded sample of the same workload."""
+def cpu_baseline(w, obs, seconds_budget=15.0, threads=None):
+    """Time the scalar oracle on a bounded sample of the same workload (all
+    host cores up to 16, or `threads`)."""
     import numpy as np
     import oracle
-    threads = oracle.default_threads()
-    B = min(w["batch"], 8192 if w["world"] <= 16 else 1024)
+    threads = threads or oracle.default_threads()
+    B = min(w["batch"], (8192 if w["world"] <= 16 else 1024) if threads > 1 else
+            (1024 if w["world"] <= 16 else 128))
     # whole episodes (regeneration steps included) until the time budget
     steps = 10 * (w["episode_length"] or 60)
     kw = dict(batch_size=B, world_size=w["world"], render_width=w["W"], render_height=w["H"],
@@ -121,7 +123,21 @@ def cpu_baseline(w, obs, seconds_budget=15.0):
                       f"{done_steps} steps from reset of the same workload ({dt:.1f} s)"}
 
 
-def run_gpu(w, obs, steps, warmup, layout="hwc", terrain="flat", collision=False):
+def preroll_steps(w, warmup):
+    """Untimed steps after the warmup that put a done step (with its world
+    regeneration) inside the timed window whatever its length: the window
+    starts two steps before the done step at index L, 2L, ... after reset."""
+    L = w["episode_length"]
+    if not L or L < 3:
+        return 0
+    return ((L - 2) - warmup) % L
+
+
+def run_gpu(w, obs, steps, warmup, layout="hwc", terrain="flat", collision=False,
+            count_rays=False):
+    """Reset, warmup + preroll untimed steps, then `steps` timed steps.
+    count_rays: instead of timing, run the ray-step counting build of the step
+    over the same window and return its counters."""
     import torch
     import minecu
     dev = torch.device("cuda", 0)
@@ -129,10 +145,21 @@ def run_gpu(w, obs, steps, warmup, layout="hwc", terrain="flat", collision=False
                          render_height=w["H"], device=str(dev), obs_dtype=obs,
                          episode_length=w["episode_length"], obs_layout=layout,
                          terrain=terrain, collision=collision)
-    acts = make_actions(w, warmup + steps, dev)
+    pre = preroll_steps(w, warmup)
+    acts = make_actions(w, warmup + pre + steps, dev)
     env.reset()
-    for t in range(warmup):
+    for t in range(warmup + pre):
         env.step(acts[t])
+    warmup = warmup + pre
+    if count_rays:
+        counters = torch.zeros(3, dtype=torch.int64, device=dev)
+        env.count_ray_steps(counters)
+        for t in range(steps):
+            env.step(acts[warmup + t])
+        torch.cuda.synchronize()
+        env.count_ray_steps(None)
+        env.close()
+        return counters.cpu().tolist()
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream(dev)
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -375,6 +402,8 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--obs", default="uint8", choices=["uint8", "float32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ray-count", action="store_true",
+                    help="skip the ray-step counting pass")
     ap.add_argument("--no-f32", action="store_true", help="skip the float32-obs side measurement")
     ap.add_argument("--terrain", default="flat", choices=["flat", "noise"],
                     help="world of the main measurement (default: the reference's flat world)")
@@ -411,6 +440,7 @@ def main():
     total_ms, wall, per_launch = run_gpu(w, args.obs, args.steps, args.warmup,
                                          terrain=args.terrain, collision=args.collision)
     K, B = args.steps, w["batch"]
+    pre = preroll_steps(w, args.warmup)
     # The reference's configuration: the flat world and its ground plane.
     ref_world = args.terrain == "flat" and not args.collision
     value = B * K / (total_ms / 1e3)
@@ -438,7 +468,11 @@ def main():
         "config": {"workload": describe(args.workload, w, args.obs), "batch": B,
                    "world": [w["world"]] * 3, "obs": [w["H"], w["W"], 3], "obs_dtype": args.obs,
                    "episode_length": w["episode_length"], "max_steps": 64, "look_delta": 0.1,
-                   "terrain": args.terrain, "collision": bool(args.collision)},
+                   "terrain": args.terrain, "collision": bool(args.collision),
+                   "preroll_steps": pre,
+                   "timed_window": f"steps {args.warmup + pre}..{args.warmup + pre + K - 1} after "
+                                   f"reset (untimed: {args.warmup} warmup + {pre} preroll, so "
+                                   "a done step with its world regeneration is timed)"},
         "wall_s": wall,
         "step_launch_ms": {"avg": avg_launch_ms, "p50": per_launch[len(per_launch) // 2],
                            "p99": per_launch[min(len(per_launch) - 1, int(0.99 * len(per_launch)))]},
@@ -450,6 +484,18 @@ def main():
         "reference_published": {"value": REFERENCE_C2, "hw": "RTX 3090", "obs": "float32",
                                 "source": "docs/benchmarks.md:34"},
     }
+    if not args.no_ray_count and ref_world:
+        # Same workload and window through the counting build of the step.
+        c = run_gpu(w, args.obs, K, args.warmup, count_rays=True)
+        out["ray_steps"] = {
+            "lookups_per_step": c[0] / K, "rays_marched_per_step": c[1] / K,
+            "pixels_per_step": c[2] / K,
+            "lookups_per_s": c[0] / K / (total_ms / K / 1e3),
+            "lookups_per_marched_ray": (c[0] - (c[2] - c[1])) / max(1, c[1]),
+            "definition": "DDA voxel lookups of every pixel (start voxel included; a pixel whose "
+                          "eye is inside a solid counts its one lookup), counted by a build of "
+                          "the step kernel over the same timed window; per second against the "
+                          "production kernel's timing"}
     if args.obs == "uint8" and not args.no_f32:
         f_ms, _, f_launch = run_gpu(w, "float32", min(K, 200), args.warmup)
         out["float32_obs_value"] = B * min(K, 200) / (f_ms / 1e3)
@@ -478,6 +524,7 @@ def main():
         out["served"]["closed_loop_frac_of_49_6M"] = closed / 49.6e6
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args.obs)
+        out["cpu_baseline_single_thread"] = cpu_baseline(w, args.obs, threads=1)
     print(json.dumps(out))
 
 

@@ -373,6 +373,15 @@ int mc_env_set_graph_mode(mc_env* env, int on, void* stream);
 int mc_env_reset(mc_env* env, const uint8_t* mask, void* stream);
 int mc_env_render(mc_env* env, void* stream);
 
+/* Measurement hook (no reference counterpart): with counters non-NULL (device,
+ * 3 x u64, accumulated, never cleared here) the group's steps run a build of the
+ * step kernel that adds [0] the DDA voxel lookups of every pixel (the start voxel
+ * included; a pixel whose eye sits inside a solid counts its one lookup), [1] the
+ * rays marched and [2] the pixels rendered.  Flat world, interleaved obs modes;
+ * envs run by the full-height pass are not counted.  NULL restores the
+ * production kernel.  Results are the same either way. */
+int mc_env_set_ray_counter(mc_env* env, unsigned long long* counters);
+
 /* Launch shape of a group's step: `threads` per workgroup (64, 256 or 512),
  * `waves_per_env` (1, 2, 4 or 8) 64-lane waves per env, so a workgroup holds
  * threads / (64 waves_per_env) envs, and `lift` (0/1; -1 keeps the current

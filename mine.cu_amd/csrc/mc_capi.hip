@@ -253,7 +253,10 @@ bool planar_mode(int obs_mode) {
 
 // Staged layers below Y (Params::ys) apply to step items with a deferred-env
 // list only; every other launch stages whole worlds.
-int launch_env(int kind, const mc::Params& q, int obs_mode, Shape s, hipStream_t st) {
+// count: the ray-step counting build of the step (flat world, interleaved
+// obs, p.ray_steps set).
+int launch_env(int kind, const mc::Params& q, int obs_mode, Shape s, hipStream_t st,
+               bool count = false) {
   mc::Params p = q;
   if (kind != mc::kItemStep || !p.fb || p.ys < 1 || p.ys > p.Y) {
     p.ys = p.Y;
@@ -267,7 +270,9 @@ int launch_env(int kind, const mc::Params& q, int obs_mode, Shape s, hipStream_t
       kind != mc::kItemRenderState)
     return fail(MC_E_UNSUPPORTED, "the MC_OBS_*_CHW modes render from MineEnv state only");
   const size_t lds = (size_t)mc::item_lds_bytes(epw_of(s), p.X, p.ys, p.Z);
-  const hipError_t e = mcl::kernels(s.nt, s.lift != 0).env(kind, p, obs_mode, s.wpe, lds, st);
+  const mcl::KernelSet& ks = mcl::kernels(s.nt, s.lift != 0);
+  const hipError_t e = count ? ks.env_count(p, obs_mode, s.wpe, lds, st)
+                             : ks.env(kind, p, obs_mode, s.wpe, lds, st);
   if (e != hipSuccess) return fail(MC_E_HIP, "env kernel launch: %s", hipGetErrorString(e));
   return MC_OK;
 }
@@ -327,6 +332,7 @@ struct mc_env {
   int shape_default;  // shape not set by the caller: steps size it for ys
   int ys;             // layers a step stages in LDS (world_y: all)
   uint32_t* fb;       // [3][B + 2] deferred-env lists (NULL when ys == world_y)
+  unsigned long long* ray_counter;  // non-NULL: steps run the ray-step counting build
 };
 
 extern "C" {
@@ -497,6 +503,7 @@ int mc_env_create(const mc_config* cfg, const mc_state* st, mc_env** out) {
   e->shape_default = 1;
   e->ys = staged_layers(*cfg);
   e->fb = nullptr;
+  e->ray_counter = nullptr;
   hipError_t he = hipMalloc(reinterpret_cast<void**>(&e->ring), MC_WORKSPACE_BYTES);
   if (he == hipSuccess) he = hipMemset(e->ring, 0, MC_WORKSPACE_BYTES);
   if (he == hipSuccess && e->ys < cfg->world_y) {
@@ -608,7 +615,8 @@ int mc_env_step(mc_env* env, const void* actions, int action_kind, const float* 
       s = Shape{d.nt, d.wpe, s.lift};
     }
   }
-  int r = launch_env(mc::kItemStep, p, env->cfg.obs_mode, s, S(stream));
+  p.ray_steps = env->ray_counter;
+  int r = launch_env(mc::kItemStep, p, env->cfg.obs_mode, s, S(stream), env->ray_counter != nullptr);
   if (!r && env->fb) r = launch_deferred(p, env->cfg.obs_mode, s.lift, S(stream));
   if (r) return r;
   if (env->graph_mode) {
@@ -616,6 +624,15 @@ int mc_env_step(mc_env* env, const void* actions, int action_kind, const float* 
     return check_launch("slot advance");
   }
   env->slot = (env->slot + 1) % 3;
+  return MC_OK;
+}
+
+int mc_env_set_ray_counter(mc_env* env, unsigned long long* counters) {
+  if (!env) return fail(MC_E_INVALID, "env is NULL");
+  if (counters && (env->cfg.terrain != MC_TERRAIN_FLAT || env->cfg.collision ||
+                   planar_mode(env->cfg.obs_mode)))
+    return fail(MC_E_UNSUPPORTED, "ray-step counting covers the flat-world interleaved modes");
+  env->ray_counter = counters;
   return MC_OK;
 }
 

@@ -166,6 +166,11 @@ struct Params {
   int collision;               // host side: 1 launches the k_*_col step kernels
   uint32_t terrain_seed;
   int terrain_amp;
+  // Ray-step counters of the counting step kernel (k_env<..., CNT = true>, a
+  // measurement build only): [0] DDA voxel lookups of every pixel (a pixel
+  // whose eye sits inside a solid counts its one lookup), [1] rays marched,
+  // [2] pixels.  Unused by every other kernel.
+  unsigned long long* ray_steps;
 };
 
 // Camera frame of one env, published by wave 0 through LDS.
@@ -739,10 +744,13 @@ __device__ __forceinline__ uint32_t oct_pos(const Params& p, const Frame& f, int
 // result is sky.  Those bounds tighten the per-axis limits of the packed
 // position, so the same test ends the ray there.
 // ---------------------------------------------------------------------------
-template <bool LIFT, bool VDF>
+// CNT: *nlook receives the ray's DDA voxel lookups, the start voxel included
+// (0 when the slab test culls the ray).
+template <bool LIFT, bool VDF, bool CNT = false>
 __device__ __forceinline__ void trace(const Params& p, const Frame& f, const OctPos& op, float u,
                                       float v, const int8_t* __restrict__ world,
-                                      const float* __restrict__ pal, float rgb[3]) {
+                                      const float* __restrict__ pal, float rgb[3],
+                                      int* nlook = nullptr) {
   float ray_x = f.fx + u * f.rx + v * f.ux;
   float ray_y = f.fy + v * f.uy;
   float ray_z = f.fz + u * f.rz + v * f.uz;
@@ -762,6 +770,7 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
   int hit_block = -1;
   int hit_face = -1;
   float t = 0.0f;
+  if (CNT) *nlook = 0;
   if (f.inside) {
     // dxp = dxn + 1 exactly: both differences are exact (Sterbenz), as
     // floor(e) <= e < floor(e) + 1.
@@ -805,6 +814,7 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
     // be merged into scalar lane masks on every iteration.
     const uint32_t moved = pos - op.reread(step_x > 0, step_y > 0, step_z > 0);
     const int steps = (int)((moved & 0x3ffu) + ((moved >> 10) & 0x3ffu) + (moved >> 20));
+    if (CNT) *nlook = steps + 1;
     const bool lim_end = (steps >= max_steps) | (((steps == 0) | !VDF) & (t >= vd));
     asm volatile("" : "+v"(blk));  // a fresh test, not the loop's lane mask
     if (!lim_end & ((pos & kGuard) == 0u) & (blk >= 0)) {
@@ -860,6 +870,7 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
         hit_block = blk;
         hit_face = face;
       }
+      if (CNT) *nlook = step + 1;
     }
   }
   if ((unsigned)hit_block < 13u) {
@@ -894,9 +905,10 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
 // Otherwise a ray outside the world that moves away from it on some axis
 // can never come back, which ends it too.
 // ---------------------------------------------------------------------------
-template <bool LIFT, typename T>
+template <bool LIFT, typename T, bool CNT = false>
 __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, const OctPos& op,
-                                          float u, float v, const int8_t* __restrict__ world) {
+                                          float u, float v, const int8_t* __restrict__ world,
+                                          int* nlook = nullptr) {
   constexpr float kInf = (sizeof(T) == 2) ? 65504.0f : 1e30f;
   float ray_x = f.fx + u * f.rx + v * f.ux;
   float ray_y = f.fy + v * f.uy;
@@ -940,6 +952,7 @@ __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, const
     // of pos's fields' advances.
     const uint32_t moved = pos - op.reread(step_x > 0, step_y > 0, step_z > 0);
     const int steps = (int)((moved & 0x3ffu) + ((moved >> 10) & 0x3ffu) + (moved >> 20));
+    if (CNT) *nlook = steps + 1;
     asm volatile("" : "+v"(blk));  // a fresh test, not the loop's lane mask
     return ((steps < max_steps) & ((pos & kGuard) == 0u) & (blk >= 0)) ? blk : -1;
   }
@@ -947,7 +960,9 @@ __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, const
   T tm_x = (T)((ray_x >= 0) ? ((float)(vx + 1) - f.ex) * inv_x : ((float)vx - f.ex) * inv_x);
   T tm_y = (T)((ray_y >= 0) ? ((float)(vy + 1) - f.ey) * inv_y : ((float)vy - f.ey) * inv_y);
   T tm_z = (T)((ray_z >= 0) ? ((float)(vz + 1) - f.ez) * inv_z : ((float)vz - f.ez) * inv_z);
+  if (CNT) *nlook = 0;
   for (int step = 0; step < max_steps; ++step) {
+    if (CNT) *nlook = step + 1;
     const bool inb = (unsigned)vx < (unsigned)X && (unsigned)vy < (unsigned)Y &&
                      (unsigned)vz < (unsigned)Z;
     if (inb) {
@@ -1109,21 +1124,33 @@ __device__ __forceinline__ void render_rgb8(const Params& p, int b, int uvm, int
 // 568-700 prebasis, 835-982 smem: same image as prebasis, 1127-1262 with a
 // binary16 DDA when T = _Float16).  flat: the packed colour table, 13 blocks
 // then the sky; ids >= 13 show the sky.
-template <bool LIFT, typename T>
+// CNT: acc[0] += DDA lookups, acc[1] += rays marched, acc[2] += pixels.
+template <bool LIFT, typename T, bool CNT = false>
 __device__ __forceinline__ void render_env_flat(const Params& p, int b, const int8_t* world,
                                                 const uint32_t* flat, const Frame& f,
                                                 const OctPos& op, int uvm, int tid, int nt,
-                                                const float* su, const float* sv) {
+                                                const float* su, const float* sv,
+                                                uint32_t* acc = nullptr) {
   const int npix = p.W * p.H;
   uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
   const bool per_env_scale = (uvm == kUvRaw);
   auto color = [&](float u, float v) -> uint32_t {
     if (per_env_scale) { u = u * f.su; v = v * f.sv; }
-    const int h = (f.flat_fill >= 0) ? f.flat_fill : trace_flat<LIFT, T>(p, f, op, u, v, world);
+    int h;
+    if (f.flat_fill >= 0) {
+      h = f.flat_fill;
+      if (CNT) acc[0] += 1;
+    } else {
+      int n = 0;
+      h = trace_flat<LIFT, T, CNT>(p, f, op, u, v, world, &n);
+      if (CNT) acc[0] += n, acc[1] += 1;
+    }
+    if (CNT) acc[2] += 1;
     return flat[((unsigned)h < (unsigned)kFlatSky) ? h : kFlatSky];
   };
   if (f.flat_fill >= 0 && ((npix * 3) & 15) == 0) {
     fill_rgb8(out, npix, flat[f.flat_fill < kFlatSky ? f.flat_fill : kFlatSky], tid, nt);
+    if (CNT && tid == 0) acc[0] += npix, acc[2] += npix;
     return;
   }
   render_rgb8(p, b, uvm, tid, nt, su, sv, color);
@@ -1131,11 +1158,11 @@ __device__ __forceinline__ void render_env_flat(const Params& p, int b, const in
 
 // planar: the image is written [3, H, W] (the MC_OBS_*_CHW modes).  Callers
 // pass a compile-time constant, so each kernel holds one store form only.
-template <int OBS, bool LIFT, bool VDF>
+template <int OBS, bool LIFT, bool VDF, bool CNT = false>
 __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t* world,
                                            const float* pal, const Frame& f, const OctPos& op,
                                            int tid, int nt, const float* su, const float* sv,
-                                           bool planar) {
+                                           bool planar, uint32_t* acc = nullptr) {
   const int W = p.W, npix = W * p.H;
   const bool tab = uv_tabled(p);
   auto ucol = [&](int px) { return tab ? su[px] : u_of(p, kUvShaded, px); };
@@ -1147,6 +1174,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
                 to_u8(pal[f.fill_block * 3]) | (to_u8(pal[f.fill_block * 3 + 1]) << 8) |
                     (to_u8(pal[f.fill_block * 3 + 2]) << 16),
                 tid, nt);
+      if (CNT && tid == 0) acc[0] += npix, acc[2] += npix;
       return;
     }
     // Packed 0x00BBGGRR per pixel.
@@ -1154,9 +1182,13 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       float rgb[3];
       if (f.fill_block >= 0) {
         for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
+        if (CNT) acc[0] += 1;
       } else {
-        trace<LIFT, VDF>(p, f, op, u, v, world, pal, rgb);
+        int n = 0;
+        trace<LIFT, VDF, CNT>(p, f, op, u, v, world, pal, rgb, &n);
+        if (CNT) acc[0] += n, acc[1] += 1;
       }
+      if (CNT) acc[2] += 1;
       return to_u8(rgb[0]) | (to_u8(rgb[1]) << 8) | (to_u8(rgb[2]) << 16);
     };
     render_rgb8(p, b, kUvShaded, tid, nt, su, sv, color, planar);
@@ -1167,9 +1199,13 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       float rgb[3];
       if (f.fill_block >= 0) {
         for (int c = 0; c < 3; ++c) rgb[c] = pal[f.fill_block * 3 + c];
+        if (CNT) acc[0] += 1;
       } else {
-        trace<LIFT, VDF>(p, f, op, ucol(px), vrow(py), world, pal, rgb);
+        int n = 0;
+        trace<LIFT, VDF, CNT>(p, f, op, ucol(px), vrow(py), world, pal, rgb, &n);
+        if (CNT) acc[0] += n, acc[1] += 1;
       }
+      if (CNT) acc[2] += 1;
       if (planar) {
         gf32* o = (gf32*)(out + (int64_t)py * W + px);
         o[0] = rgb[0];
@@ -1591,7 +1627,9 @@ __host__ __device__ __forceinline__ int item_lds_bytes(int epw, int X, int Y, in
 }
 
 // upper: one word per wave of the workgroup (staged-height check).
-template <int NT, int KIND, bool LIFT, bool CHW = false, bool TER = false, bool COL = false>
+// CNT: count ray steps into p.ray_steps (the measurement build of the step).
+template <int NT, int KIND, bool LIFT, bool CHW = false, bool TER = false, bool COL = false,
+          bool CNT = false>
 __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, int wpe,
                                          int8_t* lds, const float* pal, const uint32_t* flat,
                                          Frame* frames, float* su, float* sv, int* upper) {
@@ -1669,7 +1707,9 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
                             : (flat_img || obs_mode != kObsF32);
   if (go && ((npix * 3) & 15) == 0 && can_fill)
     fill = __builtin_amdgcn_readfirstlane(flat_img ? frames[e].flat_fill : frames[e].fill_block);
+  uint32_t acc[3] = {0u, 0u, 0u};
   if (fill >= 0) {
+    if (CNT && et == 0) acc[0] += npix, acc[2] += npix;
     const uint32_t rgb = flat_img ? flat[fill < kFlatSky ? fill : kFlatSky]
                                   : to_u8(pal[fill * 3]) | (to_u8(pal[fill * 3 + 1]) << 8) |
                                         (to_u8(pal[fill * 3 + 2]) << 16);
@@ -1687,14 +1727,25 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     const OctPos op{frames[e].pos0};
     const bool f32 = obs_mode == (CHW ? kObsF32Chw : kObsF32);
     if (uvm != kUvShaded && obs_mode == kObsU8Fp16)
-      render_env_flat<LIFT, _Float16>(p, b, world, flat, f, op, uvm, et, ent, su, sv);
+      render_env_flat<LIFT, _Float16, CNT>(p, b, world, flat, f, op, uvm, et, ent, su, sv, acc);
     else if (uvm != kUvShaded)
-      render_env_flat<LIFT, float>(p, b, world, flat, f, op, uvm, et, ent, su, sv);
-    else if (f32 && p.vd_far) render_env<1, LIFT, true>(p, b, world, pal, f, op, et, ent, su, sv, CHW);
-    else if (f32) render_env<1, LIFT, false>(p, b, world, pal, f, op, et, ent, su, sv, CHW);
-    else if (p.vd_far) render_env<0, LIFT, true>(p, b, world, pal, f, op, et, ent, su, sv, CHW);
-    else render_env<0, LIFT, false>(p, b, world, pal, f, op, et, ent, su, sv, CHW);
+      render_env_flat<LIFT, float, CNT>(p, b, world, flat, f, op, uvm, et, ent, su, sv, acc);
+    else if (f32 && p.vd_far)
+      render_env<1, LIFT, true, CNT>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
+    else if (f32) render_env<1, LIFT, false, CNT>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
+    else if (p.vd_far)
+      render_env<0, LIFT, true, CNT>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
+    else render_env<0, LIFT, false, CNT>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
     if (KIND == kItemStep && any_done) store_fresh_world<TER>(p, b, gworld, nullptr, nvox, et, ent);
+  }
+  if (CNT && p.ray_steps) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      uint32_t v = acc[k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      if ((tid & (kWave - 1)) == 0 && v) atomicAdd(&p.ray_steps[k], (unsigned long long)v);
+    }
   }
 }
 
@@ -1703,8 +1754,9 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
 // ---------------------------------------------------------------------------
 
 // One group, one item per workgroup: grid = ceil(B / epw).  CHW: the planar
-// image modes, in kernels of their own.
-template <int NT, int KIND, bool LIFT, bool CHW = false>
+// image modes, in kernels of their own.  CNT: the measurement build of the
+// step, counting ray steps into p.ray_steps.
+template <int NT, int KIND, bool LIFT, bool CHW = false, bool CNT = false>
 __global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
@@ -1714,8 +1766,8 @@ __global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
   __shared__ int s_upper[NT / kWave];
   load_palette(s_pal, s_flat, threadIdx.x);
   const int epw = (NT / kWave) / wpe;
-  run_item<NT, KIND, LIFT, CHW>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_flat,
-                                s_frames, s_u, s_v, s_upper);
+  run_item<NT, KIND, LIFT, CHW, false, false, CNT>(p, obs_mode, blockIdx.x * epw, wpe, s_lds,
+                                                  s_pal, s_flat, s_frames, s_u, s_v, s_upper);
 }
 
 // Full-height pass after a staged-height step launch (Params::ys): the envs

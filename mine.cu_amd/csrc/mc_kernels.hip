@@ -53,6 +53,13 @@ hipError_t env(int kind, const mc::Params& p, int obs_mode, int wpe, size_t lds,
   return hipGetLastError();
 }
 
+hipError_t env_count(const mc::Params& p, int obs_mode, int wpe, size_t lds, hipStream_t st) {
+  const int epw = (NT / mc::kWave) / wpe;
+  hipLaunchKernelGGL((mc::k_env<NT, mc::kItemStep, LIFT, false, true>), dim3((p.B + epw - 1) / epw),
+                     dim3(NT), lds, st, p, obs_mode, wpe);
+  return hipGetLastError();
+}
+
 #if MC_NT >= 256
 hipError_t fb(const mc::Params& p, int obs_mode, dim3 grid, size_t lds, hipStream_t st) {
   if (p.collision && p.terrain)
@@ -97,9 +104,9 @@ int multi_blocks(size_t lds) {
 
 const KernelSet& MC_SET_NAME(MC_NT, MC_LIFT)() {
 #if MC_NT >= 256
-  static const KernelSet set = {env, fb, multi, multi_fb, multi_blocks};
+  static const KernelSet set = {env, env_count, fb, multi, multi_fb, multi_blocks};
 #else
-  static const KernelSet set = {env, nullptr, nullptr, nullptr, nullptr};
+  static const KernelSet set = {env, env_count, nullptr, nullptr, nullptr, nullptr};
 #endif
   return set;
 }

@@ -19,6 +19,9 @@ struct KernelSet {
   // planar) and the world (flat or seeded terrain, plane or voxel collision).
   hipError_t (*env)(int kind, const mc::Params& p, int obs_mode, int wpe, size_t lds,
                     hipStream_t st);
+  // The step of a flat-world interleaved group with ray-step counting
+  // (p.ray_steps; a measurement build of the step kernel).
+  hipError_t (*env_count)(const mc::Params& p, int obs_mode, int wpe, size_t lds, hipStream_t st);
   // Full-height pass over the envs a staged-height step deferred (NULL for
   // the 64-thread set).
   hipError_t (*fb)(const mc::Params& p, int obs_mode, dim3 grid, size_t lds, hipStream_t st);

@@ -371,6 +371,21 @@ class MineEnv:
                    "set_graph_mode")
         self._graph_mode = bool(on)
 
+    def count_ray_steps(self, counters: Optional[torch.Tensor]) -> None:
+        """Measurement hook: with an int64 [3] device tensor, every later step
+        runs a build of the step kernel that adds to it [0] the DDA voxel
+        lookups of every pixel, [1] the rays marched, [2] the pixels rendered
+        (flat world, interleaved layouts).  None restores the production
+        kernel.  Observations and state are the same either way."""
+        ptr = None
+        if counters is not None:
+            if (counters.dtype != torch.int64 or tuple(counters.shape) != (3,)
+                    or counters.device != self.device or not counters.is_contiguous()):
+                raise ValueError("counters must be a contiguous int64 [3] tensor on the env's device")
+            ptr = ctypes.c_void_p(counters.data_ptr())
+        self._ray_counters = counters
+        _lib.check(_lib.load().mc_env_set_ray_counter(self._handle, ptr), "count_ray_steps")
+
     def capture_step(self, actions: torch.Tensor, look: Optional[torch.Tensor] = None):
         """Capture one step on static input buffers into a torch CUDA graph
         (the reference captures its kernel sequence into a CUDA graph,
@@ -417,7 +432,9 @@ class MineEnv:
 
     # -- checkpoint / resume ---------------------------------------------------
     _STATE_KEYS = ("voxels", "positions", "velocities", "yaws", "pitches", "on_ground",
-                   "step_count", "do_reset", "cameras", "rewards", "obs_buffer")
+                   "step_count", "do_reset", "cameras", "rewards", "obs_buffer", "forward_in",
+                   "strafe_in", "delta_yaw_in", "delta_pitch_in", "jump_in", "do_break",
+                   "do_place", "speed_mult")
 
     def _state_keys(self):
         return self._STATE_KEYS + (("basis",) if self.basis is not None else ())

@@ -107,6 +107,7 @@ SIGNATURES = {
     "mc_server_autotune": ([_P, _I, _P], _I),
     "mc_debug_sincosf": ([_P, _P, _P, _I, _P], _I),
     "mc_env_set_graph_mode": ([_P, _I, _P], _I),
+    "mc_env_set_ray_counter": ([_P, _P], _I),
     "mc_debug_f16": ([_P, _P, _P, _P, _P, _I, _P], _I),
     "mc_debug_fast_math": ([_I, _U, _U, _P, _P], _I),
 }

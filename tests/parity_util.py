@@ -121,3 +121,28 @@ def run_pair(env, orc, actions, stats: Stats, tag: str, look=None, check_every=1
         if (t % check_every) == 0 or t == len(actions) - 1:
             compare_env(env, orc, stats, where)
     return stats
+
+
+SAMPLED_KEYS = ("voxels", "positions", "velocities", "yaws", "pitches", "on_ground", "step_count",
+                "cameras", "rewards", "obs_buffer") + DECODE_BUFFERS
+
+
+def compare_sampled(env, idx, orc, stats: Stats, where: str):
+    """Envs idx of a (large) MineEnv against an OracleEnv that stepped exactly
+    those envs (envs are independent, and episodes run in lockstep from one
+    reset, so the group-wide regeneration happens on the same steps)."""
+    import torch
+    ti = torch.as_tensor(idx, device=env.device)
+    for k in SAMPLED_KEYS:
+        g = getattr(env, k).index_select(0, ti).cpu().numpy()
+        o = getattr(orc, k)
+        if k == "obs_buffer":
+            compare_obs(g, o, stats, where)
+        elif g.dtype in (np.float32,) and k in ("positions", "velocities", "yaws", "pitches",
+                                                 "cameras"):
+            compare_float(k, g, o, stats, where)
+        elif g.dtype == np.float32:
+            compare_exact(k, g.view(np.int32), o.view(np.int32), where)
+        else:
+            compare_exact(k, g.astype(np.int64) if g.dtype == np.bool_ else g,
+                          o.astype(np.int64) if g.dtype == np.bool_ else o, where)
