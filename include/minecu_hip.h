@@ -186,6 +186,22 @@ int mc_render(const int8_t* voxels, const float* cameras, void* obs, int obs_mod
 int mc_step(const mc_config* cfg, const mc_state* st, const void* actions, int action_kind,
             const float* look, uint8_t* dones_out, void* workspace, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory or pass a workspace: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  mc_step without the workspace argument, with
+ * the same results bit for bit: the episode flag ring and a deferred-env list
+ * (worlds taller than the layers a step stages in LDS run their full-height pass
+ * from it, as MineEnv's steps do) live in a file-scope device array of about
+ * 1 MiB per GPU, held until the process ends.  batch_size above
+ * MC_STEP_STATIC_MAX_BATCH is refused (MC_E_INVALID).  No call allocates; the
+ * first call on a device looks the array's address up (hipGetSymbolAddress), so
+ * make it outside a graph capture.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one device (different streams, threads or
+ * processes' graphs) race and must not. */
+#define MC_STEP_STATIC_MAX_BATCH 262142
+int mc_step_static(const mc_config* cfg, const mc_state* st, const void* actions, int action_kind,
+                   const float* look, uint8_t* dones_out, void* stream);
+
 /* MineEnv.reset(mask) (minecu/__init__.py:270-302) in two launches: masked
  * respawn, step_count = 0, regeneration of every world when mask is NULL or
  * any mask byte is set, camera update and render. */

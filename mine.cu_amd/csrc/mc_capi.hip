@@ -11,6 +11,7 @@
 #include <atomic>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <tuple>
 #include <vector>
@@ -454,6 +455,90 @@ int mc_step(const mc_config* cfg, const mc_state* st, const void* actions, int a
   r = check_launch("episode pre-pass");
   if (r) return r;
   return launch_env(mc::kItemStep, p, cfg->obs_mode, default_shape(*cfg), S(stream));
+}
+
+namespace {
+// Scratch of mc_step_static: the episode flag ring (the MC_WORKSPACE_BYTES
+// that mc_step takes from its caller) followed by one deferred-env list,
+// [count, finished workgroups, env...], of at most MC_STEP_STATIC_MAX_BATCH
+// envs.  One copy per GPU (a file-scope device array), about 1 MiB, held until
+// the process ends.  In host code `scratch` only names the symbol: its device
+// address comes from hipGetSymbolAddress, once per device.
+constexpr int kStaticRingWords = MC_WORKSPACE_BYTES / 4;
+constexpr size_t kStaticScratchWords = kStaticRingWords + (size_t)MC_STEP_STATIC_MAX_BATCH + 2;
+static __device__ __align__(256) uint32_t scratch[kStaticScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::atomic<uint32_t*> g_scratch_ptr[kMaxDevices];
+std::mutex g_scratch_mu;
+
+// Device address of the current device's scratch (looked up on its first use).
+int static_scratch(uint32_t** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(MC_E_HIP, "mc_step_static: hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(MC_E_UNSUPPORTED, "mc_step_static: device %d beyond %d", dev, kMaxDevices);
+  uint32_t* ptr = g_scratch_ptr[dev].load(std::memory_order_acquire);
+  if (!ptr) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    ptr = g_scratch_ptr[dev].load(std::memory_order_relaxed);
+    if (!ptr) {
+      void* addr = nullptr;
+      e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !addr)
+        return fail(MC_E_HIP, "mc_step_static: scratch address: %s", hipGetErrorString(e));
+      ptr = static_cast<uint32_t*>(addr);
+      g_scratch_ptr[dev].store(ptr, std::memory_order_release);
+    }
+  }
+  *out = ptr;
+  return MC_OK;
+}
+}  // namespace
+
+int mc_step_static(const mc_config* cfg, const mc_state* st, const void* actions, int action_kind,
+                   const float* look, uint8_t* dones_out, void* stream) {
+  if (cfg && cfg->batch_size > MC_STEP_STATIC_MAX_BATCH)
+    return fail(MC_E_INVALID, "mc_step_static: batch_size %d exceeds its scratch limit of %d envs",
+                cfg->batch_size, MC_STEP_STATIC_MAX_BATCH);
+  int r = validate_config(cfg);
+  if (r) return r;
+  r = validate_state(st);
+  if (r) return r;
+  if (!actions) return fail(MC_E_INVALID, "actions is NULL");
+  if (action_kind != MC_ACT_BUTTONS && action_kind != MC_ACT_DISCRETE)
+    return fail(MC_E_INVALID, "bad action_kind %d", action_kind);
+  if (action_kind == MC_ACT_BUTTONS && !look) return fail(MC_E_INVALID, "look is NULL");
+  uint32_t* scr = nullptr;
+  r = static_scratch(&scr);
+  if (r) return r;
+  mc::Params p = make_params(cfg, st);
+  p.actions = actions;
+  p.action_kind = action_kind;
+  p.look = look;
+  p.dones = dones_out;
+  p.ring = scr;
+  p.slot = 0;
+  // Staged height, as MineEnv's steps run it: envs the staged launch cannot
+  // run are listed in the scratch's deferred-env list for the full-height pass.
+  const mc_config& c = *cfg;
+  const int ys = staged_layers(c);
+  Shape s = default_shape(c);
+  if (ys < c.world_y) {
+    p.ys = ys;
+    p.fb = scr + kStaticRingWords;
+    const Shape d = default_shape(c.width, c.height, c.world_x, ys, c.world_z);
+    s = Shape{d.nt, d.wpe, s.lift};
+  }
+  // The ring words and the list header start each call at zero (the set-up
+  // kernel writes them; the array keeps the last call's contents).
+  hipLaunchKernelGGL(mc::k_ring_setup, dim3(1), dim3(1024), 0, S(stream), p, 0);
+  r = check_launch("episode pre-pass");
+  if (r) return r;
+  r = launch_env(mc::kItemStep, p, c.obs_mode, s, S(stream));
+  if (!r && p.fb) r = launch_deferred(p, c.obs_mode, s.lift, S(stream));
+  return r;
 }
 
 namespace {

@@ -9,7 +9,9 @@
 namespace mc {
 
 // Episode-ring set-up, one workgroup of 1024 threads.
-//   mode 0: ring[slot] = any(step_count >= L), ring[(slot+1)%3] = 0
+//   mode 0: ring[slot] = any(step_count >= L), ring[(slot+1)%3] = 0, and when
+//           a deferred-env list is given (p.fb), its slot-0 header words
+//           (count, finished workgroups) = 0
 //   mode 1 (before a reset): ring[3] = any(mask) (1 when mask is NULL),
 //           ring[slot] = ring[(slot+1)%3] = 0
 __global__ __launch_bounds__(1024) void k_ring_setup(Params p, int mode) {
@@ -27,6 +29,10 @@ __global__ __launch_bounds__(1024) void k_ring_setup(Params p, int mode) {
   if (threadIdx.x == 0) {
     if (mode == 0) {
       p.ring[slot] = any ? 1u : 0u;
+      if (p.fb) {
+        p.fb[0] = 0u;
+        p.fb[1] = 0u;
+      }
     } else {
       p.ring[3] = any ? 1u : 0u;
       p.ring[slot] = 0u;

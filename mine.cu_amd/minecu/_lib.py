@@ -20,6 +20,7 @@ MC_OBS_U8_CHW, MC_OBS_F32_CHW = 5, 6
 MC_TERRAIN_FLAT, MC_TERRAIN_NOISE = 0, 1
 MC_ACT_BUTTONS, MC_ACT_DISCRETE = 0, 1
 MC_WORKSPACE_BYTES = 256
+MC_STEP_STATIC_MAX_BATCH = 262142
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -60,6 +61,7 @@ SIGNATURES = {
     "mc_reset": ([_P, _P, _P, _P, _P, _P, _I, _F, _F, _F, _P], _I),
     "mc_render": ([_P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _F, _F, _P], _I),
     "mc_step": ([_P, _P, _P, _I, _P, _P, _P, _P], _I),
+    "mc_step_static": ([_P, _P, _P, _I, _P, _P, _P], _I),
     "mc_env_reset_state": ([_P, _P, _P, _P, _P], _I),
     "mc_render_state": ([_P, _P, _P], _I),
     "mc_launch_step_uint8": ([_P] * 20 + [_I] * 5 + [_F] * 7 + [ctypes.c_int8, _F, _F, _F, _F, _P], _I),

@@ -63,6 +63,12 @@ def test_null_and_range_errors():
     assert L.mc_env_create(ctypes.byref(cfg), ctypes.byref(st), ctypes.byref(out)) == _l.MC_E_INVALID
     assert "non-NULL" in _err(L) and not out.value
     assert L.mc_env_step(None, v, 0, v, None, None) == _l.MC_E_INVALID
+    # mc_step_static: a batch beyond its scratch limit is refused before any device call
+    big = _l.McConfig(batch_size=_l.MC_STEP_STATIC_MAX_BATCH + 1, world_x=16, world_y=16,
+                      world_z=16, width=8, height=8)
+    assert L.mc_step_static(ctypes.byref(big), ctypes.byref(st), v, 1, None, None,
+                            None) == _l.MC_E_INVALID
+    assert "scratch limit" in _err(L)
     assert L.mc_debug_sincosf(None, None, None, 4, None) == _l.MC_E_INVALID
     # mc_render takes obs modes 0..4 (flat variants 2..4); the planar modes 5..6 render
     # from MineEnv state only, 7 is no mode; NULL basis is refused
