@@ -707,6 +707,127 @@ struct OctPos {
   }
 };
 
+// The in-world DDA loop (the iterations of trace()'s in-world branch) for the
+// lanes of `run`; the other active lanes keep their values.  Per iteration and
+// lane: t = min(t_max); the axis is z when t_max_z == t, else y when
+// t_max_y == t, else x (the reference's tie order, kernels.cu:424-441);
+// t_max of that axis grows by |inv|, the LDS address va by that axis's byte
+// step, the packed position pos by pinc (1, 1 << 10 or 1 << 20: also the
+// axis, returned); then the voxel at va is read.  A lane stops when it leaves
+// the world (a guard bit of pos sets), when the voxel is not air (signed
+// value >= 0) or, without VDF, when t >= vd; every lane stops after
+// max_steps iterations.  blk returns the last voxel read (unsigned byte).
+//
+// Written as one block so the loop control is a uniform counter and an exec
+// update per iteration (the compiler's divergent-loop form spends about twice
+// the scalar instructions); the arithmetic is the same IEEE binary32 adds
+// and compares, so results are unchanged bit for bit.  Every operand of a
+// vector instruction is a VGPR or an inline constant except at most one
+// SGPR (a lane mask, the guard word or vd); a lane mask written by a vector
+// compare is read by vector instructions no earlier than three instructions
+// later.  exec is restored before the block ends.
+template <bool VDF>
+__device__ __forceinline__ void dda_loop(float& tx, float& ty, float& tz, float& t, float ix,
+                                         float iy, float iz, uint32_t& va, uint32_t dx,
+                                         uint32_t dy, uint32_t dz, uint32_t& pos, uint32_t& pinc,
+                                         uint32_t& blk, uint64_t run, int max_steps, float vd) {
+  uint64_t save, mz, my, mc, mv;
+  uint32_t d, w, step;
+  float nt;
+  const uint32_t k10 = 1u << 10, k20 = 1u << 20, kg = (1u << 9) | (1u << 19) | (1u << 29);
+  if (VDF) {
+    asm volatile(
+        "s_mov_b64 %[save], exec\n\t"
+        "s_and_b64 exec, exec, %[run]\n\t"
+        "s_mov_b32 %[step], 0\n\t"
+        "s_cbranch_execz .Ldda_end%=\n"
+        ".Ldda_loop%=:\n\t"
+        "v_min3_f32 %[t], %[tx], %[ty], %[tz]\n\t"
+        "s_add_u32 %[step], %[step], 1\n\t"
+        "v_cmp_eq_f32_e64 %[mz], %[tz], %[t]\n\t"
+        "v_cmp_eq_f32_e64 %[my], %[ty], %[t]\n\t"
+        "v_add_f32_e64 %[nt], %[tx], |%[ix]|\n\t"
+        "v_add_f32_e64 %[w], %[ty], |%[iy]|\n\t"
+        "s_andn2_b64 %[my], %[my], %[mz]\n\t"
+        "s_or_b64 %[mc], %[my], %[mz]\n\t"
+        "v_cndmask_b32_e64 %[tx], %[nt], %[tx], %[mc]\n\t"
+        "v_cndmask_b32_e64 %[ty], %[ty], %[w], %[my]\n\t"
+        "v_add_f32_e64 %[nt], %[tz], |%[iz]|\n\t"
+        "v_cndmask_b32_e64 %[tz], %[tz], %[nt], %[mz]\n\t"
+        "v_cndmask_b32_e64 %[d], %[dx], %[dy], %[my]\n\t"
+        "v_cndmask_b32_e64 %[d], %[d], %[dz], %[mz]\n\t"
+        "v_add_u32_e32 %[va], %[va], %[d]\n\t"
+        "ds_read_u8 %[blk], %[va]\n\t"
+        "v_cndmask_b32_e64 %[pi], 1, %[k10], %[my]\n\t"
+        "v_cndmask_b32_e64 %[pi], %[pi], %[k20], %[mz]\n\t"
+        "v_add_u32_e32 %[pos], %[pos], %[pi]\n\t"
+        "v_and_b32_e32 %[w], %[kg], %[pos]\n\t"
+        "s_waitcnt lgkmcnt(0)\n\t"
+        "v_sub_u32_e32 %[w], %[blk], %[w]\n\t"
+        "v_cmp_lt_i32_e32 vcc, 0x7f, %[w]\n\t"
+        "s_and_b64 exec, exec, vcc\n\t"
+        "s_cmp_ge_i32 %[step], %[ms]\n\t"
+        "s_cbranch_scc1 .Ldda_end%=\n\t"
+        "s_cbranch_execnz .Ldda_loop%=\n"
+        ".Ldda_end%=:\n\t"
+        "s_mov_b64 exec, %[save]"
+        : [tx] "+v"(tx), [ty] "+v"(ty), [tz] "+v"(tz), [t] "+v"(t), [va] "+v"(va),
+          [pos] "+v"(pos), [pi] "+v"(pinc), [blk] "+v"(blk), [d] "=&v"(d), [w] "=&v"(w),
+          [nt] "=&v"(nt), [save] "=&s"(save), [mz] "=&s"(mz), [my] "=&s"(my), [mc] "=&s"(mc),
+          [step] "=&s"(step)
+        : [ix] "v"(ix), [iy] "v"(iy), [iz] "v"(iz), [dx] "v"(dx), [dy] "v"(dy), [dz] "v"(dz),
+          [k10] "v"(k10), [k20] "v"(k20), [kg] "s"(kg), [run] "s"(run), [ms] "s"(max_steps)
+        : "vcc", "scc");
+    (void)mv;
+  } else {
+    asm volatile(
+        "s_mov_b64 %[save], exec\n\t"
+        "s_and_b64 exec, exec, %[run]\n\t"
+        "s_mov_b32 %[step], 0\n\t"
+        "s_cbranch_execz .Ldda_end%=\n"
+        ".Ldda_loop%=:\n\t"
+        "v_min3_f32 %[t], %[tx], %[ty], %[tz]\n\t"
+        "s_add_u32 %[step], %[step], 1\n\t"
+        "v_cmp_eq_f32_e64 %[mz], %[tz], %[t]\n\t"
+        "v_cmp_eq_f32_e64 %[my], %[ty], %[t]\n\t"
+        "v_cmp_gt_f32_e64 %[mv], %[vd], %[t]\n\t"
+        "v_add_f32_e64 %[nt], %[tx], |%[ix]|\n\t"
+        "v_add_f32_e64 %[w], %[ty], |%[iy]|\n\t"
+        "s_andn2_b64 %[my], %[my], %[mz]\n\t"
+        "s_or_b64 %[mc], %[my], %[mz]\n\t"
+        "v_cndmask_b32_e64 %[tx], %[nt], %[tx], %[mc]\n\t"
+        "v_cndmask_b32_e64 %[ty], %[ty], %[w], %[my]\n\t"
+        "v_add_f32_e64 %[nt], %[tz], |%[iz]|\n\t"
+        "v_cndmask_b32_e64 %[tz], %[tz], %[nt], %[mz]\n\t"
+        "v_cndmask_b32_e64 %[d], %[dx], %[dy], %[my]\n\t"
+        "v_cndmask_b32_e64 %[d], %[d], %[dz], %[mz]\n\t"
+        "v_add_u32_e32 %[va], %[va], %[d]\n\t"
+        "ds_read_u8 %[blk], %[va]\n\t"
+        "v_cndmask_b32_e64 %[pi], 1, %[k10], %[my]\n\t"
+        "v_cndmask_b32_e64 %[pi], %[pi], %[k20], %[mz]\n\t"
+        "v_add_u32_e32 %[pos], %[pos], %[pi]\n\t"
+        "v_and_b32_e32 %[w], %[kg], %[pos]\n\t"
+        "s_waitcnt lgkmcnt(0)\n\t"
+        "v_sub_u32_e32 %[w], %[blk], %[w]\n\t"
+        "v_cmp_lt_i32_e32 vcc, 0x7f, %[w]\n\t"
+        "s_and_b64 exec, exec, vcc\n\t"
+        "s_and_b64 exec, exec, %[mv]\n\t"
+        "s_cmp_ge_i32 %[step], %[ms]\n\t"
+        "s_cbranch_scc1 .Ldda_end%=\n\t"
+        "s_cbranch_execnz .Ldda_loop%=\n"
+        ".Ldda_end%=:\n\t"
+        "s_mov_b64 exec, %[save]"
+        : [tx] "+v"(tx), [ty] "+v"(ty), [tz] "+v"(tz), [t] "+v"(t), [va] "+v"(va),
+          [pos] "+v"(pos), [pi] "+v"(pinc), [blk] "+v"(blk), [d] "=&v"(d), [w] "=&v"(w),
+          [nt] "=&v"(nt), [save] "=&s"(save), [mz] "=&s"(mz), [my] "=&s"(my), [mc] "=&s"(mc),
+          [mv] "=&s"(mv), [step] "=&s"(step)
+        : [ix] "v"(ix), [iy] "v"(iy), [iz] "v"(iz), [dx] "v"(dx), [dy] "v"(dy), [dz] "v"(dz),
+          [k10] "v"(k10), [k20] "v"(k20), [kg] "s"(kg), [run] "s"(run), [ms] "s"(max_steps),
+          [vd] "s"(vd)
+        : "vcc", "scc");
+  }
+}
+
 template <bool LIFT>
 __device__ __forceinline__ uint32_t oct_pos(const Params& p, const Frame& f, int k) {
   const int X = p.X, Y = p.Y, Z = p.Z;
@@ -779,35 +900,20 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
     float tm_z = (f.dzn + ((ray_z >= 0) ? 1.0f : 0.0f)) * inv_z;
     constexpr uint32_t kGuard = (1u << 9) | (1u << 19) | (1u << 29);
     uint32_t pos = op.pick(step_x > 0, step_y > 0, step_z > 0);
-    // LDS address of the current voxel.
-    const int8_t* vp = world + (f.ivy * XZ + f.ivx * Z + f.ivz);
-    bool out = (pos & kGuard) != 0u;
-    int blk = (int)*vp;
-    bool lim = (max_steps <= 0) | (t >= vd);
+    // LDS address of the current voxel, as a 32-bit LDS offset.
+    typedef __attribute__((address_space(3))) const uint8_t lds_u8;
+    uint32_t va = (uint32_t)(uintptr_t)(lds_u8*)(world + (f.ivy * XZ + f.ivx * Z + f.ivz));
+    uint32_t blk_u = (uint32_t)*(lds_u8*)(uintptr_t)va;
     uint32_t pinc = 0u;  // the last step's increment of pos: its axis
     // The reference's iteration is (limit check, lookup, advance); here the
-    // loop test holds the limit check and the lookup, the advance is
-    // branch-free, and the next voxel's LDS read is issued before the test.
-    int step = 0;
-    while (!(lim | out | (blk >= 0))) {
-      // The reference's order (x if tm_x < tm_y and tm_x < tm_z, else y if
-      // tm_y < tm_z, else z) from the minimum t: z exactly when tm_z is a
-      // minimum, else y exactly when tm_y is one (no t_max is NaN).
-      t = fminf(fminf(tm_x, tm_y), tm_z);
-      const bool cz = tm_z == t;
-      const bool cy = !cz & (tm_y == t);
-      const bool cx = !cz & !cy;
-      tm_x = cx ? tm_x + td_x : tm_x;
-      tm_y = cy ? tm_y + td_y : tm_y;
-      tm_z = cz ? tm_z + td_z : tm_z;
-      vp += cx ? dix : (cy ? diy : diz);
-      pinc = cx ? 1u : (cy ? (1u << 10) : (1u << 20));
-      pos += pinc;
-      out = (pos & kGuard) != 0u;
-      blk = (int)*vp;  // one step past the edge at most: a margin byte
-      ++step;
-      lim = (step >= max_steps) | (!VDF & (t >= vd));
-    }
+    // first lookup precedes the loop, and each iteration advances, looks up
+    // and tests (dda_loop); a lane runs while it is inside the world on air
+    // and below the limits.
+    const bool run = ((pos & kGuard) == 0u) & ((int8_t)blk_u < 0) & (max_steps > 0) & (t < vd);
+    dda_loop<VDF>(tm_x, tm_y, tm_z, t, inv_x, inv_y, inv_z, va, (uint32_t)dix, (uint32_t)diy,
+                  (uint32_t)diz, pos, pinc, blk_u, __builtin_amdgcn_ballot_w64(run), max_steps,
+                  vd);
+    int blk = (int)(int8_t)blk_u;
     // The walk's results are read back from vector registers (pos, blk, t,
     // pinc; the step count is the sum of pos's fields' advances, the start
     // word read again from LDS): per-lane booleans kept past the loop would
