@@ -720,8 +720,10 @@ struct OctPos {
 //
 // Written as one block so the loop control is a uniform counter and an exec
 // update per iteration (the compiler's divergent-loop form spends about twice
-// the scalar instructions); the arithmetic is the same IEEE binary32 adds
-// and compares, so results are unchanged bit for bit.  Every operand of a
+// the scalar instructions), and the t_max adds run under per-axis exec masks
+// (three adds in place of three adds and three selects: the step is bound by
+// vector instruction issue); the arithmetic is the same IEEE binary32 adds and
+// compares, so results are unchanged bit for bit.  Every operand of a
 // vector instruction is a VGPR or an inline constant except at most one
 // SGPR (a lane mask, the guard word or vd); a lane mask written by a vector
 // compare is read by vector instructions no earlier than three instructions
@@ -731,101 +733,60 @@ __device__ __forceinline__ void dda_loop(float& tx, float& ty, float& tz, float&
                                          float iy, float iz, uint32_t& va, uint32_t dx,
                                          uint32_t dy, uint32_t dz, uint32_t& pos, uint32_t& pinc,
                                          uint32_t& blk, uint64_t run, int max_steps, float vd) {
-  uint64_t save, mz, my, mc, mv;
+  uint64_t save, sa, mz, my, mc, mv;
   uint32_t d, w, step;
-  float nt;
   const uint32_t k10 = 1u << 10, k20 = 1u << 20, kg = (1u << 9) | (1u << 19) | (1u << 29);
-  if (VDF) {
-    asm volatile(
-        "s_mov_b64 %[save], exec\n\t"
-        "s_and_b64 exec, exec, %[run]\n\t"
-        "s_mov_b32 %[step], 0\n\t"
-        "s_cbranch_execz .Ldda_end%=\n"
-        ".Ldda_loop%=:\n\t"
-        "v_min3_f32 %[t], %[tx], %[ty], %[tz]\n\t"
-        "s_add_u32 %[step], %[step], 1\n\t"
-        "v_cmp_eq_f32_e64 %[mz], %[tz], %[t]\n\t"
-        "v_cmp_eq_f32_e64 %[my], %[ty], %[t]\n\t"
-        "v_add_f32_e64 %[nt], %[tx], |%[ix]|\n\t"
-        "v_add_f32_e64 %[w], %[ty], |%[iy]|\n\t"
-        "s_andn2_b64 %[my], %[my], %[mz]\n\t"
-        "s_or_b64 %[mc], %[my], %[mz]\n\t"
-        "v_cndmask_b32_e64 %[tx], %[nt], %[tx], %[mc]\n\t"
-        "v_cndmask_b32_e64 %[ty], %[ty], %[w], %[my]\n\t"
-        "v_add_f32_e64 %[nt], %[tz], |%[iz]|\n\t"
-        "v_cndmask_b32_e64 %[tz], %[tz], %[nt], %[mz]\n\t"
-        "v_cndmask_b32_e64 %[d], %[dx], %[dy], %[my]\n\t"
-        "v_cndmask_b32_e64 %[d], %[d], %[dz], %[mz]\n\t"
-        "v_add_u32_e32 %[va], %[va], %[d]\n\t"
-        "ds_read_u8 %[blk], %[va]\n\t"
-        "v_cndmask_b32_e64 %[pi], 1, %[k10], %[my]\n\t"
-        "v_cndmask_b32_e64 %[pi], %[pi], %[k20], %[mz]\n\t"
-        "v_add_u32_e32 %[pos], %[pos], %[pi]\n\t"
-        "v_and_b32_e32 %[w], %[kg], %[pos]\n\t"
-        "s_waitcnt lgkmcnt(0)\n\t"
-        "v_sub_u32_e32 %[w], %[blk], %[w]\n\t"
-        "v_cmp_lt_i32_e32 vcc, 0x7f, %[w]\n\t"
-        "s_and_b64 exec, exec, vcc\n\t"
-        "s_cmp_ge_i32 %[step], %[ms]\n\t"
-        "s_cbranch_scc1 .Ldda_end%=\n\t"
-        "s_cbranch_execnz .Ldda_loop%=\n"
-        ".Ldda_end%=:\n\t"
-        "s_mov_b64 exec, %[save]"
-        : [tx] "+v"(tx), [ty] "+v"(ty), [tz] "+v"(tz), [t] "+v"(t), [va] "+v"(va),
-          [pos] "+v"(pos), [pi] "+v"(pinc), [blk] "+v"(blk), [d] "=&v"(d), [w] "=&v"(w),
-          [nt] "=&v"(nt), [save] "=&s"(save), [mz] "=&s"(mz), [my] "=&s"(my), [mc] "=&s"(mc),
-          [step] "=&s"(step)
-        : [ix] "v"(ix), [iy] "v"(iy), [iz] "v"(iz), [dx] "v"(dx), [dy] "v"(dy), [dz] "v"(dz),
-          [k10] "v"(k10), [k20] "v"(k20), [kg] "s"(kg), [run] "s"(run), [ms] "s"(max_steps)
-        : "vcc", "scc");
-    (void)mv;
-  } else {
-    asm volatile(
-        "s_mov_b64 %[save], exec\n\t"
-        "s_and_b64 exec, exec, %[run]\n\t"
-        "s_mov_b32 %[step], 0\n\t"
-        "s_cbranch_execz .Ldda_end%=\n"
-        ".Ldda_loop%=:\n\t"
-        "v_min3_f32 %[t], %[tx], %[ty], %[tz]\n\t"
-        "s_add_u32 %[step], %[step], 1\n\t"
-        "v_cmp_eq_f32_e64 %[mz], %[tz], %[t]\n\t"
-        "v_cmp_eq_f32_e64 %[my], %[ty], %[t]\n\t"
-        "v_cmp_gt_f32_e64 %[mv], %[vd], %[t]\n\t"
-        "v_add_f32_e64 %[nt], %[tx], |%[ix]|\n\t"
-        "v_add_f32_e64 %[w], %[ty], |%[iy]|\n\t"
-        "s_andn2_b64 %[my], %[my], %[mz]\n\t"
-        "s_or_b64 %[mc], %[my], %[mz]\n\t"
-        "v_cndmask_b32_e64 %[tx], %[nt], %[tx], %[mc]\n\t"
-        "v_cndmask_b32_e64 %[ty], %[ty], %[w], %[my]\n\t"
-        "v_add_f32_e64 %[nt], %[tz], |%[iz]|\n\t"
-        "v_cndmask_b32_e64 %[tz], %[tz], %[nt], %[mz]\n\t"
-        "v_cndmask_b32_e64 %[d], %[dx], %[dy], %[my]\n\t"
-        "v_cndmask_b32_e64 %[d], %[d], %[dz], %[mz]\n\t"
-        "v_add_u32_e32 %[va], %[va], %[d]\n\t"
-        "ds_read_u8 %[blk], %[va]\n\t"
-        "v_cndmask_b32_e64 %[pi], 1, %[k10], %[my]\n\t"
-        "v_cndmask_b32_e64 %[pi], %[pi], %[k20], %[mz]\n\t"
-        "v_add_u32_e32 %[pos], %[pos], %[pi]\n\t"
-        "v_and_b32_e32 %[w], %[kg], %[pos]\n\t"
-        "s_waitcnt lgkmcnt(0)\n\t"
-        "v_sub_u32_e32 %[w], %[blk], %[w]\n\t"
-        "v_cmp_lt_i32_e32 vcc, 0x7f, %[w]\n\t"
-        "s_and_b64 exec, exec, vcc\n\t"
-        "s_and_b64 exec, exec, %[mv]\n\t"
-        "s_cmp_ge_i32 %[step], %[ms]\n\t"
-        "s_cbranch_scc1 .Ldda_end%=\n\t"
-        "s_cbranch_execnz .Ldda_loop%=\n"
-        ".Ldda_end%=:\n\t"
-        "s_mov_b64 exec, %[save]"
-        : [tx] "+v"(tx), [ty] "+v"(ty), [tz] "+v"(tz), [t] "+v"(t), [va] "+v"(va),
-          [pos] "+v"(pos), [pi] "+v"(pinc), [blk] "+v"(blk), [d] "=&v"(d), [w] "=&v"(w),
-          [nt] "=&v"(nt), [save] "=&s"(save), [mz] "=&s"(mz), [my] "=&s"(my), [mc] "=&s"(mc),
-          [mv] "=&s"(mv), [step] "=&s"(step)
-        : [ix] "v"(ix), [iy] "v"(iy), [iz] "v"(iz), [dx] "v"(dx), [dy] "v"(dy), [dz] "v"(dz),
-          [k10] "v"(k10), [k20] "v"(k20), [kg] "s"(kg), [run] "s"(run), [ms] "s"(max_steps),
-          [vd] "s"(vd)
-        : "vcc", "scc");
-  }
+#define MC_DDA_ASM(VDCMP, VDAND)                                                   \
+  asm volatile(                                                                    \
+      "s_mov_b64 %[save], exec\n\t"                                               \
+      "s_and_b64 exec, exec, %[run]\n\t"                                          \
+      "s_mov_b32 %[step], 0\n\t"                                                  \
+      "s_cbranch_execz .Ldda_end%=\n"                                              \
+      ".Ldda_loop%=:\n\t"                                                         \
+      "v_min3_f32 %[t], %[tx], %[ty], %[tz]\n\t"                                  \
+      "s_add_u32 %[step], %[step], 1\n\t"                                         \
+      "v_cmp_eq_f32_e64 %[mz], %[tz], %[t]\n\t"                                   \
+      "v_cmp_eq_f32_e64 %[my], %[ty], %[t]\n\t" VDCMP                             \
+      "s_mov_b64 %[sa], exec\n\t"                                                 \
+      "s_andn2_b64 %[my], %[my], %[mz]\n\t"                                       \
+      "s_or_b64 %[mc], %[my], %[mz]\n\t"                                          \
+      "v_cndmask_b32_e64 %[d], %[dx], %[dy], %[my]\n\t"                           \
+      "v_cndmask_b32_e64 %[d], %[d], %[dz], %[mz]\n\t"                            \
+      "v_add_u32_e32 %[va], %[va], %[d]\n\t"                                      \
+      "ds_read_u8 %[blk], %[va]\n\t"                                              \
+      "v_cndmask_b32_e64 %[pi], 1, %[k10], %[my]\n\t"                             \
+      "v_cndmask_b32_e64 %[pi], %[pi], %[k20], %[mz]\n\t"                         \
+      "v_add_u32_e32 %[pos], %[pos], %[pi]\n\t"                                   \
+      "s_andn2_b64 exec, %[sa], %[mc]\n\t"                                        \
+      "v_add_f32_e64 %[tx], %[tx], |%[ix]|\n\t"                                   \
+      "s_mov_b64 exec, %[my]\n\t"                                                 \
+      "v_add_f32_e64 %[ty], %[ty], |%[iy]|\n\t"                                   \
+      "s_mov_b64 exec, %[mz]\n\t"                                                 \
+      "v_add_f32_e64 %[tz], %[tz], |%[iz]|\n\t"                                   \
+      "s_mov_b64 exec, %[sa]\n\t"                                                 \
+      "v_and_b32_e32 %[w], %[kg], %[pos]\n\t"                                     \
+      "s_waitcnt lgkmcnt(0)\n\t"                                                  \
+      "v_sub_u32_e32 %[w], %[blk], %[w]\n\t"                                      \
+      "v_cmp_lt_i32_e32 vcc, 0x7f, %[w]\n\t"                                      \
+      "s_and_b64 exec, exec, vcc\n\t" VDAND                                      \
+      "s_cmp_ge_i32 %[step], %[ms]\n\t"                                           \
+      "s_cbranch_scc1 .Ldda_end%=\n\t"                                            \
+      "s_cbranch_execnz .Ldda_loop%=\n"                                            \
+      ".Ldda_end%=:\n\t"                                                          \
+      "s_mov_b64 exec, %[save]"                                                    \
+      : [tx] "+v"(tx), [ty] "+v"(ty), [tz] "+v"(tz), [t] "+v"(t), [va] "+v"(va),     \
+        [pos] "+v"(pos), [pi] "+v"(pinc), [blk] "+v"(blk), [d] "=&v"(d),             \
+        [w] "=&v"(w), [save] "=&s"(save), [sa] "=&s"(sa), [mz] "=&s"(mz),           \
+        [my] "=&s"(my), [mc] "=&s"(mc), [mv] "=&s"(mv), [step] "=&s"(step)          \
+      : [ix] "v"(ix), [iy] "v"(iy), [iz] "v"(iz), [dx] "v"(dx), [dy] "v"(dy),        \
+        [dz] "v"(dz), [k10] "v"(k10), [k20] "v"(k20), [kg] "s"(kg), [run] "s"(run),  \
+        [ms] "s"(max_steps), [vd] "s"(vd)                                           \
+      : "vcc", "scc")
+  if (VDF)
+    MC_DDA_ASM("", "");
+  else
+    MC_DDA_ASM("v_cmp_gt_f32_e64 %[mv], %[vd], %[t]\n\t", "s_and_b64 exec, exec, %[mv]\n\t");
+#undef MC_DDA_ASM
 }
 
 template <bool LIFT>
