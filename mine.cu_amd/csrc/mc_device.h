@@ -1067,15 +1067,21 @@ __device__ __forceinline__ uint32_t to_u8(float c) { return (uint32_t)(c * 255.0
 // with a = 2 (px + 0.5) / W - 1 and b = 1 - 2 (py + 0.5) / H.  kUvRaw keeps
 // a and b, for renders whose scales come per env from a stored basis.
 enum UvMode { kUvShaded = 0, kUvMinimal = 1, kUvBasis = 2, kUvRaw = 3 };
+// x / n for a positive int n: a product with 2^-k when n = 2^k (the same
+// value: scaling by a power of two is exact here), else the IEEE division.
+__device__ __forceinline__ float div_by(float x, int n) {
+  if ((n & (n - 1)) == 0) return x * __int_as_float((127 - (31 - __clz(n))) << 23);
+  return x / (float)n;
+}
 __device__ __forceinline__ float u_of(const Params& p, int m, int px) {
-  const float a = 2.0f * (px + 0.5f) / p.W - 1.0f;
+  const float a = div_by(2.0f * (px + 0.5f), p.W) - 1.0f;
   if (m == kUvShaded) return a * p.half_fov * p.aspect;
   if (m == kUvMinimal) return a * p.flat_hf * p.aspect;
   if (m == kUvBasis) return a * p.flat_su;
   return a;
 }
 __device__ __forceinline__ float v_of(const Params& p, int m, int py) {
-  const float c = 1.0f - 2.0f * (py + 0.5f) / p.H;
+  const float c = 1.0f - div_by(2.0f * (py + 0.5f), p.H);
   if (m == kUvShaded) return c * p.half_fov;
   if (m == kUvMinimal || m == kUvBasis) return c * p.flat_hf;
   return c;
@@ -1700,6 +1706,9 @@ template <int NT, int KIND, bool LIFT, bool CHW = false, bool TER = false, bool 
 __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, int wpe,
                                          int8_t* lds, const float* pal, const uint32_t* flat,
                                          Frame* frames, float* su, float* sv, int* upper) {
+  // A 64-thread workgroup is one wave: one env per item, one wave per env
+  // (a compile-time constant there, so every per-env stride folds).
+  if (NT == kWave) wpe = 1;
   const int tid = threadIdx.x, wave = tid / kWave;
   const int epw = (NT / kWave) / wpe;
   const int e = wave / wpe;                                 // env slot within the item

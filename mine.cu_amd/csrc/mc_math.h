@@ -65,12 +65,16 @@ MC_HD double reduce_quadrant(uint32_t ax_bits, int* q_out) {
   const int j0 = (e >= 2) ? (e - 1) : 1;
   const int off = j0 - 1;
   const int wi = off >> 5, sh = off & 31;
-  uint32_t a0 = two_over_pi_word(wi), a1 = two_over_pi_word(wi + 1);
-  uint32_t a2 = two_over_pi_word(wi + 2), a3 = two_over_pi_word(wi + 3);
   uint32_t w0, w1, w2;
-  if (sh == 0) {
-    w0 = a0; w1 = a1; w2 = a2;
+  if (off == 0) {
+    // |x| < 2^25 (every angle an agent turns to in practice): the window is
+    // the first three words, the general case below with wi = sh = 0.
+    w0 = 0xa2f9836eu; w1 = 0x4e441529u; w2 = 0xfc2757d1u;
+  } else if (sh == 0) {
+    w0 = two_over_pi_word(wi); w1 = two_over_pi_word(wi + 1); w2 = two_over_pi_word(wi + 2);
   } else {
+    const uint32_t a0 = two_over_pi_word(wi), a1 = two_over_pi_word(wi + 1);
+    const uint32_t a2 = two_over_pi_word(wi + 2), a3 = two_over_pi_word(wi + 3);
     w0 = (a0 << sh) | (a1 >> (32 - sh));
     w1 = (a1 << sh) | (a2 >> (32 - sh));
     w2 = (a2 << sh) | (a3 >> (32 - sh));
