@@ -68,10 +68,11 @@ int check_dims(int B, int X, int Y, int Z) {
   if (B < 1) return fail(MC_E_INVALID, "batch_size must be >= 1 (got %d)", B);
   if (X < 1 || Y < 1 || Z < 1)
     return fail(MC_E_INVALID, "world dims must be >= 1 (got %d x %d x %d)", X, Y, Z);
-  const int budget = lds_budget();
-  if ((int64_t)X * Y * Z > budget || mc::item_lds_bytes(1, X, Y, Z) > budget)
-    return fail(MC_E_UNSUPPORTED, "world %dx%dx%d exceeds the %d-byte LDS-resident limit", X, Y,
-                Z, budget);
+  // Worlds that do not fit one workgroup's LDS run from global memory; the
+  // in-world walk packs each coordinate into 10 bits.
+  if (X > mc::kMaxInsideDim || Y > mc::kMaxInsideDim || Z > mc::kMaxInsideDim)
+    return fail(MC_E_UNSUPPORTED, "world %dx%dx%d: each dimension must be at most %d", X, Y, Z,
+                mc::kMaxInsideDim);
   return MC_OK;
 }
 
@@ -263,13 +264,22 @@ int launch_env(int kind, const mc::Params& q, int obs_mode, Shape s, hipStream_t
     p.ys = p.Y;
     p.fb = nullptr;
   }
-  s = fit_lds(s, p.X, p.ys, p.Z);
-  if (mc::item_lds_bytes(epw_of(s), p.X, p.ys, p.Z) > lds_budget())
-    return fail(MC_E_UNSUPPORTED, "world does not fit the LDS budget");
   // Planar images come from MineEnv's own step, reset and render only.
   if (planar_mode(obs_mode) && kind != mc::kItemStep && kind != mc::kItemResetRender &&
       kind != mc::kItemRenderState)
     return fail(MC_E_UNSUPPORTED, "the MC_OBS_*_CHW modes render from MineEnv state only");
+  s = fit_lds(s, p.X, p.ys, p.Z);
+  if (mc::item_lds_bytes(epw_of(s), p.X, p.ys, p.Z) > lds_budget()) {
+    // Too large for LDS even at one env per workgroup: the world stays in
+    // global memory (whole height; no counting build there).
+    if (count) return fail(MC_E_UNSUPPORTED, "ray-step counting needs an LDS-resident world");
+    p.ys = p.Y;
+    p.fb = nullptr;
+    const hipError_t e =
+        mcl::kernels(256, true).glb(kind, p, obs_mode, false, dim3(p.B), st);
+    if (e != hipSuccess) return fail(MC_E_HIP, "global-world launch: %s", hipGetErrorString(e));
+    return MC_OK;
+  }
   const size_t lds = (size_t)mc::item_lds_bytes(epw_of(s), p.X, p.ys, p.Z);
   const mcl::KernelSet& ks = mcl::kernels(s.nt, s.lift != 0);
   const hipError_t e = count ? ks.env_count(p, obs_mode, s.wpe, lds, st)
@@ -284,7 +294,14 @@ int launch_deferred(const mc::Params& q, int obs_mode, int lift, hipStream_t st)
   mc::Params p = q;
   p.ys = p.Y;
   const int full = mc::item_lds_bytes(1, p.X, p.Y, p.Z);
-  if (full > lds_budget()) return fail(MC_E_UNSUPPORTED, "world does not fit the LDS budget");
+  if (full > lds_budget()) {
+    // The whole world does not fit LDS: the deferred envs run from global
+    // memory.
+    const hipError_t e =
+        mcl::kernels(256, true).glb(mc::kItemStep, p, obs_mode, true, dim3(std::min(p.B, 256)), st);
+    if (e != hipSuccess) return fail(MC_E_HIP, "deferred-env launch: %s", hipGetErrorString(e));
+    return MC_OK;
+  }
   const int nt = full > 80 * 1024 ? 512 : 256;
   const hipError_t e = mcl::kernels(nt, lift != 0).fb(p, obs_mode, dim3(std::min(p.B, 64)),
                                                       (size_t)full, st);

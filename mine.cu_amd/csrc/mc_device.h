@@ -827,8 +827,10 @@ __device__ __forceinline__ uint32_t oct_pos(const Params& p, const Frame& f, int
 // position, so the same test ends the ray there.
 // ---------------------------------------------------------------------------
 // CNT: *nlook receives the ray's DDA voxel lookups, the start voxel included
-// (0 when the slab test culls the ray).
-template <bool LIFT, bool VDF, bool CNT = false>
+// (0 when the slab test culls the ray).  GLB: the world is the env's slab in
+// global memory (worlds too large for LDS): the in-world walk never reads
+// outside it (there is no margin) and runs as plain code.
+template <bool LIFT, bool VDF, bool CNT = false, bool GLB = false>
 __device__ __forceinline__ void trace(const Params& p, const Frame& f, const OctPos& op, float u,
                                       float v, const int8_t* __restrict__ world,
                                       const float* __restrict__ pal, float rgb[3],
@@ -861,20 +863,47 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
     float tm_z = (f.dzn + ((ray_z >= 0) ? 1.0f : 0.0f)) * inv_z;
     constexpr uint32_t kGuard = (1u << 9) | (1u << 19) | (1u << 29);
     uint32_t pos = op.pick(step_x > 0, step_y > 0, step_z > 0);
-    // LDS address of the current voxel, as a 32-bit LDS offset.
-    typedef __attribute__((address_space(3))) const uint8_t lds_u8;
-    uint32_t va = (uint32_t)(uintptr_t)(lds_u8*)(world + (f.ivy * XZ + f.ivx * Z + f.ivz));
-    uint32_t blk_u = (uint32_t)*(lds_u8*)(uintptr_t)va;
     uint32_t pinc = 0u;  // the last step's increment of pos: its axis
-    // The reference's iteration is (limit check, lookup, advance); here the
-    // first lookup precedes the loop, and each iteration advances, looks up
-    // and tests (dda_loop); a lane runs while it is inside the world on air
-    // and below the limits.
-    const bool run = ((pos & kGuard) == 0u) & ((int8_t)blk_u < 0) & (max_steps > 0) & (t < vd);
-    dda_loop<VDF>(tm_x, tm_y, tm_z, t, inv_x, inv_y, inv_z, va, (uint32_t)dix, (uint32_t)diy,
-                  (uint32_t)diz, pos, pinc, blk_u, __builtin_amdgcn_ballot_w64(run), max_steps,
-                  vd);
-    int blk = (int)(int8_t)blk_u;
+    int blk;
+    if (GLB) {
+      // The reference's loop order, the lookup guarded: a position outside
+      // the world reads the env's first voxel and discards it.
+      const int8_t* vp = world + (f.ivy * XZ + f.ivx * Z + f.ivz);
+      bool out = (pos & kGuard) != 0u;
+      blk = out ? -1 : (int)*(out ? world : vp);
+      bool lim = (max_steps <= 0) | (t >= vd);
+      int step = 0;
+      while (!(lim | out | (blk >= 0))) {
+        t = fminf(fminf(tm_x, tm_y), tm_z);
+        const bool cz = tm_z == t;
+        const bool cy = !cz & (tm_y == t);
+        const bool cx = !cz & !cy;
+        tm_x = cx ? tm_x + td_x : tm_x;
+        tm_y = cy ? tm_y + td_y : tm_y;
+        tm_z = cz ? tm_z + td_z : tm_z;
+        vp += cx ? dix : (cy ? diy : diz);
+        pinc = cx ? 1u : (cy ? (1u << 10) : (1u << 20));
+        pos += pinc;
+        out = (pos & kGuard) != 0u;
+        blk = out ? -1 : (int)*(out ? world : vp);
+        ++step;
+        lim = (step >= max_steps) | (!VDF & (t >= vd));
+      }
+    } else {
+      // LDS address of the current voxel, as a 32-bit LDS offset.
+      typedef __attribute__((address_space(3))) const uint8_t lds_u8;
+      uint32_t va = (uint32_t)(uintptr_t)(lds_u8*)(world + (f.ivy * XZ + f.ivx * Z + f.ivz));
+      uint32_t blk_u = (uint32_t)*(lds_u8*)(uintptr_t)va;
+      // The reference's iteration is (limit check, lookup, advance); here the
+      // first lookup precedes the loop, and each iteration advances, looks up
+      // and tests (dda_loop); a lane runs while it is inside the world on air
+      // and below the limits.
+      const bool run = ((pos & kGuard) == 0u) & ((int8_t)blk_u < 0) & (max_steps > 0) & (t < vd);
+      dda_loop<VDF>(tm_x, tm_y, tm_z, t, inv_x, inv_y, inv_z, va, (uint32_t)dix, (uint32_t)diy,
+                    (uint32_t)diz, pos, pinc, blk_u, __builtin_amdgcn_ballot_w64(run), max_steps,
+                    vd);
+      blk = (int)(int8_t)blk_u;
+    }
     // The walk's results are read back from vector registers (pos, blk, t,
     // pinc; the step count is the sum of pos's fields' advances, the start
     // word read again from LDS): per-lane booleans kept past the loop would
@@ -972,7 +1001,7 @@ __device__ __forceinline__ void trace(const Params& p, const Frame& f, const Oct
 // Otherwise a ray outside the world that moves away from it on some axis
 // can never come back, which ends it too.
 // ---------------------------------------------------------------------------
-template <bool LIFT, typename T, bool CNT = false>
+template <bool LIFT, typename T, bool CNT = false, bool GLB = false>
 __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, const OctPos& op,
                                           float u, float v, const int8_t* __restrict__ world,
                                           int* nlook = nullptr) {
@@ -998,7 +1027,7 @@ __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, const
     // LDS address of the current voxel.
     const int8_t* vp = world + (f.ivy * XZ + f.ivx * Z + f.ivz);
     bool out = (pos & kGuard) != 0u;
-    int blk = (int)*vp;
+    int blk = GLB ? (out ? -1 : (int)*(out ? world : vp)) : (int)*vp;
     int step = 0;
     bool lim = max_steps <= 0;
     while (!(lim | out | (blk >= 0))) {
@@ -1011,7 +1040,9 @@ __device__ __forceinline__ int trace_flat(const Params& p, const Frame& f, const
       vp += cx ? dix : (cy ? diy : diz);
       pos += cx ? 1u : (cy ? (1u << 10) : (1u << 20));
       out = (pos & kGuard) != 0u;
-      blk = (int)*vp;  // one step past the edge at most: a margin byte
+      // One step past the edge at most: a margin byte in LDS; a guarded read
+      // of global memory.
+      blk = GLB ? (out ? -1 : (int)*(out ? world : vp)) : (int)*vp;
       ++step;
       lim = step >= max_steps;
     }
@@ -1198,7 +1229,7 @@ __device__ __forceinline__ void render_rgb8(const Params& p, int b, int uvm, int
 // binary16 DDA when T = _Float16).  flat: the packed colour table, 13 blocks
 // then the sky; ids >= 13 show the sky.
 // CNT: acc[0] += DDA lookups, acc[1] += rays marched, acc[2] += pixels.
-template <bool LIFT, typename T, bool CNT = false>
+template <bool LIFT, typename T, bool CNT = false, bool GLB = false>
 __device__ __forceinline__ void render_env_flat(const Params& p, int b, const int8_t* world,
                                                 const uint32_t* flat, const Frame& f,
                                                 const OctPos& op, int uvm, int tid, int nt,
@@ -1215,7 +1246,7 @@ __device__ __forceinline__ void render_env_flat(const Params& p, int b, const in
       if (CNT) acc[0] += 1;
     } else {
       int n = 0;
-      h = trace_flat<LIFT, T, CNT>(p, f, op, u, v, world, &n);
+      h = trace_flat<LIFT, T, CNT, GLB>(p, f, op, u, v, world, &n);
       if (CNT) acc[0] += n, acc[1] += 1;
     }
     if (CNT) acc[2] += 1;
@@ -1231,7 +1262,7 @@ __device__ __forceinline__ void render_env_flat(const Params& p, int b, const in
 
 // planar: the image is written [3, H, W] (the MC_OBS_*_CHW modes).  Callers
 // pass a compile-time constant, so each kernel holds one store form only.
-template <int OBS, bool LIFT, bool VDF, bool CNT = false>
+template <int OBS, bool LIFT, bool VDF, bool CNT = false, bool GLB = false>
 __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t* world,
                                            const float* pal, const Frame& f, const OctPos& op,
                                            int tid, int nt, const float* su, const float* sv,
@@ -1258,7 +1289,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
         if (CNT) acc[0] += 1;
       } else {
         int n = 0;
-        trace<LIFT, VDF, CNT>(p, f, op, u, v, world, pal, rgb, &n);
+        trace<LIFT, VDF, CNT, GLB>(p, f, op, u, v, world, pal, rgb, &n);
         if (CNT) acc[0] += n, acc[1] += 1;
       }
       if (CNT) acc[2] += 1;
@@ -1275,7 +1306,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
         if (CNT) acc[0] += 1;
       } else {
         int n = 0;
-        trace<LIFT, VDF, CNT>(p, f, op, ucol(px), vrow(py), world, pal, rgb, &n);
+        trace<LIFT, VDF, CNT, GLB>(p, f, op, ucol(px), vrow(py), world, pal, rgb, &n);
         if (CNT) acc[0] += n, acc[1] += 1;
       }
       if (CNT) acc[2] += 1;
@@ -1701,8 +1732,11 @@ __host__ __device__ __forceinline__ int item_lds_bytes(int epw, int X, int Y, in
 
 // upper: one word per wave of the workgroup (staged-height check).
 // CNT: count ray steps into p.ray_steps (the measurement build of the step).
+// GLB: the env's world stays in global memory (worlds that do not fit LDS):
+// one env per workgroup, nothing staged, and the world rewritten on a done
+// step only after the whole workgroup has rendered.
 template <int NT, int KIND, bool LIFT, bool CHW = false, bool TER = false, bool COL = false,
-          bool CNT = false>
+          bool CNT = false, bool GLB = false>
 __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, int wpe,
                                          int8_t* lds, const float* pal, const uint32_t* flat,
                                          Frame* frames, float* su, float* sv, int* upper) {
@@ -1718,11 +1752,12 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   const bool active = (e < epw) && (b < p.B);
   const int nvox = p.X * p.Y * p.Z;
   // Staged layers (Params::ys; always all of them outside step items).
-  const bool staged = (KIND == kItemStep) && p.ys < p.Y;
+  const bool staged = !GLB && (KIND == kItemStep) && p.ys < p.Y;
   const int nstage = staged ? p.X * p.ys * p.Z : nvox;
-  int8_t* world = lds + world_margin_bytes(p.X, p.Z) +
-                  e * world_stride_bytes(p.X, staged ? p.ys : p.Y, p.Z);
   int8_t* gworld = p.voxels + (int64_t)b * nvox;
+  int8_t* world = GLB ? gworld
+                      : lds + world_margin_bytes(p.X, p.Z) +
+                            e * world_stride_bytes(p.X, staged ? p.ys : p.Y, p.Z);
   const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
   bool any_done = false;
   if (KIND == kItemStep && p.done_mode == 0 && p.ring) {
@@ -1742,8 +1777,8 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
                                                               : kUvShaded;
   fill_uv(p, uvm, su, sv, tid, NT);
   if (active) {
-    if (regen) store_fresh_world<TER>(p, b, gworld, world, nvox, et, ent);
-    else stage_world(world, gworld, nstage, et, ent);
+    if (regen) store_fresh_world<TER>(p, b, gworld, GLB ? nullptr : world, nvox, et, ent);
+    else if (!GLB) stage_world(world, gworld, nstage, et, ent);
   }
   if (staged) {
     const bool found = active && any_not_air(gworld + nstage, nvox - nstage, et, ent);
@@ -1762,6 +1797,9 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     else if (KIND == kItemRenderBasis) f = frame_from_basis(p, b, world);
     else f = camera_update(p, b, world, KIND == kItemRenderState);
     if (et == 0) {
+      // The solid-extent scan reads 16-byte chunks: a world in global memory
+      // must be 16-byte aligned for it.
+      if (GLB && ((uintptr_t)world & 15u)) f.lift = 0;
       frames[e] = f;
       if (KIND == kItemStep && p.basis && !f.deferred) store_basis(p, b, f);
     }
@@ -1792,7 +1830,8 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
     if (CHW) fill_planes8(out, npix, rgb, et, ent);
     else fill_rgb8(out, npix, rgb, et, ent);
-    if (KIND == kItemStep && any_done) store_fresh_world<TER>(p, b, gworld, nullptr, nvox, et, ent);
+    if (!GLB && KIND == kItemStep && any_done)
+      store_fresh_world<TER>(p, b, gworld, nullptr, nvox, et, ent);
   } else if (go) {
     // Every wave of the env writes the same 8 octant words before it reads
     // them; a wave's LDS operations complete in order.
@@ -1803,16 +1842,27 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     const OctPos op{frames[e].pos0};
     const bool f32 = obs_mode == (CHW ? kObsF32Chw : kObsF32);
     if (uvm != kUvShaded && obs_mode == kObsU8Fp16)
-      render_env_flat<LIFT, _Float16, CNT>(p, b, world, flat, f, op, uvm, et, ent, su, sv, acc);
+      render_env_flat<LIFT, _Float16, CNT, GLB>(p, b, world, flat, f, op, uvm, et, ent, su, sv,
+                                                acc);
     else if (uvm != kUvShaded)
-      render_env_flat<LIFT, float, CNT>(p, b, world, flat, f, op, uvm, et, ent, su, sv, acc);
+      render_env_flat<LIFT, float, CNT, GLB>(p, b, world, flat, f, op, uvm, et, ent, su, sv, acc);
     else if (f32 && p.vd_far)
-      render_env<1, LIFT, true, CNT>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
-    else if (f32) render_env<1, LIFT, false, CNT>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
+      render_env<1, LIFT, true, CNT, GLB>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
+    else if (f32)
+      render_env<1, LIFT, false, CNT, GLB>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
     else if (p.vd_far)
-      render_env<0, LIFT, true, CNT>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
-    else render_env<0, LIFT, false, CNT>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
-    if (KIND == kItemStep && any_done) store_fresh_world<TER>(p, b, gworld, nullptr, nvox, et, ent);
+      render_env<0, LIFT, true, CNT, GLB>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
+    else
+      render_env<0, LIFT, false, CNT, GLB>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
+    if (!GLB && KIND == kItemStep && any_done)
+      store_fresh_world<TER>(p, b, gworld, nullptr, nvox, et, ent);
+  }
+  if (GLB && KIND == kItemStep && any_done) {
+    // Every wave of the env has rendered from the world in global memory:
+    // only now may it be rewritten (one env per workgroup, so the barrier
+    // covers the env's waves; any_done is the same for every workgroup).
+    __syncthreads();
+    if (active && !frames[e].deferred) store_fresh_world<TER>(p, b, gworld, nullptr, nvox, et, ent);
   }
   if (CNT && p.ray_steps) {
 #pragma unroll
@@ -1956,6 +2006,44 @@ __global__ __launch_bounds__(NT) void k_env_fb_col(Params p, int obs_mode) {
     run_item<NT, kItemStep, LIFT, false, TER, true>(p, obs_mode, (int)list[2 + i], NT / kWave,
                                                     s_lds, s_pal, s_flat, s_frames, s_u, s_v,
                                                     s_upper);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&list[1], 1u) == gridDim.x - 1) {
+      list[0] = 0u;
+      list[1] = 0u;
+      __threadfence();
+    }
+  }
+}
+
+// Worlds too large for one workgroup's LDS: the env's world stays in global
+// memory, one env per 256-thread workgroup (four waves), the upward-ray exit
+// built in.  LIST: the envs a staged-height step deferred (fb list, as
+// k_env_fb), else env blockIdx.x.
+template <int KIND, bool CHW, bool TER, bool COL, bool LIST>
+__global__ __launch_bounds__(256) void k_env_glb(Params p, int obs_mode) {
+  __shared__ float s_pal[kPaletteLen];
+  __shared__ uint32_t s_flat[kFlatLen];
+  __shared__ Frame s_frames[4];
+  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
+  __shared__ int s_upper[4];
+  load_palette(s_pal, s_flat, threadIdx.x);
+  if (!LIST) {
+    run_item<256, KIND, true, CHW, TER, COL, false, true>(p, obs_mode, blockIdx.x, 4, nullptr,
+                                                          s_pal, s_flat, s_frames, s_u, s_v,
+                                                          s_upper);
+    return;
+  }
+  const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
+  uint32_t* list = p.fb + (size_t)slot * (p.B + 2);
+  const int n = (int)min(list[0], (uint32_t)p.B);
+  if (n == 0) return;  // nothing deferred: the list is already empty
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    __syncthreads();  // the previous env is done with the shared frame
+    run_item<256, kItemStep, true, CHW, TER, COL, false, true>(p, obs_mode, (int)list[2 + i], 4,
+                                                               nullptr, s_pal, s_flat, s_frames,
+                                                               s_u, s_v, s_upper);
   }
   if (threadIdx.x == 0) {
     __threadfence();

@@ -100,13 +100,58 @@ int multi_blocks(size_t lds) {
 }
 #endif
 
+#if MC_NT == 256 && MC_LIFT
+template <int KIND>
+void glb_kind(const mc::Params& p, int obs_mode, bool list, dim3 grid, hipStream_t st) {
+  const dim3 blk(256);
+  if (KIND == mc::kItemStep && p.collision) {
+    if (p.terrain && list)
+      hipLaunchKernelGGL((mc::k_env_glb<KIND, false, true, true, true>), grid, blk, 0, st, p, obs_mode);
+    else if (p.terrain)
+      hipLaunchKernelGGL((mc::k_env_glb<KIND, false, true, true, false>), grid, blk, 0, st, p, obs_mode);
+    else if (list)
+      hipLaunchKernelGGL((mc::k_env_glb<KIND, false, false, true, true>), grid, blk, 0, st, p, obs_mode);
+    else
+      hipLaunchKernelGGL((mc::k_env_glb<KIND, false, false, true, false>), grid, blk, 0, st, p, obs_mode);
+  } else if (p.terrain) {
+    if (KIND == mc::kItemStep && list)
+      hipLaunchKernelGGL((mc::k_env_glb<KIND, false, true, false, true>), grid, blk, 0, st, p, obs_mode);
+    else
+      hipLaunchKernelGGL((mc::k_env_glb<KIND, false, true, false, false>), grid, blk, 0, st, p, obs_mode);
+  } else if (KIND != mc::kItemRenderCameras && KIND != mc::kItemRenderBasis && planar(obs_mode)) {
+    if (KIND == mc::kItemStep && list)
+      hipLaunchKernelGGL((mc::k_env_glb<KIND, true, false, false, true>), grid, blk, 0, st, p, obs_mode);
+    else
+      hipLaunchKernelGGL((mc::k_env_glb<KIND, true, false, false, false>), grid, blk, 0, st, p, obs_mode);
+  } else {
+    if (KIND == mc::kItemStep && list)
+      hipLaunchKernelGGL((mc::k_env_glb<KIND, false, false, false, true>), grid, blk, 0, st, p, obs_mode);
+    else
+      hipLaunchKernelGGL((mc::k_env_glb<KIND, false, false, false, false>), grid, blk, 0, st, p, obs_mode);
+  }
+}
+
+hipError_t glb(int kind, const mc::Params& p, int obs_mode, bool list, dim3 grid, hipStream_t st) {
+  switch (kind) {
+    case mc::kItemStep: glb_kind<mc::kItemStep>(p, obs_mode, list, grid, st); break;
+    case mc::kItemResetRender: glb_kind<mc::kItemResetRender>(p, obs_mode, false, grid, st); break;
+    case mc::kItemRenderCameras: glb_kind<mc::kItemRenderCameras>(p, obs_mode, false, grid, st); break;
+    case mc::kItemRenderBasis: glb_kind<mc::kItemRenderBasis>(p, obs_mode, false, grid, st); break;
+    default: glb_kind<mc::kItemRenderState>(p, obs_mode, false, grid, st); break;
+  }
+  return hipGetLastError();
+}
+#endif
+
 }  // namespace
 
 const KernelSet& MC_SET_NAME(MC_NT, MC_LIFT)() {
-#if MC_NT >= 256
-  static const KernelSet set = {env, env_count, fb, multi, multi_fb, multi_blocks};
+#if MC_NT == 256 && MC_LIFT
+  static const KernelSet set = {env, env_count, fb, multi, multi_fb, multi_blocks, glb};
+#elif MC_NT >= 256
+  static const KernelSet set = {env, env_count, fb, multi, multi_fb, multi_blocks, nullptr};
 #else
-  static const KernelSet set = {env, env_count, nullptr, nullptr, nullptr, nullptr};
+  static const KernelSet set = {env, env_count, nullptr, nullptr, nullptr, nullptr, nullptr};
 #endif
   return set;
 }

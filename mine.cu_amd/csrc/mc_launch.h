@@ -31,6 +31,11 @@ struct KernelSet {
   hipError_t (*multi_fb)(const mc::MultiArgs& a, dim3 grid, size_t lds, hipStream_t st);
   // Resident workgroups per CU of the persistent multi kernel at `lds` bytes.
   int (*multi_blocks)(size_t lds);
+  // Worlds too large for one workgroup's LDS: the item with the world in
+  // global memory, one env per 256-thread workgroup; list: the envs a staged
+  // step deferred (p.fb).  Non-NULL in the 256-thread lift set only.
+  hipError_t (*glb)(int kind, const mc::Params& p, int obs_mode, bool list, dim3 grid,
+                    hipStream_t st);
 };
 
 const KernelSet& kKernels64_0();

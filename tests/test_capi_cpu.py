@@ -51,8 +51,9 @@ def test_null_and_range_errors():
     assert L.mc_generate_world(None, 1, 8, 8, 8, 8, 0, None) == _l.MC_E_INVALID
     v = ctypes.c_void_p(0x1000)  # never dereferenced: validation fails first
     assert L.mc_generate_world(v, 0, 8, 8, 8, 8, 0, None) == _l.MC_E_INVALID
-    assert L.mc_render(v, v, v, 0, 4, 64, 64, 64, 8, 8, 64, 32.0, 70.0, None) == _l.MC_E_UNSUPPORTED
-    assert "LDS" in _err(L)
+    # any world runs (from global memory when it does not fit LDS) up to 512 per axis
+    assert L.mc_render(v, v, v, 0, 4, 600, 64, 64, 8, 8, 64, 32.0, 70.0, None) == _l.MC_E_UNSUPPORTED
+    assert "at most 512" in _err(L)
     assert L.mc_render(v, v, v, 7, 4, 8, 8, 8, 8, 8, 64, 32.0, 70.0, None) == _l.MC_E_INVALID
     assert L.mc_render(v, v, v, 0, 4, 8, 8, 8, 0, 8, 64, 32.0, 70.0, None) == _l.MC_E_INVALID
     assert L.mc_env_create(None, None, None) == _l.MC_E_INVALID
