@@ -152,3 +152,38 @@ def test_collision_matches_oracle(ws, terrain, amp, seed, dtype):
     if terrain == "flat":
         # the agents stand on the grass (y = 8), not on the plane y = Y / 4
         assert float(env.positions[:, 1].min()) > ws // 4
+
+
+@pytest.mark.parametrize("lift", [0, 1])
+def test_terrain_collision_48_full_height_pass(lift):
+    """48^3 seeded terrain with voxel collision: envs with agents high up or
+    solids above the staged layers take the full-height pass, whose whole
+    world (110,592 B) needs the 512-thread kernels (k_env_fb_col<512, ...>),
+    with the upward-ray exit off and on.  Bit-exact against the oracle through
+    falls, landings and a done step."""
+    import minecu
+    B = 12
+    kw = dict(batch_size=B, world_size=48, render_width=32, render_height=24, obs_dtype="uint8",
+              episode_length=10, seed=9, terrain_amplitude=3, collision=True)
+    env = minecu.MineEnv(device="cuda", terrain="noise", **kw)
+    orc = oracle.OracleEnv(terrain="noise", **kw)
+    env.set_shape(0, 0, lift)
+    env.reset()
+    orc.reset()
+    torch.cuda.synchronize()
+    vox = env.voxels.cpu().numpy()
+    pos = env.positions.cpu().numpy()
+    vox[0:3, 40, :, :] = 9          # ceilings above the staged layers
+    pos[3:6, 1] = 36.0              # agents high up
+    env.voxels.copy_(torch.from_numpy(vox))
+    env.positions.copy_(torch.from_numpy(pos))
+    orc.voxels[...] = vox
+    orc.positions[...] = pos
+    st = Stats()
+    acts = _acts(22, B, 17)
+    for t in range(len(acts)):
+        _, r, d = env.step(torch.as_tensor(acts[t]).cuda())
+        _, ro, do = orc.step(acts[t])
+        assert np.array_equal(r.cpu().numpy(), ro) and np.array_equal(d.cpu().numpy(), do), t
+        compare_env(env, orc, st, f"step {t}")
+    assert st.exact, st.summary()
