@@ -1203,12 +1203,52 @@ __device__ __forceinline__ void for_tiles(const Params& p, int tid, int nt, Pixe
 
 // uint8 image of env b from a per-pixel colour function returning packed
 // 0x00BBGGRR.
+// Interleaved images whose width is a multiple of 8 store each 8x8 tile as
+// 48 dwords (8 rows of 24 bytes): the tile's packed pixels go through the
+// wave's 64-dword LDS buffer `tile`, and lane 6r + d (r < 8, d < 6) builds
+// dword d of row r from the two pixels it spans with one byte permute.  Rows
+// below the image are not stored.  Other images store per pixel.
 template <typename ColorFn>
 __device__ __forceinline__ void render_rgb8(const Params& p, int b, int uvm, int tid, int nt,
                                             const float* su, const float* sv, ColorFn color,
-                                            bool planar = false) {
+                                            bool planar = false, uint32_t* tile = nullptr) {
   uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * p.W * p.H * 3;
   const bool tab = uv_tabled(p);
+  if (!planar && tile && ((p.W & 7) == 0) && (((uintptr_t)out & 3u) == 0)) {
+    const int lane = tid & (kWave - 1), nwv = nt >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int tx = lane & 7, ty = lane >> 3;
+    const int tcols = p.W >> 3, trows = (p.H + 7) >> 3, ntiles = tcols * trows;
+    // Store role of this lane: row r, dword d, the tile pixels a and a + 1
+    // it takes its bytes from, and the permute selector (bytes 0-3 of pixel
+    // a, 4-7 of pixel a + 1, each 0x00BBGGRR).
+    const int r = lane / 6, d = lane - 6 * r, dm = d % 3;
+    const int a = r * 8 + (d / 3) * 4 + dm;
+    const uint32_t sel = (dm == 0) ? 0x04020100u : (dm == 1) ? 0x05040201u : 0x06050402u;
+    const uint32_t row_off = (uint32_t)(r * p.W * 3 + 4 * d);
+    const int dr = nwv / tcols, dc = nwv - dr * tcols;
+    int tr = wv / tcols, tc = wv - tr * tcols;
+    typedef MC_GLOBAL uint32_t gu32;
+    for (int t = wv; t < ntiles; t += nwv) {
+      const int py = tr * 8 + ty, px = tc * 8 + tx;
+      uint32_t c = 0u;
+      if (py < p.H) c = color(tab ? su[px] : u_of(p, uvm, px), tab ? sv[py] : v_of(p, uvm, py));
+      tile[lane] = c;
+      __builtin_amdgcn_wave_barrier();
+      if ((lane < 48) & (tr * 8 + r < p.H)) {
+        const uint32_t w = __builtin_amdgcn_perm(tile[a + 1], tile[a], sel);
+        *(gu32*)(out + (uint32_t)((tr * 8 * p.W + tc * 8) * 3) + row_off) = w;
+      }
+      __builtin_amdgcn_wave_barrier();
+      tr += dr;
+      tc += dc;
+      if (tc >= tcols) {
+        tc -= tcols;
+        ++tr;
+      }
+    }
+    return;
+  }
   for_tiles(p, tid, nt, [&](int px, int py) {
     const uint32_t c = color(tab ? su[px] : u_of(p, uvm, px), tab ? sv[py] : v_of(p, uvm, py));
     if (planar) {
@@ -1234,7 +1274,7 @@ __device__ __forceinline__ void render_env_flat(const Params& p, int b, const in
                                                 const uint32_t* flat, const Frame& f,
                                                 const OctPos& op, int uvm, int tid, int nt,
                                                 const float* su, const float* sv,
-                                                uint32_t* acc = nullptr) {
+                                                uint32_t* acc = nullptr, uint32_t* tile = nullptr) {
   const int npix = p.W * p.H;
   uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
   const bool per_env_scale = (uvm == kUvRaw);
@@ -1257,7 +1297,7 @@ __device__ __forceinline__ void render_env_flat(const Params& p, int b, const in
     if (CNT && tid == 0) acc[0] += npix, acc[2] += npix;
     return;
   }
-  render_rgb8(p, b, uvm, tid, nt, su, sv, color);
+  render_rgb8(p, b, uvm, tid, nt, su, sv, color, false, tile);
 }
 
 // planar: the image is written [3, H, W] (the MC_OBS_*_CHW modes).  Callers
@@ -1266,7 +1306,8 @@ template <int OBS, bool LIFT, bool VDF, bool CNT = false, bool GLB = false>
 __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t* world,
                                            const float* pal, const Frame& f, const OctPos& op,
                                            int tid, int nt, const float* su, const float* sv,
-                                           bool planar, uint32_t* acc = nullptr) {
+                                           bool planar, uint32_t* acc = nullptr,
+                                           uint32_t* tile = nullptr) {
   const int W = p.W, npix = W * p.H;
   const bool tab = uv_tabled(p);
   auto ucol = [&](int px) { return tab ? su[px] : u_of(p, kUvShaded, px); };
@@ -1295,7 +1336,7 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       if (CNT) acc[2] += 1;
       return to_u8(rgb[0]) | (to_u8(rgb[1]) << 8) | (to_u8(rgb[2]) << 16);
     };
-    render_rgb8(p, b, kUvShaded, tid, nt, su, sv, color, planar);
+    render_rgb8(p, b, kUvShaded, tid, nt, su, sv, color, planar, tile);
   } else {
     float* out = reinterpret_cast<float*>(p.obs) + (int64_t)b * npix * 3;
     typedef MC_GLOBAL float gf32;
@@ -1822,6 +1863,9 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   if (go && ((npix * 3) & 15) == 0 && can_fill)
     fill = __builtin_amdgcn_readfirstlane(flat_img ? frames[e].flat_fill : frames[e].fill_block);
   uint32_t acc[3] = {0u, 0u, 0u};
+  // Per-wave staging of one 8x8 tile of uint8 pixels (render_rgb8).
+  __shared__ uint32_t s_tile[NT / kWave][kWave];
+  uint32_t* tile = s_tile[threadIdx.x >> 6];
   if (fill >= 0) {
     if (CNT && et == 0) acc[0] += npix, acc[2] += npix;
     const uint32_t rgb = flat_img ? flat[fill < kFlatSky ? fill : kFlatSky]
@@ -1843,17 +1887,20 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     const bool f32 = obs_mode == (CHW ? kObsF32Chw : kObsF32);
     if (uvm != kUvShaded && obs_mode == kObsU8Fp16)
       render_env_flat<LIFT, _Float16, CNT, GLB>(p, b, world, flat, f, op, uvm, et, ent, su, sv,
-                                                acc);
+                                                acc, tile);
     else if (uvm != kUvShaded)
-      render_env_flat<LIFT, float, CNT, GLB>(p, b, world, flat, f, op, uvm, et, ent, su, sv, acc);
+      render_env_flat<LIFT, float, CNT, GLB>(p, b, world, flat, f, op, uvm, et, ent, su, sv, acc,
+                                             tile);
     else if (f32 && p.vd_far)
       render_env<1, LIFT, true, CNT, GLB>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
     else if (f32)
       render_env<1, LIFT, false, CNT, GLB>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
     else if (p.vd_far)
-      render_env<0, LIFT, true, CNT, GLB>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
+      render_env<0, LIFT, true, CNT, GLB>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc,
+                                          tile);
     else
-      render_env<0, LIFT, false, CNT, GLB>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
+      render_env<0, LIFT, false, CNT, GLB>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc,
+                                           tile);
     if (!GLB && KIND == kItemStep && any_done)
       store_fresh_world<TER>(p, b, gworld, nullptr, nvox, et, ent);
   }
@@ -2083,7 +2130,8 @@ struct MultiArgs {
 };
 
 static_assert(sizeof(float) * (kPaletteLen + 2 * kMaxUV) + sizeof(uint32_t) * kFlatLen +
-                      sizeof(Frame) * (512 / kWave) + sizeof(int) * (kMaxGroups + 1 + 512 / kWave) + 64 <=
+                      sizeof(Frame) * (512 / kWave) + sizeof(int) * (kMaxGroups + 1 + 512 / kWave) +
+                      sizeof(uint32_t) * 512 + 64 <=
                   kStaticLdsReserve,
               "static LDS of the env kernels exceeds its reserve");
 
