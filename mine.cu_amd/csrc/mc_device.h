@@ -389,7 +389,36 @@ __device__ __forceinline__ void store_fresh_world(const Params& p, int b, int8_t
     }
     return;
   }
-  if ((nvox & 15) == 0) {
+  if ((p.Z & 15) == 0) {
+    // 16-voxel chunks along z: chunk c is (y, x, zq) with c = (y X + x) zc + zq.
+    // A row without tree or house voxels is its layer's block 16 times; the
+    // chunk coordinates advance by the thread stride with carries.
+    guint4* g = (guint4*)gworld;
+    v4u* l = reinterpret_cast<v4u*>(lds);
+    const int zc = p.Z >> 4, X = p.X, n16 = nvox >> 4;
+    int zq = tid % zc, yx = tid / zc;
+    int x = yx % X, y = yx / X;
+    const int dzq = nt % zc, dyx = nt / zc, dx = dyx % X, dy = dyx / X;
+    for (int c = tid; c < n16; c += nt) {
+      const int hy = y - p.ground_height, hx = x - p.house_x;
+      const bool special = ((x == p.tree_x) & (hy >= 0) & (hy <= 3)) |
+                           ((hx >= 0) & (hx < 5) & (hy >= 0) & (hy <= 4));
+      v4u v;
+      if (special) {
+        v = fresh_chunk(p, c << 4);
+      } else {
+        const uint32_t w = (uint32_t)(uint8_t)layer_block(y, p.ground_height) * 0x01010101u;
+        v = v4u{w, w, w, w};
+      }
+      g[c] = v;
+      if (lds) l[c] = v;
+      zq += dzq;
+      x += dx;
+      y += dy;
+      if (zq >= zc) { zq -= zc; ++x; }
+      if (x >= X) { x -= X; ++y; }
+    }
+  } else if ((nvox & 15) == 0) {
     guint4* g = (guint4*)gworld;
     v4u* l = reinterpret_cast<v4u*>(lds);
     for (int c = tid; c < (nvox >> 4); c += nt) {
