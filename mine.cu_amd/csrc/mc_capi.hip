@@ -1091,15 +1091,27 @@ int mc_launch_render_uint8_smem(const int8_t* voxels, const float* basis, uint8_
 namespace {
 constexpr int kEntryRing = 8;   // descriptor tables in flight
 
-using BucketKey = std::tuple<int, int, int, int, int, int>;  // X, Y, Z, W, H, obs_mode
+// X, Y, Z, W, H, obs_mode, world (terrain << 1 | collision)
+using BucketKey = std::tuple<int, int, int, int, int, int, int>;
 
 BucketKey bucket_of(const mc_config& c) {
-  return BucketKey{c.world_x, c.world_y, c.world_z, c.width, c.height, c.obs_mode};
+  return BucketKey{c.world_x, c.world_y, c.world_z, c.width, c.height, c.obs_mode,
+                   (c.terrain << 1) | (c.collision ? 1 : 0)};
 }
 
 struct Occupancy {
-  int nt, lift, lds, blocks;
+  int nt, lift, family, lds, blocks;
 };
+
+// Mode family of a group (mc_launch.h): each family's groups run kernels of
+// their own, launched back to back in one batched step.
+int family_of(const mc_config& c) {
+  if (c.obs_mode == MC_OBS_U8_CHW || c.obs_mode == MC_OBS_F32_CHW) return mcl::kFamilyPlanar;
+  if (c.collision)
+    return c.terrain == MC_TERRAIN_NOISE ? mcl::kFamilyCollisionTerrain : mcl::kFamilyCollision;
+  if (c.terrain == MC_TERRAIN_NOISE) return mcl::kFamilyTerrain;
+  return mcl::kFamilyFlat;
+}
 }  // namespace
 
 struct mc_batch {
@@ -1118,19 +1130,22 @@ struct mc_batch {
 };
 
 namespace {
-int occupancy_blocks(mc_batch* b, int nt, bool lift, int lds) {
+int occupancy_blocks(mc_batch* b, int nt, bool lift, int family, int lds) {
   for (const auto& o : b->occ)
-    if (o.nt == nt && o.lift == (int)lift && o.lds == lds) return o.blocks;
-  const int nb = mcl::kernels(nt, lift).multi_blocks((size_t)lds);
-  b->occ.push_back({nt, (int)lift, lds, nb});
+    if (o.nt == nt && o.lift == (int)lift && o.family == family && o.lds == lds) return o.blocks;
+  const int nb = mcl::kernels(nt, lift).multi_blocks(family, (size_t)lds);
+  b->occ.push_back({nt, (int)lift, family, lds, nb});
   return nb;
 }
 
-int launch_multi(mc_batch* b, int nt, bool lift, const mc::MultiArgs& a, int lds, hipStream_t st) {
-  const int grid = b->persistent
-                       ? std::max(1, std::min(a.total_items, occupancy_blocks(b, nt, lift, lds) * b->cus))
-                       : std::max(1, a.total_items);
-  const hipError_t e = mcl::kernels(nt, lift).multi(a, b->persistent != 0, grid, (size_t)lds, st);
+int launch_multi(mc_batch* b, int nt, bool lift, int family, const mc::MultiArgs& a, int lds,
+                 hipStream_t st) {
+  const int grid =
+      b->persistent
+          ? std::max(1, std::min(a.total_items, occupancy_blocks(b, nt, lift, family, lds) * b->cus))
+          : std::max(1, a.total_items);
+  const hipError_t e =
+      mcl::kernels(nt, lift).multi(a, family, b->persistent != 0, grid, (size_t)lds, st);
   if (e != hipSuccess) return fail(MC_E_HIP, "batched step launch: %s", hipGetErrorString(e));
   return MC_OK;
 }
@@ -1188,9 +1203,6 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
   if (!b || !envs || !actions || !action_kinds) return fail(MC_E_INVALID, "NULL argument");
   if (n < 1 || n > mc::kMaxGroups)
     return fail(MC_E_INVALID, "group count %d outside [1, %d]", n, mc::kMaxGroups);
-  // Launch width: 512 threads if some group wants 8 waves per env, else 256.
-  int nt = 256;
-  bool lift = false;
   for (int i = 0; i < n; ++i) {
     if (!envs[i] || !actions[i]) return fail(MC_E_INVALID, "group %d: NULL env/actions", i);
     const int k = action_kinds[i];
@@ -1202,82 +1214,105 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
       if (envs[j] == envs[i]) return fail(MC_E_INVALID, "group %d listed twice", i);
     if (envs[i]->graph_mode)
       return fail(MC_E_UNSUPPORTED, "group %d is in graph mode (its slot is device-resident)", i);
-    if (planar_mode(envs[i]->cfg.obs_mode) || envs[i]->cfg.terrain != MC_TERRAIN_FLAT ||
-        envs[i]->cfg.collision)
-      return fail(MC_E_UNSUPPORTED,
-                  "group %d: the MC_OBS_*_CHW modes, MC_TERRAIN_NOISE and collision step alone, "
-                  "not in a batch", i);
-    if (envs[i]->shape.wpe == 8) nt = 512;
-    lift = lift || envs[i]->shape.lift;
   }
+  // One launch per mode family present (flat-world interleaved groups, planar
+  // images, seeded terrain, voxel collision), each with its own width: 512
+  // threads if one of its groups wants 8 waves per env, else 256.
   struct G {
     int idx, wpe, items;
     double cost;
   };
-  std::vector<G> gs(n);
-  int lds = 0, lds_full = 0;  // the step launch; the full-height pass (0: none)
+  struct Fam {
+    std::vector<G> gs;
+    int nt = 256, lds = 0, lds_full = 0;
+    bool lift = false;
+  };
+  Fam fam[mcl::kFamilies];
   for (int i = 0; i < n; ++i) {
-    const mc_config& c = envs[i]->cfg;
-    const int ys = envs[i]->fb ? envs[i]->ys : c.world_y;  // layers the descriptor stages
-    Shape s{nt, std::min(envs[i]->shape.wpe, nt / mc::kWave), lift ? 1 : 0};
-    s = fit_lds(s, c.world_x, ys, c.world_z);
-    const int stride = mc::world_stride_bytes(c.world_x, c.world_y, c.world_z);
-    const int item_lds = mc::item_lds_bytes(epw_of(s), c.world_x, ys, c.world_z);
-    if (item_lds > lds_budget())
-      return fail(MC_E_UNSUPPORTED, "group %d: world does not fit the LDS budget", i);
-    lds = std::max(lds, item_lds);
-    if (envs[i]->fb) {
-      const int full = mc::item_lds_bytes(1, c.world_x, c.world_y, c.world_z);
-      if (full > lds_budget())
-        return fail(MC_E_UNSUPPORTED, "group %d: world does not fit the LDS budget", i);
-      lds_full = std::max(lds_full, full);
-    }
-    const int epw = epw_of(s);
-    const double per_env = (double)c.width * c.height * (1.0 + c.world_y / 16.0) + stride;
-    gs[i] = G{i, s.wpe, (c.batch_size + epw - 1) / epw, per_env * epw};
+    Fam& f = fam[family_of(envs[i]->cfg)];
+    if (envs[i]->shape.wpe == 8) f.nt = 512;
+    f.lift = f.lift || envs[i]->shape.lift;
+    f.gs.push_back(G{i, 0, 0, 0.0});
   }
-  // Most expensive items first, so the tail of the persistent loop is short.
-  std::stable_sort(gs.begin(), gs.end(), [](const G& x, const G& y) { return x.cost > y.cost; });
+  for (Fam& f : fam) {
+    for (G& g : f.gs) {
+      const int i = g.idx;
+      const mc_config& c = envs[i]->cfg;
+      const int ys = envs[i]->fb ? envs[i]->ys : c.world_y;  // layers the descriptor stages
+      Shape s{f.nt, std::min(envs[i]->shape.wpe, f.nt / mc::kWave), f.lift ? 1 : 0};
+      s = fit_lds(s, c.world_x, ys, c.world_z);
+      const int stride = mc::world_stride_bytes(c.world_x, c.world_y, c.world_z);
+      const int item_lds = mc::item_lds_bytes(epw_of(s), c.world_x, ys, c.world_z);
+      if (item_lds > lds_budget())
+        return fail(MC_E_UNSUPPORTED, "group %d: world does not fit the LDS budget", i);
+      f.lds = std::max(f.lds, item_lds);
+      if (envs[i]->fb) {
+        const int full = mc::item_lds_bytes(1, c.world_x, c.world_y, c.world_z);
+        if (full > lds_budget())
+          return fail(MC_E_UNSUPPORTED, "group %d: world does not fit the LDS budget", i);
+        f.lds_full = std::max(f.lds_full, full);
+      }
+      const int epw = epw_of(s);
+      const double per_env = (double)c.width * c.height * (1.0 + c.world_y / 16.0) + stride;
+      g = G{i, s.wpe, (c.batch_size + epw - 1) / epw, per_env * epw};
+    }
+    // Most expensive items first, so the tail of the persistent loop is short.
+    std::stable_sort(f.gs.begin(), f.gs.end(),
+                     [](const G& x, const G& y) { return x.cost > y.cost; });
+  }
   const int slot = b->next;
   b->next = (b->next + 1) % kEntryRing;
   if (b->ev_live[slot]) {
     const hipError_t e = hipEventSynchronize(b->ev[slot]);
     if (e != hipSuccess) return fail(MC_E_HIP, "batch table wait: %s", hipGetErrorString(e));
   }
+  // The families' entries lie one after another in this slot's table.
   mc::MultiEntry* he = b->host_entries + (size_t)slot * mc::kMaxGroups;
   mc::MultiEntry* de = b->dev_entries + (size_t)slot * mc::kMaxGroups;
-  int items = 0;
-  for (int k = 0; k < n; ++k) {
-    const int i = gs[k].idx;
-    mc::MultiEntry& m = he[k];
-    m.desc = envs[i]->desc;
-    m.actions = actions[i];
-    m.look = looks ? looks[i] : nullptr;
-    m.dones = dones ? dones[i] : nullptr;
-    m.action_kind = action_kinds[i];
-    m.slot = envs[i]->slot;
-    m.item_begin = items;
-    m.wpe = gs[k].wpe;
-    m.obs_mode = envs[i]->cfg.obs_mode;
-    m.pad = 0;
-    items += gs[k].items;
+  int at = 0;
+  int first[mcl::kFamilies], total[mcl::kFamilies];
+  for (int fi = 0; fi < mcl::kFamilies; ++fi) {
+    first[fi] = at;
+    int items = 0;
+    for (const G& g : fam[fi].gs) {
+      const int i = g.idx;
+      mc::MultiEntry& m = he[at++];
+      m.desc = envs[i]->desc;
+      m.actions = actions[i];
+      m.look = looks ? looks[i] : nullptr;
+      m.dones = dones ? dones[i] : nullptr;
+      m.action_kind = action_kinds[i];
+      m.slot = envs[i]->slot;
+      m.item_begin = items;
+      m.wpe = g.wpe;
+      m.obs_mode = envs[i]->cfg.obs_mode;
+      m.pad = 0;
+      items += g.items;
+    }
+    total[fi] = items;
   }
   hipError_t e = hipMemcpyAsync(de, he, sizeof(mc::MultiEntry) * n, hipMemcpyHostToDevice,
                                 S(stream));
   if (e != hipSuccess) return fail(MC_E_HIP, "batch table upload: %s", hipGetErrorString(e));
-  mc::MultiArgs a;
-  a.n_groups = n;
-  a.total_items = items;
-  a.counters = b->counters + 2 * slot;
-  a.entries = de;
-  const int r = launch_multi(b, nt, lift, a, lds, S(stream));
-  if (r) return r;
-  if (lds_full > 0) {
-    // Full-height pass over the envs the staged groups deferred.
-    const hipError_t ef = mcl::kernels(lds_full > 80 * 1024 ? 512 : 256, lift)
-                              .multi_fb(a, dim3(64), (size_t)lds_full, S(stream));
-    if (ef != hipSuccess)
-      return fail(MC_E_HIP, "batched deferred-env launch: %s", hipGetErrorString(ef));
+  for (int fi = 0; fi < mcl::kFamilies; ++fi) {
+    const Fam& f = fam[fi];
+    if (f.gs.empty()) continue;
+    mc::MultiArgs a;
+    a.n_groups = (int)f.gs.size();
+    a.total_items = total[fi];
+    // Launches of one slot run one after another on `stream`; each leaves the
+    // counters at zero for the next.
+    a.counters = b->counters + 2 * slot;
+    a.entries = de + first[fi];
+    const int r = launch_multi(b, f.nt, f.lift, fi, a, f.lds, S(stream));
+    if (r) return r;
+    if (f.lds_full > 0) {
+      // Full-height pass over the envs the family's staged groups deferred.
+      const hipError_t ef = mcl::kernels(f.lds_full > 80 * 1024 ? 512 : 256, f.lift)
+                                .multi_fb(a, fi, dim3(64), (size_t)f.lds_full, S(stream));
+      if (ef != hipSuccess)
+        return fail(MC_E_HIP, "batched deferred-env launch: %s", hipGetErrorString(ef));
+    }
   }
   e = hipEventRecord(b->ev[slot], S(stream));
   if (e != hipSuccess) return fail(MC_E_HIP, "batch event: %s", hipGetErrorString(e));
@@ -1333,11 +1368,8 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
     if (!envs[i]) return fail(MC_E_INVALID, "env %d is NULL", i);
     for (int j = 0; j < i; ++j)
       if (envs[j] == envs[i]) return fail(MC_E_INVALID, "group %d listed twice", i);
-    if (planar_mode(envs[i]->cfg.obs_mode) || envs[i]->cfg.terrain != MC_TERRAIN_FLAT ||
-        envs[i]->cfg.collision)
-      return fail(MC_E_UNSUPPORTED,
-                  "group %d: the MC_OBS_*_CHW modes, MC_TERRAIN_NOISE and collision step alone, "
-                  "not in a batch", i);
+    if (envs[i]->graph_mode)
+      return fail(MC_E_UNSUPPORTED, "group %d is in graph mode (its slot is device-resident)", i);
     const mc_config& c = envs[i]->cfg;
     const BucketKey k = bucket_of(c);
     const int j = (int)(std::find(keys.begin(), keys.end(), k) - keys.begin());
@@ -1372,7 +1404,8 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
     c.episode_length = -1;
     const int B = c.batch_size;
     const size_t nvox = (size_t)c.world_x * c.world_y * c.world_z;
-    const size_t obs_b = (size_t)c.width * c.height * 3 * (c.obs_mode == MC_OBS_F32 ? 4 : 1);
+    const size_t obs_b = (size_t)c.width * c.height * 3 *
+                         ((c.obs_mode == MC_OBS_F32 || c.obs_mode == MC_OBS_F32_CHW) ? 4 : 1);
     mir.emplace_back(new (std::nothrow) Mirror);
     if (!mir.back()) return fail(MC_E_INVALID, "out of host memory");
     Scratch& sc = mir.back()->sc;

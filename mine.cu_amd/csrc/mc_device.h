@@ -2168,7 +2168,9 @@ static_assert(sizeof(float) * (kPaletteLen + 2 * kMaxUV) + sizeof(uint32_t) * kF
 // items taken from an atomic counter; otherwise one workgroup per item (the
 // loop is compiled out; the hardware dispatcher then balances the load, and
 // the kernel needs fewer registers: 73 VGPRs against 118 on gfx950).
-template <int NT, bool LIFT, bool PERSIST>
+// CHW / TER / COL: a launch over groups of one mode family (planar images,
+// seeded terrain, voxel collision), as the single-group kernels separate them.
+template <int NT, bool LIFT, bool PERSIST, bool CHW = false, bool TER = false, bool COL = false>
 __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
@@ -2216,8 +2218,8 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
     p.slot = me.slot;
     const int wpe = me.wpe;
     const int epw = (NT / kWave) / wpe;
-    run_item<NT, kItemStep, LIFT>(p, me.obs_mode, (item - me.item_begin) * epw, wpe, s_lds, s_pal,
-                                  s_flat, s_frames, s_u, s_v, s_upper);
+    run_item<NT, kItemStep, LIFT, CHW, TER, COL>(p, me.obs_mode, (item - me.item_begin) * epw, wpe,
+                                                 s_lds, s_pal, s_flat, s_frames, s_u, s_v, s_upper);
     if (!PERSIST) break;
   }
   if (PERSIST && threadIdx.x == 0) {
@@ -2236,7 +2238,7 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
 // deferred, one env per workgroup with its whole world in LDS (the launch's
 // LDS fits the largest).  The last workgroup done with a group's list empties
 // it for the next launch on that slot.
-template <int NT, bool LIFT>
+template <int NT, bool LIFT, bool CHW = false, bool TER = false, bool COL = false>
 __global__ __launch_bounds__(NT) void k_multi_fb(MultiArgs a) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
@@ -2273,8 +2275,9 @@ __global__ __launch_bounds__(NT) void k_multi_fb(MultiArgs a) {
     p.ys = p.Y;
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
       __syncthreads();  // the previous env is done with LDS
-      run_item<NT, kItemStep, LIFT>(p, me.obs_mode, (int)list[2 + i], NT / kWave, s_lds, s_pal,
-                                    s_flat, s_frames, s_u, s_v, s_upper);
+      run_item<NT, kItemStep, LIFT, CHW, TER, COL>(p, me.obs_mode, (int)list[2 + i], NT / kWave,
+                                                   s_lds, s_pal, s_flat, s_frames, s_u, s_v,
+                                                   s_upper);
     }
     if (threadIdx.x == 0) {
       __threadfence();

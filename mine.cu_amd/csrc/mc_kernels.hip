@@ -75,28 +75,69 @@ hipError_t fb(const mc::Params& p, int obs_mode, dim3 grid, size_t lds, hipStrea
   return hipGetLastError();
 }
 
-hipError_t multi(const mc::MultiArgs& a, bool persist, int grid, size_t lds, hipStream_t st) {
-  if (persist)
-    hipLaunchKernelGGL((mc::k_multi<NT, LIFT, true>), dim3(grid), dim3(NT), lds, st, a);
-  else
-    hipLaunchKernelGGL((mc::k_multi<NT, LIFT, false>), dim3(grid), dim3(NT), lds, st, a);
+template <bool PERSIST, bool CHW, bool TER, bool COL>
+void multi_launch(const mc::MultiArgs& a, int grid, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((mc::k_multi<NT, LIFT, PERSIST, CHW, TER, COL>), dim3(grid), dim3(NT), lds, st,
+                     a);
+}
+
+template <bool PERSIST>
+void multi_family(const mc::MultiArgs& a, int family, int grid, size_t lds, hipStream_t st) {
+  switch (family) {
+    case kFamilyPlanar: multi_launch<PERSIST, true, false, false>(a, grid, lds, st); break;
+    case kFamilyTerrain: multi_launch<PERSIST, false, true, false>(a, grid, lds, st); break;
+    case kFamilyCollision: multi_launch<PERSIST, false, false, true>(a, grid, lds, st); break;
+    case kFamilyCollisionTerrain: multi_launch<PERSIST, false, true, true>(a, grid, lds, st); break;
+    default: multi_launch<PERSIST, false, false, false>(a, grid, lds, st); break;
+  }
+}
+
+hipError_t multi(const mc::MultiArgs& a, int family, bool persist, int grid, size_t lds,
+                 hipStream_t st) {
+  if (persist) multi_family<true>(a, family, grid, lds, st);
+  else multi_family<false>(a, family, grid, lds, st);
   return hipGetLastError();
 }
 
-hipError_t multi_fb(const mc::MultiArgs& a, dim3 grid, size_t lds, hipStream_t st) {
-  hipLaunchKernelGGL((mc::k_multi_fb<NT, LIFT>), grid, dim3(NT), lds, st, a);
+hipError_t multi_fb(const mc::MultiArgs& a, int family, dim3 grid, size_t lds, hipStream_t st) {
+  switch (family) {
+    case kFamilyPlanar:
+      hipLaunchKernelGGL((mc::k_multi_fb<NT, LIFT, true, false, false>), grid, dim3(NT), lds, st, a);
+      break;
+    case kFamilyTerrain:
+      hipLaunchKernelGGL((mc::k_multi_fb<NT, LIFT, false, true, false>), grid, dim3(NT), lds, st, a);
+      break;
+    case kFamilyCollision:
+      hipLaunchKernelGGL((mc::k_multi_fb<NT, LIFT, false, false, true>), grid, dim3(NT), lds, st, a);
+      break;
+    case kFamilyCollisionTerrain:
+      hipLaunchKernelGGL((mc::k_multi_fb<NT, LIFT, false, true, true>), grid, dim3(NT), lds, st, a);
+      break;
+    default: hipLaunchKernelGGL((mc::k_multi_fb<NT, LIFT>), grid, dim3(NT), lds, st, a); break;
+  }
   return hipGetLastError();
 }
 
-int multi_blocks(size_t lds) {
+template <bool CHW, bool TER, bool COL>
+int blocks_of(size_t lds) {
   int nb = 0;
-  const hipError_t e =
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mc::k_multi<NT, LIFT, true>, NT, lds);
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &nb, mc::k_multi<NT, LIFT, true, CHW, TER, COL>, NT, lds);
   if (e != hipSuccess || nb < 1) {
     (void)hipGetLastError();
     nb = 1;
   }
   return nb;
+}
+
+int multi_blocks(int family, size_t lds) {
+  switch (family) {
+    case kFamilyPlanar: return blocks_of<true, false, false>(lds);
+    case kFamilyTerrain: return blocks_of<false, true, false>(lds);
+    case kFamilyCollision: return blocks_of<false, false, true>(lds);
+    case kFamilyCollisionTerrain: return blocks_of<false, true, true>(lds);
+    default: return blocks_of<false, false, false>(lds);
+  }
 }
 #endif
 

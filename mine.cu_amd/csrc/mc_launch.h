@@ -13,6 +13,10 @@
 
 namespace mcl {
 
+// Mode families of batched groups: each runs kernels of its own.
+enum { kFamilyFlat = 0, kFamilyPlanar = 1, kFamilyTerrain = 2, kFamilyCollision = 3,
+       kFamilyCollisionTerrain = 4, kFamilies = 5 };
+
 struct KernelSet {
   // One item per workgroup over a group: grid = ceil(B / epw) with epw = NT /
   // (64 wpe).  Dispatches on the item kind, the obs mode (interleaved or
@@ -25,12 +29,14 @@ struct KernelSet {
   // Full-height pass over the envs a staged-height step deferred (NULL for
   // the 64-thread set).
   hipError_t (*fb)(const mc::Params& p, int obs_mode, dim3 grid, size_t lds, hipStream_t st);
-  // Several groups in one launch (NULL for the 64-thread set).
-  hipError_t (*multi)(const mc::MultiArgs& a, bool persist, int grid, size_t lds,
+  // Several groups of one mode family (kFamily*) in one launch (NULL for the
+  // 64-thread set).
+  hipError_t (*multi)(const mc::MultiArgs& a, int family, bool persist, int grid, size_t lds,
                       hipStream_t st);
-  hipError_t (*multi_fb)(const mc::MultiArgs& a, dim3 grid, size_t lds, hipStream_t st);
+  hipError_t (*multi_fb)(const mc::MultiArgs& a, int family, dim3 grid, size_t lds,
+                         hipStream_t st);
   // Resident workgroups per CU of the persistent multi kernel at `lds` bytes.
-  int (*multi_blocks)(size_t lds);
+  int (*multi_blocks)(int family, size_t lds);
   // Worlds too large for one workgroup's LDS: the item with the world in
   // global memory, one env per 256-thread workgroup; list: the envs a staged
   // step deferred (p.fb).  Non-NULL in the 256-thread lift set only.

@@ -341,10 +341,6 @@ int mc_server_add_group(mc_server* s, mc_env* env, int64_t bytes, int batch_size
   if (bytes > s->budget) return sfail(MC_E_INVALID, "group needs more than the whole budget");
   mc_config c{};
   if (mc_env_get_config(env, &c) != MC_OK) return sfail(MC_E_INVALID, "bad env handle");
-  if (c.obs_mode == MC_OBS_U8_CHW || c.obs_mode == MC_OBS_F32_CHW || c.terrain != MC_TERRAIN_FLAT ||
-      c.collision)
-    return sfail(MC_E_UNSUPPORTED, "the MC_OBS_*_CHW modes, MC_TERRAIN_NOISE and collision step "
-                                   "alone, not in a server group");
   Group g{};
   g.env = env;
   g.bytes = bytes;

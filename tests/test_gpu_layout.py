@@ -2,7 +2,7 @@
 shaded uint8 / float32 images written [B, 3, H, W], bit-identical to the
 oracle's [B, H, W, 3] images transposed, through reset, falling, landed
 (constant-image), regeneration and full-height (32-high world) steps and a
-state render.  Batches and server groups refuse planar envs."""
+state render."""
 import numpy as np
 import pytest
 import torch
@@ -69,31 +69,6 @@ def test_planar_render_matches_oracle():
     env._render()
     orc._render()
     _check(env, orc, "render")
-
-
-def test_planar_groups_step_alone_only():
-    """Batches and server groups run the interleaved kernels only: a planar env
-    is refused with an error, and nothing is launched."""
-    import minecu
-    from minecu import _lib
-    from minecu.serve import StepServer
-    kw = dict(batch_size=8, world_size=16, render_width=16, render_height=16, episode_length=9)
-    chw = minecu.MineEnv(device="cuda", obs_layout="chw", **kw)
-    hwc = minecu.MineEnv(device="cuda", **kw)
-    chw.reset()
-    hwc.reset()
-    a = torch.zeros(8, dtype=torch.int64, device="cuda")
-    batch = minecu.EnvBatch()
-    with pytest.raises(_lib.MineCuError, match="CHW"):
-        batch.step([hwc, chw], [a, a])
-    with pytest.raises(_lib.MineCuError, match="CHW"):
-        batch.autotune([chw], steps=2)
-    batch.step([hwc], [a])
-    batch.close()
-    srv = StepServer(hbm_budget_bytes=1 << 30)
-    with pytest.raises(_lib.MineCuError, match="CHW"):
-        srv.add_group(chw)
-    srv.close()
 
 
 def test_planar_layout_is_refused_for_flat_variants():

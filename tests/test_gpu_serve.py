@@ -165,3 +165,37 @@ def test_scheduled_open_loop_equals_alone():
     for i in range(len(SHAPES)):
         _same(_state(alone[i]), _state(served[i]), f"group {i}")
     srv.close()
+
+
+def test_served_mode_families_equal_alone():
+    """Planar, seeded-terrain and collision groups served beside a flat
+    interleaved group (closed-loop replay of discrete ids): each equals
+    stepping it alone, bit for bit."""
+    from minecu.serve import StepServer
+    kws = [
+        SHAPES[0],
+        dict(batch_size=24, world_size=16, render_width=32, render_height=24, obs_dtype="float32",
+             obs_layout="chw", episode_length=7),
+        dict(batch_size=32, world_size=32, render_width=32, render_height=32, obs_dtype="uint8",
+             terrain="noise", terrain_amplitude=5, seed=5, episode_length=8),
+        dict(batch_size=20, world_size=32, render_width=24, render_height=16, obs_dtype="uint8",
+             terrain="noise", terrain_amplitude=3, collision=True, seed=6, episode_length=10),
+    ]
+    T = 16
+    served = [_make(kw) for kw in kws]
+    alone = [_make(kw) for kw in kws]
+    acts = [_acts(T, kw["batch_size"], 70 + i) for i, kw in enumerate(kws)]
+    torch.cuda.synchronize()
+    srv = StepServer(n_streams=2)
+    gids = [srv.add_group(e) for e in served]
+    for g, a in zip(gids, acts):
+        srv.replay(g, a)
+    srv.drain()
+    assert len(srv.records()) == T * len(kws)
+    for i, e in enumerate(alone):
+        for t in range(T):
+            e.step(acts[i][t])
+    torch.cuda.synchronize()
+    for i in range(len(kws)):
+        _same(_state(alone[i]), _state(served[i]), f"group {i}")
+    srv.close()

@@ -2,8 +2,7 @@
 (collision=True) on the GPU: worlds, positions,
 rewards, dones and images equal the oracle's bit for bit through reset,
 steps, done steps with regeneration, the staged-height fast path and the
-full-height pass; amplitude 0 steps exactly like the flat world; batches and
-server groups refuse terrain envs."""
+full-height pass; amplitude 0 steps exactly like the flat world."""
 import numpy as np
 import pytest
 import torch
@@ -91,25 +90,15 @@ def test_amplitude_zero_steps_like_the_flat_world():
         assert torch.equal(getattr(flat, k), getattr(noise, k)), k
 
 
-def test_seeds_give_different_worlds_and_batches_refuse_terrain():
+def test_seeds_give_different_worlds():
     import minecu
     from minecu import _lib
-    from minecu.serve import StepServer
     kw = dict(batch_size=4, world_size=16, render_width=16, render_height=16, episode_length=5)
     a = minecu.MineEnv(device="cuda", terrain="noise", seed=1, **kw)
     b = minecu.MineEnv(device="cuda", terrain="noise", seed=2, **kw)
     a.reset()
     b.reset()
     assert not torch.equal(a.voxels, b.voxels)
-    batch = minecu.EnvBatch()
-    z = torch.zeros(4, dtype=torch.int64, device="cuda")
-    with pytest.raises(_lib.MineCuError, match="TERRAIN"):
-        batch.step([a], [z])
-    batch.close()
-    srv = StepServer(hbm_budget_bytes=1 << 30)
-    with pytest.raises(_lib.MineCuError, match="TERRAIN"):
-        srv.add_group(a)
-    srv.close()
     with pytest.raises(ValueError):
         minecu.MineEnv(device="cuda", terrain="noise", obs_layout="chw", **kw)
     with pytest.raises(_lib.MineCuError):
