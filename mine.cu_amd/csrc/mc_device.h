@@ -1329,6 +1329,221 @@ __device__ __forceinline__ void render_env_flat(const Params& p, int b, const in
   render_rgb8(p, b, uvm, tid, nt, su, sv, color, false, tile);
 }
 
+// ---------------------------------------------------------------------------
+// The shaded uint8 render of an env whose eye voxel lies in the world
+// (trace()'s in-world branch for every pixel), with u and v from the tables
+// and the tile stores of render_rgb8: the same arithmetic as trace(), laid out
+// for instruction count.  The env's camera frame, face distances and address
+// steps are held in vector registers for the whole image (as scalars they
+// outnumber the scalar file and are reloaded per pixel); the ray set-up tests
+// its fast-path range once, on the smallest component; the DDA block decides
+// the hit itself (dda_in), so no per-lane state outlives it.
+// ---------------------------------------------------------------------------
+
+// Unit direction and reciprocals of a shaded ray (ray_dir<false>): the fast
+// path takes rays whose components all have magnitude at least 2^-30 (then
+// len^2 >= 3 * 2^-60 and no component or quotient is 0) and len^2 <= 2^60,
+// a subset of ray_dir's; a NaN or infinite component fails the len^2 test.
+__device__ __forceinline__ void ray_dir_in(float& x, float& y, float& z, float& ix, float& iy,
+                                           float& iz) {
+  const float len2 = x * x + y * y + z * z;
+  const float m = fminf(fminf(fabsf(x), fabsf(y)), fabsf(z));
+  if ((m >= 0x1p-30f) & (len2 <= 0x1p60f)) {
+    const float len = sqrt_core(len2);
+    const float rl = recip_core(len);
+    x = div_core(x, len, rl);
+    y = div_core(y, len, rl);
+    z = div_core(z, len, rl);
+    ix = rcp_cr(x);
+    iy = rcp_cr(y);
+    iz = rcp_cr(z);
+  } else {
+    ray_dir<false>(x, y, z, ix, iy, iz, 1e30f);
+  }
+}
+
+// dda_loop with the run test and the hit decision in the same block, for an
+// eye voxel of air, max_steps > 0 and view_distance > 0.  Entry: the lanes
+// whose starting packed position is in the world.  Exit:
+// the lanes that ran max_steps iterations are those active at the start of
+// the last iteration when the counter ended the loop (all lanes when
+// max_steps <= 0, as trace()'s steps >= max_steps).  Returns the voxel byte
+// of a hit (in the world, not limited by max_steps, and without VDF t < vd:
+// trace()'s !lim_end & in-world test) or 0xffffffff; the caller's test
+// `< 13` then also applies trace()'s valid-block test (air bytes are >= 128).
+template <bool VDF>
+__device__ __forceinline__ uint32_t dda_in(float& tx, float& ty, float& tz, float& t, float ix,
+                                           float iy, float iz, uint32_t va, uint32_t dx,
+                                           uint32_t dy, uint32_t dz, uint32_t& pos,
+                                           uint32_t& pinc, uint32_t blk,
+                                           int max_steps, float vd) {
+  uint64_t save, sa, mz, my, mc, mv;
+  uint32_t d, w, step, hb;
+  const uint32_t k10 = 1u << 10, k20 = 1u << 20, kg = (1u << 9) | (1u << 19) | (1u << 29);
+#define MC_DDA_IN_ASM(VDCMP, VDAND, VDEND, VDHIT)                                 \
+  asm volatile(                                                                    \
+      "s_mov_b64 %[save], exec\n\t"                                               \
+      "s_mov_b64 %[sa], exec\n\t"                                                 \
+      "s_mov_b32 %[step], 0\n\t"                                                  \
+      "v_and_b32_e32 %[w], %[kg], %[pos]\n\t"                                     \
+      "v_cmp_eq_u32_e32 vcc, 0, %[w]\n\t"                                         \
+      "s_and_b64 exec, exec, vcc\n\t"                                             \
+      "s_cbranch_execz .Ldin_end%=\n"                                              \
+      ".Ldin_loop%=:\n\t"                                                         \
+      "v_min3_f32 %[t], %[tx], %[ty], %[tz]\n\t"                                  \
+      "s_add_u32 %[step], %[step], 1\n\t"                                         \
+      "v_cmp_eq_f32_e64 %[mz], %[tz], %[t]\n\t"                                   \
+      "v_cmp_eq_f32_e64 %[my], %[ty], %[t]\n\t" VDCMP                             \
+      "s_mov_b64 %[sa], exec\n\t"                                                 \
+      "s_andn2_b64 %[my], %[my], %[mz]\n\t"                                       \
+      "s_or_b64 %[mc], %[my], %[mz]\n\t"                                          \
+      "v_cndmask_b32_e64 %[d], %[dx], %[dy], %[my]\n\t"                           \
+      "v_cndmask_b32_e64 %[d], %[d], %[dz], %[mz]\n\t"                            \
+      "v_add_u32_e32 %[va], %[va], %[d]\n\t"                                      \
+      "ds_read_u8 %[blk], %[va]\n\t"                                              \
+      "v_cndmask_b32_e64 %[pi], 1, %[k10], %[my]\n\t"                             \
+      "v_cndmask_b32_e64 %[pi], %[pi], %[k20], %[mz]\n\t"                         \
+      "v_add_u32_e32 %[pos], %[pos], %[pi]\n\t"                                   \
+      "s_andn2_b64 exec, %[sa], %[mc]\n\t"                                        \
+      "v_add_f32_e64 %[tx], %[tx], |%[ix]|\n\t"                                   \
+      "s_mov_b64 exec, %[my]\n\t"                                                 \
+      "v_add_f32_e64 %[ty], %[ty], |%[iy]|\n\t"                                   \
+      "s_mov_b64 exec, %[mz]\n\t"                                                 \
+      "v_add_f32_e64 %[tz], %[tz], |%[iz]|\n\t"                                   \
+      "s_mov_b64 exec, %[sa]\n\t"                                                 \
+      "v_and_b32_e32 %[w], %[kg], %[pos]\n\t"                                     \
+      "s_waitcnt lgkmcnt(0)\n\t"                                                  \
+      "v_sub_u32_e32 %[w], %[blk], %[w]\n\t"                                      \
+      "v_cmp_lt_i32_e32 vcc, 0x7f, %[w]\n\t"                                      \
+      "s_and_b64 exec, exec, vcc\n\t" VDAND                                      \
+      "s_cmp_ge_i32 %[step], %[ms]\n\t"                                           \
+      "s_cbranch_scc1 .Ldin_end%=\n\t"                                            \
+      "s_cbranch_execnz .Ldin_loop%=\n"                                            \
+      ".Ldin_end%=:\n\t"                                                          \
+      "s_cmp_ge_i32 %[step], %[ms]\n\t"                                           \
+      "s_cselect_b64 %[sa], %[sa], 0\n\t"                                         \
+      "s_mov_b64 exec, %[save]\n\t"                                               \
+      "v_and_b32_e32 %[w], %[kg], %[pos]\n\t"                                     \
+      "v_cmp_eq_u32_e64 %[mz], 0, %[w]\n\t" VDEND                                 \
+      "s_andn2_b64 %[mz], %[mz], %[sa]\n\t" VDHIT                                 \
+      "s_nop 1\n\t"                                                               \
+      "v_cndmask_b32_e64 %[hb], -1, %[blk], %[mz]"                                \
+      : [tx] "+v"(tx), [ty] "+v"(ty), [tz] "+v"(tz), [t] "+v"(t), [va] "+v"(va),     \
+        [pos] "+v"(pos), [pi] "+v"(pinc), [blk] "+v"(blk), [d] "=&v"(d),             \
+        [w] "=&v"(w), [hb] "=&v"(hb), [save] "=&s"(save), [sa] "=&s"(sa),            \
+        [mz] "=&s"(mz), [my] "=&s"(my), [mc] "=&s"(mc), [mv] "=&s"(mv),              \
+        [step] "=&s"(step)                                                          \
+      : [ix] "v"(ix), [iy] "v"(iy), [iz] "v"(iz), [dx] "v"(dx), [dy] "v"(dy),        \
+        [dz] "v"(dz), [k10] "v"(k10), [k20] "v"(k20), [kg] "s"(kg),                  \
+        [ms] "s"(max_steps), [vd] "s"(vd)                                           \
+      : "vcc", "scc")
+  if (VDF)
+    MC_DDA_IN_ASM("", "", "", "");
+  else
+    MC_DDA_IN_ASM("v_cmp_gt_f32_e64 %[mv], %[vd], %[t]\n\t", "s_and_b64 exec, exec, %[mv]\n\t",
+                  "v_cmp_gt_f32_e64 %[mv], %[vd], %[t]\n\t", "s_and_b64 %[mz], %[mz], %[mv]\n\t");
+#undef MC_DDA_IN_ASM
+  return hb;
+}
+
+template <bool LIFT, bool VDF>
+__device__ __forceinline__ void render_in_rgb8(const Params& p, uint8_t* out,
+                                               const int8_t* world, const float* pal,
+                                               const Frame& f, const OctPos& op, int tid, int nt,
+                                               const float* su, const float* sv,
+                                               uint32_t* tile) {
+  typedef __attribute__((address_space(3))) const uint8_t lds_u8;
+  typedef __attribute__((address_space(3))) const uint32_t lds_u32;
+  const int X = p.X, Z = p.Z, XZ = X * Z;
+  float fx = f.fx, fy = f.fy, fz = f.fz, rx = f.rx, rz = f.rz, ux = f.ux, uy = f.uy, uz = f.uz;
+  float dxn = f.dxn, dxp = f.dxp, dyn = f.dyn, dyp = f.dyp, dzn = f.dzn, dzp = f.dzp;
+  uint32_t zp = (uint32_t)Z, zn = (uint32_t)-Z, xzp = (uint32_t)XZ, xzn = (uint32_t)-XZ;
+  const uint32_t va0 = (uint32_t)(uintptr_t)(lds_u8*)(world + (f.ivy * XZ + f.ivx * Z + f.ivz));
+  const uint32_t ob = (uint32_t)(uintptr_t)(lds_u32*)op.v;
+  const uint32_t blk0 = __builtin_amdgcn_readfirstlane((uint32_t)*(lds_u8*)(uintptr_t)va0);
+  const float vd = p.view_distance;
+  const int max_steps = p.max_steps;
+  const uint32_t go =
+      __builtin_amdgcn_readfirstlane((((int8_t)blk0 < 0) & (max_steps > 0) & (vd > 0.0f)) ? 1u : 0u);
+  const float sk0 = pal[kSkyBase], sk1 = pal[kSkyBase + 1], sk2 = pal[kSkyBase + 2];
+  const uint32_t sky = to_u8(sk0) | (to_u8(sk1) << 8) | (to_u8(sk2) << 16);
+  // Uniform values moved to vector registers (see above).
+  asm volatile("" : "+v"(fx), "+v"(fy), "+v"(fz), "+v"(rx), "+v"(rz), "+v"(ux), "+v"(uy),
+               "+v"(uz));
+  asm volatile("" : "+v"(dxn), "+v"(dxp), "+v"(dyn), "+v"(dyp), "+v"(dzn), "+v"(dzp));
+  asm volatile("" : "+v"(zp), "+v"(zn), "+v"(xzp), "+v"(xzn));
+  auto color = [&](float u, float v) -> uint32_t {
+    float ray_x = fx + u * rx + v * ux;
+    float ray_y = fy + v * uy;
+    float ray_z = fz + u * rz + v * uz;
+    float inv_x, inv_y, inv_z;
+    ray_dir_in(ray_x, ray_y, ray_z, inv_x, inv_y, inv_z);
+    const bool bx = ray_x >= 0, by = ray_y >= 0, bz = ray_z >= 0;
+    // (d*n + 1) = d*p exactly (trace()).
+    float tm_x = (bx ? dxp : dxn) * inv_x;
+    float tm_y = (by ? dyp : dyn) * inv_y;
+    float tm_z = (bz ? dzp : dzn) * inv_z;
+    uint32_t pos = *(lds_u32*)(uintptr_t)(ob + (bx ? 4u : 0u) + (by ? 8u : 0u) + (bz ? 16u : 0u));
+    uint32_t pinc = 0u;
+    float t = 0.0f;
+    uint32_t hb;
+    if (go) {
+      hb = dda_in<VDF>(tm_x, tm_y, tm_z, t, inv_x, inv_y, inv_z, va0, bx ? zp : zn,
+                       by ? xzp : xzn, bz ? 1u : 0xffffffffu, pos, pinc, blk0, max_steps, vd);
+    } else {
+      // No iteration (trace(): steps 0, so lim_end is max_steps <= 0 or t = 0 >= vd):
+      // the eye voxel, if solid, is hit at t = 0 with no face.
+      hb = (((pos & ((1u << 9) | (1u << 19) | (1u << 29))) == 0u) & (max_steps > 0) &
+            !(0.0f >= vd))
+               ? blk0
+               : 0xffffffffu;
+    }
+    uint32_t c = sky;
+    if (hb < 13u) {
+      float shade = (pinc == 1u) ? 0.8f : 1.0f;
+      shade = (pinc == (1u << 20)) ? 0.9f : shade;
+      shade = ((pinc == (1u << 10)) & !by) ? 0.6f : shade;
+      float fog = fminf(p.vd_pow2 ? t * p.inv_vd : t / vd, 1.0f);
+      fog = fog * fog;
+      const float keep = 1.0f - fog;
+      const float* hc = pal + hb * 3;
+      c = to_u8(hc[0] * shade * keep + sk0 * fog) | (to_u8(hc[1] * shade * keep + sk1 * fog) << 8) |
+          (to_u8(hc[2] * shade * keep + sk2 * fog) << 16);
+    }
+    return c;
+  };
+  // render_rgb8's tile loop (tables, width a multiple of 8, dword-aligned image).
+  const int lane = tid & (kWave - 1), nwv = nt >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int tx = lane & 7, ty = lane >> 3;
+  const int tcols = p.W >> 3, trows = (p.H + 7) >> 3, ntiles = tcols * trows;
+  const int r = lane / 6, d = lane - 6 * r, dm = d % 3;
+  const int a = r * 8 + (d / 3) * 4 + dm;
+  const uint32_t sel = (dm == 0) ? 0x04020100u : (dm == 1) ? 0x05040201u : 0x06050402u;
+  const uint32_t row_off = (uint32_t)(r * p.W * 3 + 4 * d);
+  const int dr = nwv / tcols, dc = nwv - dr * tcols;
+  int tr = wv / tcols, tc = wv - tr * tcols;
+  typedef MC_GLOBAL uint32_t gu32;
+  for (int tt = wv; tt < ntiles; tt += nwv) {
+    const int py = tr * 8 + ty, px = tc * 8 + tx;
+    uint32_t c = 0u;
+    if (py < p.H) c = color(su[px], sv[py]);
+    tile[lane] = c;
+    __builtin_amdgcn_wave_barrier();
+    if ((lane < 48) & (tr * 8 + r < p.H)) {
+      const uint32_t w = __builtin_amdgcn_perm(tile[a + 1], tile[a], sel);
+      *(gu32*)(out + (uint32_t)((tr * 8 * p.W + tc * 8) * 3) + row_off) = w;
+    }
+    __builtin_amdgcn_wave_barrier();
+    tr += dr;
+    tc += dc;
+    if (tc >= tcols) {
+      tc -= tcols;
+      ++tr;
+    }
+  }
+}
+
 // planar: the image is written [3, H, W] (the MC_OBS_*_CHW modes).  Callers
 // pass a compile-time constant, so each kernel holds one store form only.
 template <int OBS, bool LIFT, bool VDF, bool CNT = false, bool GLB = false>
@@ -1349,6 +1564,11 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
                     (to_u8(pal[f.fill_block * 3 + 2]) << 16),
                 tid, nt);
       if (CNT && tid == 0) acc[0] += npix, acc[2] += npix;
+      return;
+    }
+    if (!CNT && !GLB && !planar && tile && f.inside && f.fill_block < 0 && tab && (W & 7) == 0 &&
+        (((uintptr_t)out & 3u) == 0)) {
+      render_in_rgb8<LIFT, VDF>(p, out, world, pal, f, op, tid, nt, su, sv, tile);
       return;
     }
     // Packed 0x00BBGGRR per pixel.
