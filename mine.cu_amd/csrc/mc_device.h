@@ -1371,21 +1371,23 @@ __device__ __forceinline__ void ray_dir_in(float& x, float& y, float& z, float& 
 // of a hit (in the world, not limited by max_steps, and without VDF t < vd:
 // trace()'s !lim_end & in-world test) or 0xffffffff; the caller's test
 // `< 13` then also applies trace()'s valid-block test (air bytes are >= 128).
+// t, pinc and the voxel byte are written by the first iteration; a lane that
+// runs none has its start position outside the world, so its result is -1 and
+// they are not read.
 template <bool VDF>
 __device__ __forceinline__ uint32_t dda_in(float& tx, float& ty, float& tz, float& t, float ix,
                                            float iy, float iz, uint32_t va, uint32_t dx,
                                            uint32_t dy, uint32_t dz, uint32_t& pos,
-                                           uint32_t& pinc, uint32_t blk,
-                                           int max_steps, float vd) {
-  uint64_t save, sa, mz, my, mc, mv;
-  uint32_t d, w, step, hb;
-  const uint32_t k10 = 1u << 10, k20 = 1u << 20, kg = (1u << 9) | (1u << 19) | (1u << 29);
+                                           uint32_t& pinc, int max_steps, float vd) {
+  uint64_t save, sa, mz, my, mv;
+  uint32_t d, w, step, hb, blk;
+  const uint32_t k10 = 1u << 10, k20 = 1u << 20;  // the guard bits 9, 19, 29 are a literal
 #define MC_DDA_IN_ASM(VDCMP, VDAND, VDEND, VDHIT)                                 \
   asm volatile(                                                                    \
       "s_mov_b64 %[save], exec\n\t"                                               \
       "s_mov_b64 %[sa], exec\n\t"                                                 \
       "s_mov_b32 %[step], 0\n\t"                                                  \
-      "v_and_b32_e32 %[w], %[kg], %[pos]\n\t"                                     \
+      "v_and_b32_e32 %[w], 0x20080200, %[pos]\n\t"                                     \
       "v_cmp_eq_u32_e32 vcc, 0, %[w]\n\t"                                         \
       "s_and_b64 exec, exec, vcc\n\t"                                             \
       "s_cbranch_execz .Ldin_end%=\n"                                              \
@@ -1396,7 +1398,6 @@ __device__ __forceinline__ uint32_t dda_in(float& tx, float& ty, float& tz, floa
       "v_cmp_eq_f32_e64 %[my], %[ty], %[t]\n\t" VDCMP                             \
       "s_mov_b64 %[sa], exec\n\t"                                                 \
       "s_andn2_b64 %[my], %[my], %[mz]\n\t"                                       \
-      "s_or_b64 %[mc], %[my], %[mz]\n\t"                                          \
       "v_cndmask_b32_e64 %[d], %[dx], %[dy], %[my]\n\t"                           \
       "v_cndmask_b32_e64 %[d], %[d], %[dz], %[mz]\n\t"                            \
       "v_add_u32_e32 %[va], %[va], %[d]\n\t"                                      \
@@ -1404,14 +1405,15 @@ __device__ __forceinline__ uint32_t dda_in(float& tx, float& ty, float& tz, floa
       "v_cndmask_b32_e64 %[pi], 1, %[k10], %[my]\n\t"                             \
       "v_cndmask_b32_e64 %[pi], %[pi], %[k20], %[mz]\n\t"                         \
       "v_add_u32_e32 %[pos], %[pos], %[pi]\n\t"                                   \
-      "s_andn2_b64 exec, %[sa], %[mc]\n\t"                                        \
+      "s_andn2_b64 exec, %[sa], %[my]\n\t"                                        \
+      "s_andn2_b64 exec, exec, %[mz]\n\t"                                         \
       "v_add_f32_e64 %[tx], %[tx], |%[ix]|\n\t"                                   \
       "s_mov_b64 exec, %[my]\n\t"                                                 \
       "v_add_f32_e64 %[ty], %[ty], |%[iy]|\n\t"                                   \
       "s_mov_b64 exec, %[mz]\n\t"                                                 \
       "v_add_f32_e64 %[tz], %[tz], |%[iz]|\n\t"                                   \
       "s_mov_b64 exec, %[sa]\n\t"                                                 \
-      "v_and_b32_e32 %[w], %[kg], %[pos]\n\t"                                     \
+      "v_and_b32_e32 %[w], 0x20080200, %[pos]\n\t"                                     \
       "s_waitcnt lgkmcnt(0)\n\t"                                                  \
       "v_sub_u32_e32 %[w], %[blk], %[w]\n\t"                                      \
       "v_cmp_lt_i32_e32 vcc, 0x7f, %[w]\n\t"                                      \
@@ -1423,18 +1425,18 @@ __device__ __forceinline__ uint32_t dda_in(float& tx, float& ty, float& tz, floa
       "s_cmp_ge_i32 %[step], %[ms]\n\t"                                           \
       "s_cselect_b64 %[sa], %[sa], 0\n\t"                                         \
       "s_mov_b64 exec, %[save]\n\t"                                               \
-      "v_and_b32_e32 %[w], %[kg], %[pos]\n\t"                                     \
+      "v_and_b32_e32 %[w], 0x20080200, %[pos]\n\t"                                     \
       "v_cmp_eq_u32_e64 %[mz], 0, %[w]\n\t" VDEND                                 \
       "s_andn2_b64 %[mz], %[mz], %[sa]\n\t" VDHIT                                 \
       "s_nop 1\n\t"                                                               \
       "v_cndmask_b32_e64 %[hb], -1, %[blk], %[mz]"                                \
-      : [tx] "+v"(tx), [ty] "+v"(ty), [tz] "+v"(tz), [t] "+v"(t), [va] "+v"(va),     \
-        [pos] "+v"(pos), [pi] "+v"(pinc), [blk] "+v"(blk), [d] "=&v"(d),             \
+      : [tx] "+v"(tx), [ty] "+v"(ty), [tz] "+v"(tz), [t] "=&v"(t), [va] "+v"(va),     \
+        [pos] "+v"(pos), [pi] "=&v"(pinc), [blk] "=&v"(blk), [d] "=&v"(d),           \
         [w] "=&v"(w), [hb] "=&v"(hb), [save] "=&s"(save), [sa] "=&s"(sa),            \
-        [mz] "=&s"(mz), [my] "=&s"(my), [mc] "=&s"(mc), [mv] "=&s"(mv),              \
+        [mz] "=&s"(mz), [my] "=&s"(my), [mv] "=&s"(mv),                              \
         [step] "=&s"(step)                                                          \
       : [ix] "v"(ix), [iy] "v"(iy), [iz] "v"(iz), [dx] "v"(dx), [dy] "v"(dy),        \
-        [dz] "v"(dz), [k10] "v"(k10), [k20] "v"(k20), [kg] "s"(kg),                  \
+        [dz] "v"(dz), [k10] "v"(k10), [k20] "v"(k20),                                \
         [ms] "s"(max_steps), [vd] "s"(vd)                                           \
       : "vcc", "scc")
   if (VDF)
@@ -1446,12 +1448,13 @@ __device__ __forceinline__ uint32_t dda_in(float& tx, float& ty, float& tz, floa
   return hb;
 }
 
-template <bool LIFT, bool VDF>
-__device__ __forceinline__ void render_in_rgb8(const Params& p, uint8_t* out,
-                                               const int8_t* world, const float* pal,
-                                               const Frame& f, const OctPos& op, int tid, int nt,
-                                               const float* su, const float* sv,
-                                               uint32_t* tile) {
+// OBS 0: uint8 (interleaved: render_rgb8's tile stores, width a multiple of 8
+// and a dword-aligned image); OBS 1: float32; planar: [3, H, W] per pixel.
+template <int OBS, bool LIFT, bool VDF>
+__device__ __forceinline__ void render_in(const Params& p, void* out_v, const int8_t* world,
+                                          const float* pal, const Frame& f, const OctPos& op,
+                                          int tid, int nt, const float* su, const float* sv,
+                                          bool planar, uint32_t* tile) {
   typedef __attribute__((address_space(3))) const uint8_t lds_u8;
   typedef __attribute__((address_space(3))) const uint32_t lds_u32;
   const int X = p.X, Z = p.Z, XZ = X * Z;
@@ -1461,18 +1464,25 @@ __device__ __forceinline__ void render_in_rgb8(const Params& p, uint8_t* out,
   const uint32_t va0 = (uint32_t)(uintptr_t)(lds_u8*)(world + (f.ivy * XZ + f.ivx * Z + f.ivz));
   const uint32_t ob = (uint32_t)(uintptr_t)(lds_u32*)op.v;
   const uint32_t blk0 = __builtin_amdgcn_readfirstlane((uint32_t)*(lds_u8*)(uintptr_t)va0);
-  const float vd = p.view_distance;
-  const int max_steps = p.max_steps;
+  // Scalars used inside the pixel loop, as values of their own (not parts of
+  // the parameter block, which the register allocator reloads as a whole).
+  float vd = p.view_distance;
+  int max_steps = p.max_steps, W = p.W, H = p.H;
+  asm volatile("" : "+s"(vd), "+s"(max_steps), "+s"(W), "+s"(H));
   const uint32_t go =
       __builtin_amdgcn_readfirstlane((((int8_t)blk0 < 0) & (max_steps > 0) & (vd > 0.0f)) ? 1u : 0u);
   const float sk0 = pal[kSkyBase], sk1 = pal[kSkyBase + 1], sk2 = pal[kSkyBase + 2];
   const uint32_t sky = to_u8(sk0) | (to_u8(sk1) << 8) | (to_u8(sk2) << 16);
+  const int pow2 = __builtin_amdgcn_readfirstlane(p.vd_pow2);
+  float ivd = p.inv_vd;
   // Uniform values moved to vector registers (see above).
   asm volatile("" : "+v"(fx), "+v"(fy), "+v"(fz), "+v"(rx), "+v"(rz), "+v"(ux), "+v"(uy),
                "+v"(uz));
   asm volatile("" : "+v"(dxn), "+v"(dxp), "+v"(dyn), "+v"(dyp), "+v"(dzn), "+v"(dzp));
-  asm volatile("" : "+v"(zp), "+v"(zn), "+v"(xzp), "+v"(xzn));
-  auto color = [&](float u, float v) -> uint32_t {
+  asm volatile("" : "+v"(zp), "+v"(zn), "+v"(xzp), "+v"(xzn), "+v"(ivd));
+  // The shaded colour of the ray through (u, v), as trace() computes it.
+  // Returns whether the ray hits a block (else the colour is the sky's).
+  auto shade_ray = [&](float u, float v, float& c0, float& c1, float& c2) -> bool {
     float ray_x = fx + u * rx + v * ux;
     float ray_y = fy + v * uy;
     float ray_z = fz + u * rz + v * uz;
@@ -1489,7 +1499,7 @@ __device__ __forceinline__ void render_in_rgb8(const Params& p, uint8_t* out,
     uint32_t hb;
     if (go) {
       hb = dda_in<VDF>(tm_x, tm_y, tm_z, t, inv_x, inv_y, inv_z, va0, bx ? zp : zn,
-                       by ? xzp : xzn, bz ? 1u : 0xffffffffu, pos, pinc, blk0, max_steps, vd);
+                       by ? xzp : xzn, bz ? 1u : 0xffffffffu, pos, pinc, max_steps, vd);
     } else {
       // No iteration (trace(): steps 0, so lim_end is max_steps <= 0 or t = 0 >= vd):
       // the eye voxel, if solid, is hit at t = 0 with no face.
@@ -1498,55 +1508,99 @@ __device__ __forceinline__ void render_in_rgb8(const Params& p, uint8_t* out,
                ? blk0
                : 0xffffffffu;
     }
-    uint32_t c = sky;
-    if (hb < 13u) {
+    c0 = sk0;
+    c1 = sk1;
+    c2 = sk2;
+    const bool hit = hb < 13u;
+    if (hit) {
       float shade = (pinc == 1u) ? 0.8f : 1.0f;
       shade = (pinc == (1u << 20)) ? 0.9f : shade;
       shade = ((pinc == (1u << 10)) & !by) ? 0.6f : shade;
-      float fog = fminf(p.vd_pow2 ? t * p.inv_vd : t / vd, 1.0f);
+      float fog = fminf(pow2 ? t * ivd : t / vd, 1.0f);
       fog = fog * fog;
       const float keep = 1.0f - fog;
-      const float* hc = pal + hb * 3;
-      c = to_u8(hc[0] * shade * keep + sk0 * fog) | (to_u8(hc[1] * shade * keep + sk1 * fog) << 8) |
-          (to_u8(hc[2] * shade * keep + sk2 * fog) << 16);
+      // hb * 3 floats as a 24-bit product (a full 32-bit multiply issues at a
+      // quarter of the rate).
+      uint32_t hoff;
+      asm("v_mul_u32_u24_e32 %0, 12, %1" : "=v"(hoff) : "v"(hb));
+      const float* hc = (const float*)((const char*)pal + hoff);
+      c0 = hc[0] * shade * keep + sk0 * fog;
+      c1 = hc[1] * shade * keep + sk1 * fog;
+      c2 = hc[2] * shade * keep + sk2 * fog;
     }
-    return c;
+    return hit;
   };
-  // render_rgb8's tile loop (tables, width a multiple of 8, dword-aligned image).
-  const int lane = tid & (kWave - 1), nwv = nt >> 6;
-  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int tx = lane & 7, ty = lane >> 3;
-  const int tcols = p.W >> 3, trows = (p.H + 7) >> 3, ntiles = tcols * trows;
-  const int r = lane / 6, d = lane - 6 * r, dm = d % 3;
-  const int a = r * 8 + (d / 3) * 4 + dm;
-  const uint32_t sel = (dm == 0) ? 0x04020100u : (dm == 1) ? 0x05040201u : 0x06050402u;
-  const uint32_t row_off = (uint32_t)(r * p.W * 3 + 4 * d);
-  const int dr = nwv / tcols, dc = nwv - dr * tcols;
-  int tr = wv / tcols, tc = wv - tr * tcols;
-  typedef MC_GLOBAL uint32_t gu32;
-  for (int tt = wv; tt < ntiles; tt += nwv) {
-    const int py = tr * 8 + ty, px = tc * 8 + tx;
-    uint32_t c = 0u;
-    if (py < p.H) c = color(su[px], sv[py]);
-    tile[lane] = c;
-    __builtin_amdgcn_wave_barrier();
-    if ((lane < 48) & (tr * 8 + r < p.H)) {
-      const uint32_t w = __builtin_amdgcn_perm(tile[a + 1], tile[a], sel);
-      *(gu32*)(out + (uint32_t)((tr * 8 * p.W + tc * 8) * 3) + row_off) = w;
+  if (OBS == 0 && !planar) {
+    // render_rgb8's tile loop.
+    uint8_t* out = reinterpret_cast<uint8_t*>(out_v);
+    const int lane = tid & (kWave - 1), nwv = nt >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int tx = lane & 7, ty = lane >> 3;
+    const int tcols = W >> 3, trows = (H + 7) >> 3, ntiles = tcols * trows;
+    const int r = lane / 6, d = lane - 6 * r, dm = d % 3;
+    const int a = r * 8 + (d / 3) * 4 + dm;
+    const uint32_t sel = (dm == 0) ? 0x04020100u : (dm == 1) ? 0x05040201u : 0x06050402u;
+    const uint32_t row_off = (uint32_t)(r * W * 3 + 4 * d);
+    const int dr = nwv / tcols, dc = nwv - dr * tcols;
+    int tr = wv / tcols, tc = wv - tr * tcols;
+    typedef MC_GLOBAL uint32_t gu32;
+    for (int tt = wv; tt < ntiles; tt += nwv) {
+      const int py = tr * 8 + ty, px = tc * 8 + tx;
+      uint32_t c = 0u;
+      if (py < H) {
+        float c0, c1, c2;
+        c = shade_ray(su[px], sv[py], c0, c1, c2)
+                ? to_u8(c0) | (to_u8(c1) << 8) | (to_u8(c2) << 16)
+                : sky;
+      }
+      tile[lane] = c;
+      __builtin_amdgcn_wave_barrier();
+      if ((lane < 48) & (tr * 8 + r < H)) {
+        const uint32_t w = __builtin_amdgcn_perm(tile[a + 1], tile[a], sel);
+        *(gu32*)(out + (uint32_t)((tr * 8 * W + tc * 8) * 3) + row_off) = w;
+      }
+      __builtin_amdgcn_wave_barrier();
+      tr += dr;
+      tc += dc;
+      if (tc >= tcols) {
+        tc -= tcols;
+        ++tr;
+      }
     }
-    __builtin_amdgcn_wave_barrier();
-    tr += dr;
-    tc += dc;
-    if (tc >= tcols) {
-      tc -= tcols;
-      ++tr;
-    }
+    return;
   }
+  const int npix = W * H;
+  for_tiles(p, tid, nt, [&](int px, int py) {
+    float c0, c1, c2;
+    shade_ray(su[px], sv[py], c0, c1, c2);
+    if (OBS == 0) {
+      typedef MC_GLOBAL uint8_t gu8;
+      gu8* o = (gu8*)(reinterpret_cast<uint8_t*>(out_v) + py * W + px);
+      o[0] = (uint8_t)to_u8(c0);
+      o[npix] = (uint8_t)to_u8(c1);
+      o[2 * npix] = (uint8_t)to_u8(c2);
+    } else if (planar) {
+      typedef MC_GLOBAL float gf32;
+      gf32* o = (gf32*)(reinterpret_cast<float*>(out_v) + (int64_t)py * W + px);
+      o[0] = c0;
+      o[npix] = c1;
+      o[2 * npix] = c2;
+    } else {
+      typedef MC_GLOBAL float gf32;
+      gf32* o = (gf32*)(reinterpret_cast<float*>(out_v) + ((int64_t)py * W + px) * 3);
+      o[0] = c0;
+      o[1] = c1;
+      o[2] = c2;
+    }
+  });
 }
 
 // planar: the image is written [3, H, W] (the MC_OBS_*_CHW modes).  Callers
 // pass a compile-time constant, so each kernel holds one store form only.
-template <int OBS, bool LIFT, bool VDF, bool CNT = false, bool GLB = false>
+// IN32: float32 images of an in-world eye through render_in (the kernels
+// built for the float32 modes only: in a kernel that also holds the uint8
+// path, its registers would cost the uint8 render more than it gains).
+template <int OBS, bool LIFT, bool VDF, bool CNT = false, bool GLB = false, bool IN32 = false>
 __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t* world,
                                            const float* pal, const Frame& f, const OctPos& op,
                                            int tid, int nt, const float* su, const float* sv,
@@ -1566,9 +1620,9 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       if (CNT && tid == 0) acc[0] += npix, acc[2] += npix;
       return;
     }
-    if (!CNT && !GLB && !planar && tile && f.inside && f.fill_block < 0 && tab && (W & 7) == 0 &&
-        (((uintptr_t)out & 3u) == 0)) {
-      render_in_rgb8<LIFT, VDF>(p, out, world, pal, f, op, tid, nt, su, sv, tile);
+    if (!CNT && !GLB && f.inside && f.fill_block < 0 && tab &&
+        (planar || (tile && (W & 7) == 0 && (((uintptr_t)out & 3u) == 0)))) {
+      render_in<0, LIFT, VDF>(p, out, world, pal, f, op, tid, nt, su, sv, planar, tile);
       return;
     }
     // Packed 0x00BBGGRR per pixel.
@@ -1588,6 +1642,10 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
     render_rgb8(p, b, kUvShaded, tid, nt, su, sv, color, planar, tile);
   } else {
     float* out = reinterpret_cast<float*>(p.obs) + (int64_t)b * npix * 3;
+    if (IN32 && !CNT && !GLB && f.inside && f.fill_block < 0 && tab) {
+      render_in<1, LIFT, VDF>(p, out, world, pal, f, op, tid, nt, su, sv, planar, tile);
+      return;
+    }
     typedef MC_GLOBAL float gf32;
     for_tiles(p, tid, nt, [&](int px, int py) {
       float rgb[3];
@@ -2025,8 +2083,10 @@ __host__ __device__ __forceinline__ int item_lds_bytes(int epw, int X, int Y, in
 // GLB: the env's world stays in global memory (worlds that do not fit LDS):
 // one env per workgroup, nothing staged, and the world rewritten on a done
 // step only after the whole workgroup has rendered.
+// F32: a kernel for the float32 modes only (MC_OBS_F32, and with CHW
+// MC_OBS_F32_CHW), whose render is render_env's IN32 form.
 template <int NT, int KIND, bool LIFT, bool CHW = false, bool TER = false, bool COL = false,
-          bool CNT = false, bool GLB = false>
+          bool CNT = false, bool GLB = false, bool F32 = false>
 __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, int wpe,
                                          int8_t* lds, const float* pal, const uint32_t* flat,
                                          Frame* frames, float* su, float* sv, int* upper) {
@@ -2134,7 +2194,12 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     const Frame f = uniform_frame(&frames[e]);
     const OctPos op{frames[e].pos0};
     const bool f32 = obs_mode == (CHW ? kObsF32Chw : kObsF32);
-    if (uvm != kUvShaded && obs_mode == kObsU8Fp16)
+    if (F32 && p.vd_far)
+      render_env<1, LIFT, true, CNT, GLB, true>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
+    else if (F32)
+      render_env<1, LIFT, false, CNT, GLB, true>(p, b, world, pal, f, op, et, ent, su, sv, CHW,
+                                                 acc);
+    else if (uvm != kUvShaded && obs_mode == kObsU8Fp16)
       render_env_flat<LIFT, _Float16, CNT, GLB>(p, b, world, flat, f, op, uvm, et, ent, su, sv,
                                                 acc, tile);
     else if (uvm != kUvShaded)
@@ -2178,7 +2243,7 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
 // One group, one item per workgroup: grid = ceil(B / epw).  CHW: the planar
 // image modes, in kernels of their own.  CNT: the measurement build of the
 // step, counting ray steps into p.ray_steps.
-template <int NT, int KIND, bool LIFT, bool CHW = false, bool CNT = false>
+template <int NT, int KIND, bool LIFT, bool CHW = false, bool CNT = false, bool F32 = false>
 __global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
@@ -2188,8 +2253,8 @@ __global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
   __shared__ int s_upper[NT / kWave];
   load_palette(s_pal, s_flat, threadIdx.x);
   const int epw = (NT / kWave) / wpe;
-  run_item<NT, KIND, LIFT, CHW, false, false, CNT>(p, obs_mode, blockIdx.x * epw, wpe, s_lds,
-                                                  s_pal, s_flat, s_frames, s_u, s_v, s_upper);
+  run_item<NT, KIND, LIFT, CHW, false, false, CNT, false, F32>(
+      p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_flat, s_frames, s_u, s_v, s_upper);
 }
 
 // Full-height pass after a staged-height step launch (Params::ys): the envs

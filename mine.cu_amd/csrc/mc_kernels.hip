@@ -23,9 +23,9 @@ template <int KIND>
 void launch_kind(const mc::Params& p, int obs_mode, int wpe, size_t lds, hipStream_t st) {
   const int epw = (NT / mc::kWave) / wpe;
   const dim3 grid((p.B + epw - 1) / epw);
-  // The planar modes, the seeded terrain and voxel collision run kernels of
-  // their own (template constants), so the flat-world interleaved kernels keep
-  // their code unchanged.
+  // The planar modes, the float32 modes, the seeded terrain and voxel
+  // collision run kernels of their own (template constants), so the
+  // flat-world interleaved uint8 kernels keep their code unchanged.
   if (KIND == mc::kItemStep && p.collision && p.terrain)
     hipLaunchKernelGGL((mc::k_env_col<NT, LIFT, true>), grid, dim3(NT), lds, st, p, obs_mode, wpe);
   else if (KIND == mc::kItemStep && p.collision)
@@ -33,9 +33,17 @@ void launch_kind(const mc::Params& p, int obs_mode, int wpe, size_t lds, hipStre
   else if (p.terrain)
     hipLaunchKernelGGL((mc::k_env_terrain<NT, KIND, LIFT>), grid, dim3(NT), lds, st, p, obs_mode,
                        wpe);
+  else if (KIND != mc::kItemRenderCameras && KIND != mc::kItemRenderBasis &&
+           obs_mode == mc::kObsF32Chw)
+    hipLaunchKernelGGL((mc::k_env<NT, KIND, LIFT, true, false, true>), grid, dim3(NT), lds, st, p,
+                       obs_mode, wpe);
   else if (KIND != mc::kItemRenderCameras && KIND != mc::kItemRenderBasis && planar(obs_mode))
     hipLaunchKernelGGL((mc::k_env<NT, KIND, LIFT, true>), grid, dim3(NT), lds, st, p, obs_mode,
                        wpe);
+  else if (KIND != mc::kItemRenderCameras && KIND != mc::kItemRenderBasis &&
+           obs_mode == mc::kObsF32)
+    hipLaunchKernelGGL((mc::k_env<NT, KIND, LIFT, false, false, true>), grid, dim3(NT), lds, st,
+                       p, obs_mode, wpe);
   else
     hipLaunchKernelGGL((mc::k_env<NT, KIND, LIFT>), grid, dim3(NT), lds, st, p, obs_mode, wpe);
 }
