@@ -58,9 +58,11 @@ def measured_traffic(workload: str, obs: str):
 
 WORKLOADS = {
     # name: (batch, world, W, H, episode_length, action mix, seed, steps)
-    "c2": dict(batch=32768, world=16, W=32, H=32, episode_length=60, mix="uniform", seed=1),
+    "c2": dict(batch=32768, world=16, W=32, H=32, episode_length=60, mix="uniform", seed=1,
+               config_steps=1000),
     "c1": dict(batch=256, world=16, W=64, H=64, episode_length=None, mix="uniform", seed=0),
-    "c3": dict(batch=4096, world=32, W=64, H=64, episode_length=60, mix="break", seed=2),
+    "c3": dict(batch=4096, world=32, W=64, H=64, episode_length=60, mix="break", seed=2,
+               config_steps=500),
 }
 
 
@@ -415,6 +417,8 @@ def main():
     ap.add_argument("--no-ray-count", action="store_true",
                     help="skip the ray-step counting pass")
     ap.add_argument("--no-f32", action="store_true", help="skip the float32-obs side measurement")
+    ap.add_argument("--no-config-run", action="store_true",
+                    help="skip the run over the workload's own step count")
     ap.add_argument("--terrain", default="flat", choices=["flat", "noise"],
                     help="world of the main measurement (default: the reference's flat world)")
     ap.add_argument("--collision", action="store_true",
@@ -506,6 +510,16 @@ def main():
                           "eye is inside a solid counts its one lookup), counted by a build of "
                           "the step kernel over the same timed window; per second against the "
                           "production kernel's timing"}
+    if ref_world and not args.no_config_run and K != w.get("config_steps", K):
+        # The workload's own length (the C2 configuration names 1000 steps,
+        # C3 500): about 16 (C2) or 8 (C3) whole episodes, so the falling
+        # phase after each respawn and the standing phase weigh as they do
+        # over a long run.  A short window starting just before a done step is
+        # mostly falling-phase steps, whose renders trace every ray.
+        n_full = w["config_steps"]
+        f_ms, _, _ = run_gpu(w, args.obs, n_full, args.warmup)
+        out["config_length_run"] = {"steps": n_full, "value": B * n_full / (f_ms / 1e3),
+                                    "ms_per_step": f_ms / n_full}
     if args.obs == "uint8" and not args.no_f32:
         f_ms, _, f_launch = run_gpu(w, "float32", min(K, 200), args.warmup)
         out["float32_obs_value"] = B * min(K, 200) / (f_ms / 1e3)

@@ -231,7 +231,7 @@ bool valid_shape(Shape s) {
 // steps 7% slower, 7% overall; at 32^3 / 64x64 1.8x.
 Shape default_shape(int W, int H, int X, int Y, int Z) {
   int wpe = ((int64_t)W * H <= 1024) ? 1 : 4;
-  const int per_cu = std::max(1, mc::kLdsPerWorkgroup / mc::item_lds_bytes(1, X, Y, Z));
+  const int per_cu = std::max(1, mc::kLdsPerWorkgroup / mc::item_lds_bytes(1, X, Y, Z, W, H));
   while (wpe < 8 && per_cu * wpe < 16) wpe = (wpe == 1) ? 4 : 8;
   return Shape{wpe * mc::kWave, wpe, 1};
 }
@@ -243,8 +243,8 @@ Shape default_shape(const mc_config& c) {
 int epw_of(Shape s) { return (s.nt / mc::kWave) / s.wpe; }
 
 // Raise wpe until the envs of one workgroup fit the LDS budget.
-Shape fit_lds(Shape s, int X, int Y, int Z) {
-  while (mc::item_lds_bytes(epw_of(s), X, Y, Z) > lds_budget() && s.wpe * 64 < s.nt)
+Shape fit_lds(Shape s, int X, int Y, int Z, int W, int H) {
+  while (mc::item_lds_bytes(epw_of(s), X, Y, Z, W, H) > lds_budget() && s.wpe * 64 < s.nt)
     s.wpe *= 2;
   return s;
 }
@@ -268,8 +268,8 @@ int launch_env(int kind, const mc::Params& q, int obs_mode, Shape s, hipStream_t
   if (planar_mode(obs_mode) && kind != mc::kItemStep && kind != mc::kItemResetRender &&
       kind != mc::kItemRenderState)
     return fail(MC_E_UNSUPPORTED, "the MC_OBS_*_CHW modes render from MineEnv state only");
-  s = fit_lds(s, p.X, p.ys, p.Z);
-  if (mc::item_lds_bytes(epw_of(s), p.X, p.ys, p.Z) > lds_budget()) {
+  s = fit_lds(s, p.X, p.ys, p.Z, p.W, p.H);
+  if (mc::item_lds_bytes(epw_of(s), p.X, p.ys, p.Z, p.W, p.H) > lds_budget()) {
     // Too large for LDS even at one env per workgroup: the world stays in
     // global memory (whole height; no counting build there).
     if (count) return fail(MC_E_UNSUPPORTED, "ray-step counting needs an LDS-resident world");
@@ -280,7 +280,7 @@ int launch_env(int kind, const mc::Params& q, int obs_mode, Shape s, hipStream_t
     if (e != hipSuccess) return fail(MC_E_HIP, "global-world launch: %s", hipGetErrorString(e));
     return MC_OK;
   }
-  const size_t lds = (size_t)mc::item_lds_bytes(epw_of(s), p.X, p.ys, p.Z);
+  const size_t lds = (size_t)mc::item_lds_bytes(epw_of(s), p.X, p.ys, p.Z, p.W, p.H);
   const mcl::KernelSet& ks = mcl::kernels(s.nt, s.lift != 0);
   const hipError_t e = count ? ks.env_count(p, obs_mode, s.wpe, lds, st)
                              : ks.env(kind, p, obs_mode, s.wpe, lds, st);
@@ -293,7 +293,7 @@ int launch_env(int kind, const mc::Params& q, int obs_mode, Shape s, hipStream_t
 int launch_deferred(const mc::Params& q, int obs_mode, int lift, hipStream_t st) {
   mc::Params p = q;
   p.ys = p.Y;
-  const int full = mc::item_lds_bytes(1, p.X, p.Y, p.Z);
+  const int full = mc::item_lds_bytes(1, p.X, p.Y, p.Z, p.W, p.H);
   if (full > lds_budget()) {
     // The whole world does not fit LDS: the deferred envs run from global
     // memory.
@@ -658,7 +658,8 @@ int mc_env_set_shape(mc_env* env, int threads, int waves_per_env, int lift) {
 
 int mc_env_get_shape(const mc_env* env, int* threads, int* waves_per_env, int* lift) {
   if (!env || !threads || !waves_per_env) return fail(MC_E_INVALID, "NULL argument");
-  const Shape s = fit_lds(env->shape, env->cfg.world_x, env->cfg.world_y, env->cfg.world_z);
+  const Shape s = fit_lds(env->shape, env->cfg.world_x, env->cfg.world_y, env->cfg.world_z,
+                          env->cfg.width, env->cfg.height);
   *threads = s.nt;
   *waves_per_env = s.wpe;
   if (lift) *lift = s.lift;
@@ -1240,14 +1241,16 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
       const mc_config& c = envs[i]->cfg;
       const int ys = envs[i]->fb ? envs[i]->ys : c.world_y;  // layers the descriptor stages
       Shape s{f.nt, std::min(envs[i]->shape.wpe, f.nt / mc::kWave), f.lift ? 1 : 0};
-      s = fit_lds(s, c.world_x, ys, c.world_z);
+      s = fit_lds(s, c.world_x, ys, c.world_z, c.width, c.height);
       const int stride = mc::world_stride_bytes(c.world_x, c.world_y, c.world_z);
-      const int item_lds = mc::item_lds_bytes(epw_of(s), c.world_x, ys, c.world_z);
+      const int item_lds =
+          mc::item_lds_bytes(epw_of(s), c.world_x, ys, c.world_z, c.width, c.height);
       if (item_lds > lds_budget())
         return fail(MC_E_UNSUPPORTED, "group %d: world does not fit the LDS budget", i);
       f.lds = std::max(f.lds, item_lds);
       if (envs[i]->fb) {
-        const int full = mc::item_lds_bytes(1, c.world_x, c.world_y, c.world_z);
+        const int full =
+            mc::item_lds_bytes(1, c.world_x, c.world_y, c.world_z, c.width, c.height);
         if (full > lds_budget())
           return fail(MC_E_UNSUPPORTED, "group %d: world does not fit the LDS budget", i);
         f.lds_full = std::max(f.lds_full, full);
@@ -1508,7 +1511,8 @@ int mc_batch_autotune(mc_batch* b, mc_env* const* envs, int n, int steps, void* 
       if (r) break;
       const Shape& now = cur.shape[j];
       if (cand.nt == now.nt && cand.wpe == now.wpe && cand.lift == now.lift) continue;
-      if (fit_lds(cand, X, Y, Z).wpe != cand.wpe) continue;  // does not fit as asked
+      const int W = std::get<3>(keys[j]), H = std::get<4>(keys[j]);
+      if (fit_lds(cand, X, Y, Z, W, H).wpe != cand.wpe) continue;  // does not fit as asked
       Config next = cur;
       next.shape[j] = cand;
       float tc = 0.0f, tn = 0.0f;

@@ -1344,9 +1344,10 @@ __device__ __forceinline__ void render_env_flat(const Params& p, int b, const in
 // path takes rays whose components all have magnitude at least 2^-30 (then
 // len^2 >= 3 * 2^-60 and no component or quotient is 0) and len^2 <= 2^60,
 // a subset of ray_dir's; a NaN or infinite component fails the len^2 test.
-__device__ __forceinline__ void ray_dir_in(float& x, float& y, float& z, float& ix, float& iy,
-                                           float& iz) {
-  const float len2 = x * x + y * y + z * z;
+// yy: y * y, computed once per image row by the caller.
+__device__ __forceinline__ void ray_dir_in(float& x, float& y, float& z, float yy, float& ix,
+                                           float& iy, float& iz) {
+  const float len2 = x * x + yy + z * z;
   const float m = fminf(fminf(fabsf(x), fabsf(y)), fabsf(z));
   if ((m >= 0x1p-30f) & (len2 <= 0x1p60f)) {
     const float len = sqrt_core(len2);
@@ -1451,14 +1452,18 @@ __device__ __forceinline__ uint32_t dda_in(float& tx, float& ty, float& tz, floa
 // OBS 0: uint8 (interleaved: render_rgb8's tile stores, width a multiple of 8
 // and a dword-aligned image); OBS 1: float32; planar: [3, H, W] per pixel.
 template <int OBS, bool LIFT, bool VDF>
+// colA: the env's column terms of its camera rays, (fx + u rx, fz + u rz)
+// per column (run_item); the row terms (v ux, v uz, fy + v uy and the
+// square of the latter) are formed once per tile row, so a ray costs two
+// additions (the reference's sums, same operands and order).
 __device__ __forceinline__ void render_in(const Params& p, void* out_v, const int8_t* world,
                                           const float* pal, const Frame& f, const OctPos& op,
-                                          int tid, int nt, const float* su, const float* sv,
+                                          int tid, int nt, const float2* colA, const float* sv,
                                           bool planar, uint32_t* tile) {
   typedef __attribute__((address_space(3))) const uint8_t lds_u8;
   typedef __attribute__((address_space(3))) const uint32_t lds_u32;
   const int X = p.X, Z = p.Z, XZ = X * Z;
-  float fx = f.fx, fy = f.fy, fz = f.fz, rx = f.rx, rz = f.rz, ux = f.ux, uy = f.uy, uz = f.uz;
+  const float fy = f.fy, ux = f.ux, uy = f.uy, uz = f.uz;
   float dxn = f.dxn, dxp = f.dxp, dyn = f.dyn, dyp = f.dyp, dzn = f.dzn, dzp = f.dzp;
   uint32_t zp = (uint32_t)Z, zn = (uint32_t)-Z, xzp = (uint32_t)XZ, xzn = (uint32_t)-XZ;
   const uint32_t va0 = (uint32_t)(uintptr_t)(lds_u8*)(world + (f.ivy * XZ + f.ivx * Z + f.ivz));
@@ -1476,18 +1481,27 @@ __device__ __forceinline__ void render_in(const Params& p, void* out_v, const in
   const int pow2 = __builtin_amdgcn_readfirstlane(p.vd_pow2);
   float ivd = p.inv_vd;
   // Uniform values moved to vector registers (see above).
-  asm volatile("" : "+v"(fx), "+v"(fy), "+v"(fz), "+v"(rx), "+v"(rz), "+v"(ux), "+v"(uy),
-               "+v"(uz));
   asm volatile("" : "+v"(dxn), "+v"(dxp), "+v"(dyn), "+v"(dyp), "+v"(dzn), "+v"(dzp));
   asm volatile("" : "+v"(zp), "+v"(zn), "+v"(xzp), "+v"(xzn), "+v"(ivd));
-  // The shaded colour of the ray through (u, v), as trace() computes it.
-  // Returns whether the ray hits a block (else the colour is the sky's).
-  auto shade_ray = [&](float u, float v, float& c0, float& c1, float& c2) -> bool {
-    float ray_x = fx + u * rx + v * ux;
-    float ray_y = fy + v * uy;
-    float ray_z = fz + u * rz + v * uz;
+  // Row terms of the current tile row (row_terms).
+  float rbx = 0.0f, rbz = 0.0f, rcy = 0.0f, ryy = 0.0f;
+  auto row_terms = [&](int py) {
+    const float v = sv[py];
+    rbx = v * ux;
+    rbz = v * uz;
+    rcy = fy + v * uy;
+    ryy = rcy * rcy;
+  };
+  // The shaded colour of the ray through column px of the current row, as
+  // trace() computes it.  Returns whether the ray hits a block (else the
+  // colour is the sky's).
+  auto shade_ray = [&](int px, float& c0, float& c1, float& c2) -> bool {
+    const float2 a = colA[px];
+    float ray_x = a.x + rbx;
+    float ray_y = rcy;
+    float ray_z = a.y + rbz;
     float inv_x, inv_y, inv_z;
-    ray_dir_in(ray_x, ray_y, ray_z, inv_x, inv_y, inv_z);
+    ray_dir_in(ray_x, ray_y, ray_z, ryy, inv_x, inv_y, inv_z);
     const bool bx = ray_x >= 0, by = ray_y >= 0, bz = ray_z >= 0;
     // (d*n + 1) = d*p exactly (trace()).
     float tm_x = (bx ? dxp : dxn) * inv_x;
@@ -1543,15 +1557,18 @@ __device__ __forceinline__ void render_in(const Params& p, void* out_v, const in
     const uint32_t row_off = (uint32_t)(r * W * 3 + 4 * d);
     const int dr = nwv / tcols, dc = nwv - dr * tcols;
     int tr = wv / tcols, tc = wv - tr * tcols;
+    int row_tr = -1;
     typedef MC_GLOBAL uint32_t gu32;
     for (int tt = wv; tt < ntiles; tt += nwv) {
       const int py = tr * 8 + ty, px = tc * 8 + tx;
+      if (tr != row_tr) {
+        row_tr = tr;
+        row_terms(min(py, H - 1));
+      }
       uint32_t c = 0u;
       if (py < H) {
         float c0, c1, c2;
-        c = shade_ray(su[px], sv[py], c0, c1, c2)
-                ? to_u8(c0) | (to_u8(c1) << 8) | (to_u8(c2) << 16)
-                : sky;
+        c = shade_ray(px, c0, c1, c2) ? to_u8(c0) | (to_u8(c1) << 8) | (to_u8(c2) << 16) : sky;
       }
       tile[lane] = c;
       __builtin_amdgcn_wave_barrier();
@@ -1570,9 +1587,29 @@ __device__ __forceinline__ void render_in(const Params& p, void* out_v, const in
     return;
   }
   const int npix = W * H;
-  for_tiles(p, tid, nt, [&](int px, int py) {
+  // for_tiles' walk over the tiles, with the row terms of each tile row.
+  const int lane = tid & (kWave - 1), nwv = nt >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int tx = lane & 7, ty = lane >> 3;
+  const int tcols = (W + 7) >> 3, ntiles = tcols * ((H + 7) >> 3);
+  const int dr = nwv / tcols, dc = nwv - dr * tcols;
+  int tr = wv / tcols, tc = wv - tr * tcols;
+  int row_tr = -1;
+  for (int tt = wv; tt < ntiles; tt += nwv) {
+    const int py = tr * 8 + ty, px = tc * 8 + tx;
+    if (tr != row_tr) {
+      row_tr = tr;
+      row_terms(min(py, H - 1));
+    }
+    tr += dr;
+    tc += dc;
+    if (tc >= tcols) {
+      tc -= tcols;
+      ++tr;
+    }
+    if (!((py < H) & (px < W))) continue;
     float c0, c1, c2;
-    shade_ray(su[px], sv[py], c0, c1, c2);
+    shade_ray(px, c0, c1, c2);
     if (OBS == 0) {
       typedef MC_GLOBAL uint8_t gu8;
       gu8* o = (gu8*)(reinterpret_cast<uint8_t*>(out_v) + py * W + px);
@@ -1592,7 +1629,7 @@ __device__ __forceinline__ void render_in(const Params& p, void* out_v, const in
       o[1] = c1;
       o[2] = c2;
     }
-  });
+  }
 }
 
 // planar: the image is written [3, H, W] (the MC_OBS_*_CHW modes).  Callers
@@ -1605,7 +1642,8 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
                                            const float* pal, const Frame& f, const OctPos& op,
                                            int tid, int nt, const float* su, const float* sv,
                                            bool planar, uint32_t* acc = nullptr,
-                                           uint32_t* tile = nullptr) {
+                                           uint32_t* tile = nullptr,
+                                           const float2* colA = nullptr) {
   const int W = p.W, npix = W * p.H;
   const bool tab = uv_tabled(p);
   auto ucol = [&](int px) { return tab ? su[px] : u_of(p, kUvShaded, px); };
@@ -1620,9 +1658,9 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
       if (CNT && tid == 0) acc[0] += npix, acc[2] += npix;
       return;
     }
-    if (!CNT && !GLB && f.inside && f.fill_block < 0 && tab &&
+    if (!CNT && !GLB && colA && f.inside && f.fill_block < 0 && tab &&
         (planar || (tile && (W & 7) == 0 && (((uintptr_t)out & 3u) == 0)))) {
-      render_in<0, LIFT, VDF>(p, out, world, pal, f, op, tid, nt, su, sv, planar, tile);
+      render_in<0, LIFT, VDF>(p, out, world, pal, f, op, tid, nt, colA, sv, planar, tile);
       return;
     }
     // Packed 0x00BBGGRR per pixel.
@@ -1642,8 +1680,8 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
     render_rgb8(p, b, kUvShaded, tid, nt, su, sv, color, planar, tile);
   } else {
     float* out = reinterpret_cast<float*>(p.obs) + (int64_t)b * npix * 3;
-    if (IN32 && !CNT && !GLB && f.inside && f.fill_block < 0 && tab) {
-      render_in<1, LIFT, VDF>(p, out, world, pal, f, op, tid, nt, su, sv, planar, tile);
+    if (IN32 && !CNT && !GLB && colA && f.inside && f.fill_block < 0 && tab) {
+      render_in<1, LIFT, VDF>(p, out, world, pal, f, op, tid, nt, colA, sv, planar, tile);
       return;
     }
     typedef MC_GLOBAL float gf32;
@@ -1694,50 +1732,62 @@ __device__ __forceinline__ float spawn_of(const Params& p, int b) {
 // ---------------------------------------------------------------------------
 // Inputs of the per-env logic, loaded before the world staging completes so
 // both HBM round trips overlap.
+// Env b's inputs as loaded (load_env_in), decoded by env_update.
 struct EnvIn {
-  int32_t sc;
-  bool done;
-  uint32_t bt_lo, bt_hi;       // the 8 int8 buttons, little-endian
-  float look_yaw, look_pitch;
+  int32_t sc;                  // step_count[b] (done_mode 0)
+  uint32_t dr;                 // do_reset[b]
+  uint32_t a_lo, a_hi;         // the 8 action bytes: an int64 id, or the 8 int8 buttons
+  float look_yaw, look_pitch;  // look[b] (buttons)
   float yaw, pitch, px, py, pz, vy;
-  bool og;
+  uint32_t og;
 };
 
+// sin and cos of two angles for the price of one evaluation: a on lane 0, b
+// on lane 1, each result read back from its lane.  a and b must be the same
+// on every lane, and lanes 0 and 1 active (a whole wave in uniform flow).
+__device__ __forceinline__ void sincos_pair(float a, float b, float& sa, float& ca, float& sb,
+                                            float& cb) {
+  float s, c;
+  mc_sincosf((threadIdx.x & (kWave - 1)) == 1 ? b : a, &s, &c);
+  sa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s), 0));
+  ca = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c), 0));
+  sb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s), 1));
+  cb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c), 1));
+}
+
+// Every load of env b's inputs, unconditional and independent (issued
+// together, one memory round trip; uniform choices pick a valid address whose
+// value is then ignored).  The 8 bytes at actions + 8 b are the int64 id or
+// the 8 buttons; a base address off 4-byte alignment reads them by byte.
 __device__ __forceinline__ EnvIn load_env_in(const Params& p, int b) {
   EnvIn in;
-  in.sc = (p.done_mode == 0) ? p.step_count[b] : 0;
-  in.done = (p.done_mode == 0) ? ((p.episode_length >= 0) && (in.sc >= p.episode_length))
-                               : (p.do_reset[b] != 0);
-  in.bt_lo = 0u;
-  in.bt_hi = 0u;
-  in.look_yaw = 0.0f;
-  in.look_pitch = 0.0f;
-  if (p.action_kind == 1) {
-    // Discrete ids (docs/environment.md:3-17): 1..6 press one button.
-    // Selects rather than a branch chain; ids outside 1..10 do nothing.
-    const int64_t a64 = reinterpret_cast<const int64_t*>(p.actions)[b];
-    const int a = (a64 >= 1 && a64 <= 10) ? (int)a64 : 0;
-    in.bt_lo = (a >= 1 && a <= 4) ? (1u << ((8 * (a - 1)) & 31)) : 0u;
-    in.bt_hi = (a == 5 || a == 6) ? (1u << ((8 * (a - 5)) & 31)) : 0u;
-    in.look_yaw = (a == 7) ? p.look_delta : (a == 8) ? -p.look_delta : 0.0f;
-    in.look_pitch = (a == 9) ? p.look_delta : (a == 10) ? -p.look_delta : 0.0f;
+  const bool m0 = p.done_mode == 0;
+  in.sc = *(m0 ? p.step_count + b : reinterpret_cast<const int32_t*>(p.yaws + b));
+  in.dr = p.do_reset[b];
+  const uint8_t* src = reinterpret_cast<const uint8_t*>(p.actions) + (int64_t)b * 8;
+  if (((uintptr_t)p.actions & 3u) == 0) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(src);
+    in.a_lo = w[0];
+    in.a_hi = w[1];
   } else {
-    const uint8_t* src = reinterpret_cast<const uint8_t*>(p.actions) + (int64_t)b * 8;
+    in.a_lo = in.a_hi = 0u;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      in.bt_lo |= (uint32_t)src[k] << (8 * k);
-      in.bt_hi |= (uint32_t)src[4 + k] << (8 * k);
+      in.a_lo |= (uint32_t)src[k] << (8 * k);
+      in.a_hi |= (uint32_t)src[4 + k] << (8 * k);
     }
-    in.look_yaw = p.look[b * 2 + 0];
-    in.look_pitch = p.look[b * 2 + 1];
   }
+  const bool lk = p.action_kind != 1 && p.look;
+  const float* lp = lk ? p.look + (int64_t)b * 2 : p.positions + (int64_t)b * 3;
+  in.look_yaw = lp[0];
+  in.look_pitch = lp[1];
   in.yaw = p.yaws[b];
   in.pitch = p.pitches[b];
   in.px = p.positions[b * 3 + 0];
   in.py = p.positions[b * 3 + 1];
   in.pz = p.positions[b * 3 + 2];
   in.vy = p.velocities[b * 3 + 1];
-  in.og = p.on_ground[b] != 0;
+  in.og = p.on_ground[b];
   return in;
 }
 
@@ -1747,17 +1797,32 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
                                             int8_t* gworld, bool any_done, const EnvIn& in,
                                             bool upper_solid, float spawn_y) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int sc = in.sc;
-  const bool done = in.done;
+  const int sc = (p.done_mode == 0) ? in.sc : 0;
+  const bool done = (p.done_mode == 0) ? ((p.episode_length >= 0) && (sc >= p.episode_length))
+                                       : ((in.dr & 0xffu) != 0u);
+  // Buttons and look: from the discrete id (docs/environment.md:3-17; 1..6
+  // press one button, 7..10 turn, other ids do nothing), by selects rather
+  // than a branch chain, or as given.
+  uint32_t bt_lo = in.a_lo, bt_hi = in.a_hi;
+  float look_yaw = in.look_yaw, look_pitch = in.look_pitch;
+  if (p.action_kind == 1) {
+    const int64_t a64 = (int64_t)(((uint64_t)in.a_hi << 32) | in.a_lo);
+    const int a = (a64 >= 1 && a64 <= 10) ? (int)a64 : 0;
+    bt_lo = (a >= 1 && a <= 4) ? (1u << ((8 * (a - 1)) & 31)) : 0u;
+    bt_hi = (a == 5 || a == 6) ? (1u << ((8 * (a - 5)) & 31)) : 0u;
+    look_yaw = (a == 7) ? p.look_delta : (a == 8) ? -p.look_delta : 0.0f;
+    look_pitch = (a == 9) ? p.look_delta : (a == 10) ? -p.look_delta : 0.0f;
+  } else if (!p.look) {
+    look_yaw = look_pitch = 0.0f;
+  }
 
   // Decode (src/kernels.cu:1477-1503).
   int8_t bt[8];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    bt[k] = (int8_t)(in.bt_lo >> (8 * k));
-    bt[4 + k] = (int8_t)(in.bt_hi >> (8 * k));
+    bt[k] = (int8_t)(bt_lo >> (8 * k));
+    bt[4 + k] = (int8_t)(bt_hi >> (8 * k));
   }
-  const float look_yaw = in.look_yaw, look_pitch = in.look_pitch;
   const float fwd = (float)bt[0] - (float)bt[1];
   const float str = (float)bt[3] - (float)bt[2];
   const bool jump = bt[4] != 0, brk = bt[5] != 0, place = bt[6] != 0;
@@ -1767,13 +1832,13 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
   float yaw = in.yaw + look_yaw;
   const float new_pitch = in.pitch + look_pitch;
   float pitch = fminf(fmaxf(new_pitch, -1.56f), 1.56f);
-  float sin_yaw, cos_yaw;
-  mc_sincosf(yaw, &sin_yaw, &cos_yaw);
+  float sin_yaw, cos_yaw, sin_pitch, cos_pitch;
+  sincos_pair(yaw, pitch, sin_yaw, cos_yaw, sin_pitch, cos_pitch);
   const float speed = p.walk_speed * speed_mult;
   const float move_x = sin_yaw * fwd * speed * p.dt + cos_yaw * str * p.walk_speed * p.dt;
   const float move_z = cos_yaw * fwd * speed * p.dt - sin_yaw * str * p.walk_speed * p.dt;
   float vel_y = in.vy;
-  bool og = in.og;
+  bool og = (in.og & 0xffu) != 0u;
   if (jump && og) vel_y = p.jump_vel;
   vel_y += p.gravity * p.dt;
   float nx = in.px + move_x;
@@ -1861,11 +1926,7 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
 
   // Raycast: sample k of 32 on lane k, first solid in-bounds sample by ballot.
   float reward = 0.0f;
-  float sin_pitch = 0.0f, cos_pitch = 1.0f;
-  bool have_pitch = false;
   if (brk || place) {
-    mc_sincosf(pitch, &sin_pitch, &cos_pitch);
-    have_pitch = true;
     const float dx = sin_yaw * cos_pitch, dy = sin_pitch, dz = cos_yaw * cos_pitch;
     const float ex = nx, ey = ny + 1.6f, ez = nz;
     bool inb = false, solid = false;
@@ -1962,15 +2023,17 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
   // respawn, where sin/cos are exactly 0 and 1).
   if (done) {
     sin_yaw = 0.0f; cos_yaw = 1.0f; sin_pitch = 0.0f; cos_pitch = 1.0f;
-  } else if (!have_pitch) {
-    mc_sincosf(pitch, &sin_pitch, &cos_pitch);
   }
   return make_frame(p, nx, ny, nz, sin_yaw, cos_yaw, sin_pitch, cos_pitch, world);
 }
 
+// Both tables are read by every thread at clamped indices before either is
+// stored: one memory round trip.
 __device__ __forceinline__ void load_palette(float* pal, uint32_t* flat, int tid) {
-  if (tid < kPaletteLen) pal[tid] = kPaletteTab[tid];
-  if (tid < kFlatLen) flat[tid] = kFlatTab[tid];
+  const float pv = kPaletteTab[min(tid, kPaletteLen - 1)];
+  const uint32_t fv = kFlatTab[min(tid, kFlatLen - 1)];
+  if (tid < kPaletteLen) pal[tid] = pv;
+  if (tid < kFlatLen) flat[tid] = fv;
 }
 
 // Wave-0 logic of MineEnv.reset for env b (minecu/__init__.py:270-302):
@@ -2010,8 +2073,7 @@ __device__ __forceinline__ Frame reset_update(const Params& p, int slot, int b,
     p.cameras[b * 5 + 4] = pitch;
   }
   float sy, cy, sp, cp;
-  mc_sincosf(yaw, &sy, &cy);
-  mc_sincosf(pitch, &sp, &cp);
+  sincos_pair(yaw, pitch, sy, cy, sp, cp);
   return make_frame(p, x, y, z, sy, cy, sp, cp, world);
 }
 
@@ -2036,8 +2098,7 @@ __device__ __forceinline__ Frame camera_update(const Params& p, int b, const int
     yaw = p.cameras[b * 5 + 3]; pitch = p.cameras[b * 5 + 4];
   }
   float sy, cy, sp, cp;
-  mc_sincosf(yaw, &sy, &cy);
-  mc_sincosf(pitch, &sp, &cp);
+  sincos_pair(yaw, pitch, sy, cy, sp, cp);
   return make_frame(p, x, y, z, sy, cy, sp, cp, world);
 }
 
@@ -2074,8 +2135,26 @@ __host__ __device__ __forceinline__ int world_stride_bytes(int X, int Y, int Z) 
 __host__ __device__ __forceinline__ int world_margin_bytes(int X, int Z) {
   return (X * Z + 15) & ~15;
 }
-__host__ __device__ __forceinline__ int item_lds_bytes(int epw, int X, int Y, int Z) {
-  return epw * world_stride_bytes(X, Y, Z) + 2 * world_margin_bytes(X, Z);
+// The item's u and v tables (fill_uv: W then H floats) when the image fits
+// them.  They lie at the start of the item's LDS, in the leading margin (its
+// bytes are only ever read by a walk one step out of the world, and then
+// discarded), which grows to hold them and the column tables when it is
+// smaller.
+__host__ __device__ __forceinline__ int uv_table_bytes(int W, int H) {
+  return (W <= kMaxUV && H <= kMaxUV) ? ((W + H) * 4 + 15) & ~15 : 0;
+}
+// After them, per env of the item, the column terms of its camera rays
+// (render_in's colA: W float pairs).
+__host__ __device__ __forceinline__ int lead_bytes(int epw, int X, int Z, int W, int H) {
+  const int u = uv_table_bytes(W, H);
+  const int t = u ? u + epw * W * 8 : 0;
+  const int m = world_margin_bytes(X, Z);
+  return m > t ? m : t;
+}
+__host__ __device__ __forceinline__ int item_lds_bytes(int epw, int X, int Y, int Z, int W,
+                                                       int H) {
+  return lead_bytes(epw, X, Z, W, H) + epw * world_stride_bytes(X, Y, Z) +
+         world_margin_bytes(X, Z);
 }
 
 // upper: one word per wave of the workgroup (staged-height check).
@@ -2106,17 +2185,25 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   const int nstage = staged ? p.X * p.ys * p.Z : nvox;
   int8_t* gworld = p.voxels + (int64_t)b * nvox;
   int8_t* world = GLB ? gworld
-                      : lds + world_margin_bytes(p.X, p.Z) +
+                      : lds + lead_bytes(epw, p.X, p.Z, p.W, p.H) +
                             e * world_stride_bytes(p.X, staged ? p.ys : p.Y, p.Z);
+  if (!GLB) {
+    su = reinterpret_cast<float*>(lds);
+    sv = su + p.W;
+  }
   const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
-  bool any_done = false;
+  // The flag word is read here and tested after the world is staged, so its
+  // load overlaps the env's other loads.
+  uint32_t ring_word = 0u;
   if (KIND == kItemStep && p.done_mode == 0 && p.ring) {
-    any_done = p.ring[slot] != 0u;
+    ring_word = p.ring[slot];
     if (b == 0 && et == 0) p.ring[(slot + 2) % 3] = 0u;
   }
   const bool regen = (KIND == kItemResetRender) && (p.ring[3] != 0u);
+  // Every lane loads (env 0's inputs when its env is past the batch): no
+  // divergent region whose end would wait for the loads.
   EnvIn in;
-  if (KIND == kItemStep && active && et < kWave) in = load_env_in(p, b);
+  if (KIND == kItemStep) in = load_env_in(p, active ? b : 0);
   // Flat images for steps and camera renders of the flat modes; shaded
   // uint8 for their reset and state renders.
   const bool flat_kind = (KIND == kItemStep || KIND == kItemRenderCameras);
@@ -2136,6 +2223,7 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     if ((tid & (kWave - 1)) == 0) upper[wave] = w ? 1 : 0;
   }
   __syncthreads();
+  const bool any_done = ring_word != 0u;
   if (active && et < kWave) {
     bool upper_solid = false;
     if (staged)
@@ -2156,9 +2244,23 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   }
   __syncthreads();
   const bool go = active && !frames[e].deferred;
+  // Column terms of the env's camera rays for render_in (shaded renders
+  // with u and v tables), published by the barrier below.
+  const bool coltab = !GLB && uvm == kUvShaded && uv_tabled(p);
+  float2* colA = coltab ? reinterpret_cast<float2*>(lds + uv_table_bytes(p.W, p.H)) + e * p.W
+                        : nullptr;
+  if (coltab && go) {
+    const float fx = frames[e].fx, fz = frames[e].fz, rx = frames[e].rx, rz = frames[e].rz;
+    for (int px = et; px < p.W; px += ent) {
+      const float u = su[px];
+      colA[px] = make_float2(fx + u * rx, fz + u * rz);
+    }
+  }
   if (LIFT) {
     if (go && frames[e].lift)
       solid_extent(p, world, &frames[e], (int)floorf(frames[e].ey), et, ent);
+    __syncthreads();
+  } else if (coltab) {
     __syncthreads();
   }
   // An eye inside a solid block sees that block in every pixel: a constant
@@ -2195,10 +2297,11 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
     const OctPos op{frames[e].pos0};
     const bool f32 = obs_mode == (CHW ? kObsF32Chw : kObsF32);
     if (F32 && p.vd_far)
-      render_env<1, LIFT, true, CNT, GLB, true>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
+      render_env<1, LIFT, true, CNT, GLB, true>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc,
+                                                tile, colA);
     else if (F32)
       render_env<1, LIFT, false, CNT, GLB, true>(p, b, world, pal, f, op, et, ent, su, sv, CHW,
-                                                 acc);
+                                                 acc, tile, colA);
     else if (uvm != kUvShaded && obs_mode == kObsU8Fp16)
       render_env_flat<LIFT, _Float16, CNT, GLB>(p, b, world, flat, f, op, uvm, et, ent, su, sv,
                                                 acc, tile);
@@ -2211,10 +2314,10 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
       render_env<1, LIFT, false, CNT, GLB>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc);
     else if (p.vd_far)
       render_env<0, LIFT, true, CNT, GLB>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc,
-                                          tile);
+                                          tile, colA);
     else
       render_env<0, LIFT, false, CNT, GLB>(p, b, world, pal, f, op, et, ent, su, sv, CHW, acc,
-                                           tile);
+                                           tile, colA);
     if (!GLB && KIND == kItemStep && any_done)
       store_fresh_world<TER>(p, b, gworld, nullptr, nvox, et, ent);
   }
@@ -2249,12 +2352,12 @@ __global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
   __shared__ float s_pal[kPaletteLen];
   __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
-  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
   __shared__ int s_upper[NT / kWave];
   load_palette(s_pal, s_flat, threadIdx.x);
   const int epw = (NT / kWave) / wpe;
   run_item<NT, KIND, LIFT, CHW, false, false, CNT, false, F32>(
-      p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_flat, s_frames, s_u, s_v, s_upper);
+      p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_flat, s_frames, nullptr, nullptr,
+      s_upper);
 }
 
 // Full-height pass after a staged-height step launch (Params::ys): the envs
@@ -2268,7 +2371,6 @@ __global__ __launch_bounds__(NT) void k_env_fb(Params p, int obs_mode) {
   __shared__ float s_pal[kPaletteLen];
   __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
-  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
   __shared__ int s_upper[NT / kWave];
   const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
   uint32_t* list = p.fb + (size_t)slot * (p.B + 2);
@@ -2278,7 +2380,7 @@ __global__ __launch_bounds__(NT) void k_env_fb(Params p, int obs_mode) {
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     __syncthreads();  // the previous env is done with LDS
     run_item<NT, kItemStep, LIFT, CHW>(p, obs_mode, (int)list[2 + i], NT / kWave, s_lds, s_pal,
-                                       s_flat, s_frames, s_u, s_v, s_upper);
+                                       s_flat, s_frames, nullptr, nullptr, s_upper);
   }
   if (threadIdx.x == 0) {
     __threadfence();
@@ -2298,12 +2400,11 @@ __global__ __launch_bounds__(NT) void k_env_terrain(Params p, int obs_mode, int 
   __shared__ float s_pal[kPaletteLen];
   __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
-  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
   __shared__ int s_upper[NT / kWave];
   load_palette(s_pal, s_flat, threadIdx.x);
   const int epw = (NT / kWave) / wpe;
   run_item<NT, KIND, LIFT, false, true>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_flat,
-                                        s_frames, s_u, s_v, s_upper);
+                                        s_frames, nullptr, nullptr, s_upper);
 }
 
 template <int NT, bool LIFT>
@@ -2312,7 +2413,6 @@ __global__ __launch_bounds__(NT) void k_env_fb_terrain(Params p, int obs_mode) {
   __shared__ float s_pal[kPaletteLen];
   __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
-  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
   __shared__ int s_upper[NT / kWave];
   const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
   uint32_t* list = p.fb + (size_t)slot * (p.B + 2);
@@ -2322,7 +2422,7 @@ __global__ __launch_bounds__(NT) void k_env_fb_terrain(Params p, int obs_mode) {
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     __syncthreads();
     run_item<NT, kItemStep, LIFT, false, true>(p, obs_mode, (int)list[2 + i], NT / kWave, s_lds,
-                                               s_pal, s_flat, s_frames, s_u, s_v, s_upper);
+                                               s_pal, s_flat, s_frames, nullptr, nullptr, s_upper);
   }
   if (threadIdx.x == 0) {
     __threadfence();
@@ -2341,12 +2441,11 @@ __global__ __launch_bounds__(NT) void k_env_col(Params p, int obs_mode, int wpe)
   __shared__ float s_pal[kPaletteLen];
   __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
-  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
   __shared__ int s_upper[NT / kWave];
   load_palette(s_pal, s_flat, threadIdx.x);
   const int epw = (NT / kWave) / wpe;
   run_item<NT, kItemStep, LIFT, false, TER, true>(p, obs_mode, blockIdx.x * epw, wpe, s_lds,
-                                                  s_pal, s_flat, s_frames, s_u, s_v, s_upper);
+                                                  s_pal, s_flat, s_frames, nullptr, nullptr, s_upper);
 }
 
 template <int NT, bool LIFT, bool TER>
@@ -2355,7 +2454,6 @@ __global__ __launch_bounds__(NT) void k_env_fb_col(Params p, int obs_mode) {
   __shared__ float s_pal[kPaletteLen];
   __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
-  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
   __shared__ int s_upper[NT / kWave];
   const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
   uint32_t* list = p.fb + (size_t)slot * (p.B + 2);
@@ -2365,7 +2463,7 @@ __global__ __launch_bounds__(NT) void k_env_fb_col(Params p, int obs_mode) {
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     __syncthreads();
     run_item<NT, kItemStep, LIFT, false, TER, true>(p, obs_mode, (int)list[2 + i], NT / kWave,
-                                                    s_lds, s_pal, s_flat, s_frames, s_u, s_v,
+                                                    s_lds, s_pal, s_flat, s_frames, nullptr, nullptr,
                                                     s_upper);
   }
   if (threadIdx.x == 0) {
@@ -2462,7 +2560,6 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
   __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
   __shared__ int s_item;
-  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
   __shared__ int s_upper[NT / kWave];
   // The entry table and the descriptors are written by the host before the
   // launch and only read here: constant address space, so uniform reads are
@@ -2504,7 +2601,7 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
     const int wpe = me.wpe;
     const int epw = (NT / kWave) / wpe;
     run_item<NT, kItemStep, LIFT, CHW, TER, COL>(p, me.obs_mode, (item - me.item_begin) * epw, wpe,
-                                                 s_lds, s_pal, s_flat, s_frames, s_u, s_v, s_upper);
+                                                 s_lds, s_pal, s_flat, s_frames, nullptr, nullptr, s_upper);
     if (!PERSIST) break;
   }
   if (PERSIST && threadIdx.x == 0) {
@@ -2529,7 +2626,6 @@ __global__ __launch_bounds__(NT) void k_multi_fb(MultiArgs a) {
   __shared__ float s_pal[kPaletteLen];
   __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
-  __shared__ float s_u[kMaxUV], s_v[kMaxUV];
   __shared__ int s_upper[NT / kWave];
   typedef __attribute__((address_space(4))) const MultiEntry CEntry;
   typedef __attribute__((address_space(4))) const uint32_t CWord;
@@ -2561,7 +2657,7 @@ __global__ __launch_bounds__(NT) void k_multi_fb(MultiArgs a) {
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
       __syncthreads();  // the previous env is done with LDS
       run_item<NT, kItemStep, LIFT, CHW, TER, COL>(p, me.obs_mode, (int)list[2 + i], NT / kWave,
-                                                   s_lds, s_pal, s_flat, s_frames, s_u, s_v,
+                                                   s_lds, s_pal, s_flat, s_frames, nullptr, nullptr,
                                                    s_upper);
     }
     if (threadIdx.x == 0) {

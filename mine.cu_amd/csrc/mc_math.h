@@ -146,13 +146,12 @@ MC_HD void sincos_d(float x, double* s_out, double* c_out) {
     r = reduce_quadrant(ab, &q);
   }
   const double sp = sin_poly(r), cp = cos_poly(r);
-  double s, c;
-  switch (q) {
-    case 0: s = sp; c = cp; break;
-    case 1: s = cp; c = -sp; break;
-    case 2: s = -sp; c = -cp; break;
-    default: s = -cp; c = sp; break;
-  }
+  // Quadrant q: (s, c) = (sp, cp), (cp, -sp), (-sp, -cp), (-cp, sp), as
+  // selects (one straight-line sequence on the GPU).
+  double s = (q & 1) ? cp : sp;
+  double c = (q & 1) ? sp : cp;
+  if (q & 2) s = -s;
+  if ((q + 1) & 2) c = -c;
   if (xb >> 31) s = -s;
   *s_out = s; *c_out = c;
 }
