@@ -155,6 +155,40 @@ def test_buttons_and_look_continuous():
     print(st.summary())
 
 
+def test_buttons_from_unaligned_buffer():
+    """Buttons whose [B, 8] buffer starts one byte past an aligned address
+    (the step reads them byte by byte then) give the same states as the same
+    buttons from an aligned buffer, and the oracle's."""
+    import minecu
+    B, T = 40, 24
+    rng = np.random.default_rng(23)
+    buttons = rng.choice(np.array([0, 0, 1, 1, 3, -1], np.int8), size=(T, B, 8))
+    look = rng.normal(0, 0.5, size=(T, B, 2)).astype(np.float32)
+    kw = dict(batch_size=B, world_size=16, render_width=32, render_height=24, obs_dtype="uint8",
+              episode_length=10)
+    a = minecu.MineEnv(device="cuda", **kw)
+    u = minecu.MineEnv(device="cuda", **kw)
+    orc = oracle.OracleEnv(**kw)
+    a.reset()
+    u.reset()
+    orc.reset()
+    raw = torch.zeros(B * 8 + 1, dtype=torch.int8, device="cuda")
+    bt_u = raw[1:].view(B, 8)
+    assert bt_u.data_ptr() % 4 != 0
+    for t in range(T):
+        bt_u.copy_(torch.from_numpy(buttons[t]))
+        lk = torch.from_numpy(look[t]).cuda()
+        oa, ra, da = a.step(torch.from_numpy(buttons[t]).cuda(), lk)
+        ou, ru, du = u.step(bt_u, lk)
+        oo, ro, do = orc.step(buttons[t], look[t])
+        for name, x, y in (("obs", ou, oa), ("rewards", ru, ra), ("dones", du, da)):
+            assert np.array_equal(x.cpu().numpy(), y.cpu().numpy()), f"t{t} {name}"
+        assert np.array_equal(ou.cpu().numpy(), oo), f"t{t} obs vs oracle"
+        assert np.array_equal(u.positions.cpu().numpy().view(np.int32),
+                              orc.positions.view(np.int32)), f"t{t} positions vs oracle"
+        assert np.array_equal(u.voxels.cpu().numpy(), orc.voxels), f"t{t} voxels vs oracle"
+
+
 @pytest.mark.parametrize("max_steps", [8, 45, 46])
 def test_step_limit_boundaries(max_steps):
     """16^3 (X + Y + Z - 3 = 45): with max_steps 46 no in-world ray can reach
