@@ -437,22 +437,21 @@ __device__ __forceinline__ void store_fresh_world(const Params& p, int b, int8_t
   }
 }
 
-// HBM -> LDS copy of one env's world: up to 8 16-byte loads per lane in
-// flight before their LDS writes.
+// HBM -> LDS copy of one env's world.  Aligned worlds go by LDS-direct loads
+// (global_load_lds_dwordx4): one wave instruction moves 64 consecutive
+// 16-byte chunks from HBM into LDS, lane i's chunk at the instruction's LDS
+// base + 16 i, with no vector register holding the data.  tid and nt count
+// whole waves (tid's wave starts at tid - lane).
 __device__ __forceinline__ void stage_world(int8_t* lds, const int8_t* gworld, int nvox, int tid,
                                             int nt) {
-  if (((nvox | (int)(uintptr_t)gworld) & 15) == 0) {
-    const guint4* g = (const guint4*)gworld;
-    v4u* l = reinterpret_cast<v4u*>(lds);
+  typedef __attribute__((address_space(3))) void lds_void;
+  if (((nvox | (int)(uintptr_t)gworld | (int)(uintptr_t)lds) & 15) == 0) {
     const int n16 = nvox >> 4;
-    for (int c0 = tid; c0 < n16; c0 += 8 * nt) {
-      // Out-of-range slots reload the last chunk (in bounds) and are not stored.
-      v4u v[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) v[k] = g[min(c0 + k * nt, n16 - 1)];
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (c0 + k * nt < n16) l[c0 + k * nt] = v[k];
+    const int lane = tid & (kWave - 1);
+    for (int c0 = __builtin_amdgcn_readfirstlane(tid - lane); c0 < n16; c0 += nt) {
+      if (c0 + lane < n16)
+        __builtin_amdgcn_global_load_lds((const void*)(gworld + (c0 + lane) * 16),
+                                         (lds_void*)(lds + c0 * 16), 16, 0, 0);
     }
   } else {
     for (int i = tid; i < nvox; i += nt) lds[i] = gworld[i];
@@ -2345,9 +2344,13 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
 
 // One group, one item per workgroup: grid = ceil(B / epw).  CHW: the planar
 // image modes, in kernels of their own.  CNT: the measurement build of the
-// step, counting ray steps into p.ray_steps.
+// step, counting ray steps into p.ray_steps.  The 256-thread step kernel is
+// held to 72 vector registers (7 waves per SIMD, 7 workgroups per CU); it
+// needs no scratch for that.
 template <int NT, int KIND, bool LIFT, bool CHW = false, bool CNT = false, bool F32 = false>
-__global__ __launch_bounds__(NT) void k_env(Params p, int obs_mode, int wpe) {
+__global__ __launch_bounds__(NT)
+__attribute__((amdgpu_waves_per_eu((NT == 256 && KIND == kItemStep) ? 7 : 1)))
+void k_env(Params p, int obs_mode, int wpe) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
   __shared__ uint32_t s_flat[kFlatLen];
