@@ -231,7 +231,8 @@ bool valid_shape(Shape s) {
 // steps 7% slower, 7% overall; at 32^3 / 64x64 1.8x.
 Shape default_shape(int W, int H, int X, int Y, int Z) {
   int wpe = ((int64_t)W * H <= 1024) ? 1 : 4;
-  const int per_cu = std::max(1, mc::kLdsPerWorkgroup / mc::item_lds_bytes(1, X, Y, Z, W, H));
+  const int per_cu =
+      std::max(1, mc::kLdsPerWorkgroup / mc::item_lds_bytes(mc::kWave, 1, X, Y, Z, W, H));
   while (wpe < 8 && per_cu * wpe < 16) wpe = (wpe == 1) ? 4 : 8;
   return Shape{wpe * mc::kWave, wpe, 1};
 }
@@ -244,7 +245,7 @@ int epw_of(Shape s) { return (s.nt / mc::kWave) / s.wpe; }
 
 // Raise wpe until the envs of one workgroup fit the LDS budget.
 Shape fit_lds(Shape s, int X, int Y, int Z, int W, int H) {
-  while (mc::item_lds_bytes(epw_of(s), X, Y, Z, W, H) > lds_budget() && s.wpe * 64 < s.nt)
+  while (mc::item_lds_bytes(s.nt, epw_of(s), X, Y, Z, W, H) > lds_budget() && s.wpe * 64 < s.nt)
     s.wpe *= 2;
   return s;
 }
@@ -269,7 +270,7 @@ int launch_env(int kind, const mc::Params& q, int obs_mode, Shape s, hipStream_t
       kind != mc::kItemRenderState)
     return fail(MC_E_UNSUPPORTED, "the MC_OBS_*_CHW modes render from MineEnv state only");
   s = fit_lds(s, p.X, p.ys, p.Z, p.W, p.H);
-  if (mc::item_lds_bytes(epw_of(s), p.X, p.ys, p.Z, p.W, p.H) > lds_budget()) {
+  if (mc::item_lds_bytes(s.nt, epw_of(s), p.X, p.ys, p.Z, p.W, p.H) > lds_budget()) {
     // Too large for LDS even at one env per workgroup: the world stays in
     // global memory (whole height; no counting build there).
     if (count) return fail(MC_E_UNSUPPORTED, "ray-step counting needs an LDS-resident world");
@@ -280,7 +281,7 @@ int launch_env(int kind, const mc::Params& q, int obs_mode, Shape s, hipStream_t
     if (e != hipSuccess) return fail(MC_E_HIP, "global-world launch: %s", hipGetErrorString(e));
     return MC_OK;
   }
-  const size_t lds = (size_t)mc::item_lds_bytes(epw_of(s), p.X, p.ys, p.Z, p.W, p.H);
+  const size_t lds = (size_t)mc::item_lds_bytes(s.nt, epw_of(s), p.X, p.ys, p.Z, p.W, p.H);
   const mcl::KernelSet& ks = mcl::kernels(s.nt, s.lift != 0);
   const hipError_t e = count ? ks.env_count(p, obs_mode, s.wpe, lds, st)
                              : ks.env(kind, p, obs_mode, s.wpe, lds, st);
@@ -293,7 +294,10 @@ int launch_env(int kind, const mc::Params& q, int obs_mode, Shape s, hipStream_t
 int launch_deferred(const mc::Params& q, int obs_mode, int lift, hipStream_t st) {
   mc::Params p = q;
   p.ys = p.Y;
-  const int full = mc::item_lds_bytes(1, p.X, p.Y, p.Z, p.W, p.H);
+  // 512 threads for worlds above 80 KiB (the workgroup's tile buffers sit in
+  // its LDS trail, so the size follows the width).
+  const int nt = mc::item_lds_bytes(256, 1, p.X, p.Y, p.Z, p.W, p.H) > 80 * 1024 ? 512 : 256;
+  const int full = mc::item_lds_bytes(nt, 1, p.X, p.Y, p.Z, p.W, p.H);
   if (full > lds_budget()) {
     // The whole world does not fit LDS: the deferred envs run from global
     // memory.
@@ -302,7 +306,6 @@ int launch_deferred(const mc::Params& q, int obs_mode, int lift, hipStream_t st)
     if (e != hipSuccess) return fail(MC_E_HIP, "deferred-env launch: %s", hipGetErrorString(e));
     return MC_OK;
   }
-  const int nt = full > 80 * 1024 ? 512 : 256;
   const hipError_t e = mcl::kernels(nt, lift != 0).fb(p, obs_mode, dim3(std::min(p.B, 64)),
                                                       (size_t)full, st);
   if (e != hipSuccess) return fail(MC_E_HIP, "deferred-env launch: %s", hipGetErrorString(e));
@@ -1244,13 +1247,13 @@ int mc_batch_step(mc_batch* b, mc_env* const* envs, const void* const* actions,
       s = fit_lds(s, c.world_x, ys, c.world_z, c.width, c.height);
       const int stride = mc::world_stride_bytes(c.world_x, c.world_y, c.world_z);
       const int item_lds =
-          mc::item_lds_bytes(epw_of(s), c.world_x, ys, c.world_z, c.width, c.height);
+          mc::item_lds_bytes(s.nt, epw_of(s), c.world_x, ys, c.world_z, c.width, c.height);
       if (item_lds > lds_budget())
         return fail(MC_E_UNSUPPORTED, "group %d: world does not fit the LDS budget", i);
       f.lds = std::max(f.lds, item_lds);
       if (envs[i]->fb) {
         const int full =
-            mc::item_lds_bytes(1, c.world_x, c.world_y, c.world_z, c.width, c.height);
+            mc::item_lds_bytes(512, 1, c.world_x, c.world_y, c.world_z, c.width, c.height);
         if (full > lds_budget())
           return fail(MC_E_UNSUPPORTED, "group %d: world does not fit the LDS budget", i);
         f.lds_full = std::max(f.lds_full, full);

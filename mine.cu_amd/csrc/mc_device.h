@@ -31,8 +31,9 @@ typedef MC_GLOBAL v4u guint4;
 typedef MC_GLOBAL uint32_t guint32;
 typedef MC_GLOBAL v4f gfloat4;
 // LDS of one workgroup: gfx950 lets a workgroup allocate up to 160 KiB.
-// kStaticLdsReserve bounds the kernels' static LDS (palette, colour table,
-// frames, u/v tables, work-item bookkeeping); the rest holds worlds.  The host
+// kStaticLdsReserve bounds the kernels' static LDS (palette, frames,
+// work-item bookkeeping; the global-world kernel's tables); the rest holds
+// the items' LDS (item_lds_bytes).  The host
 // also caps the world bytes by the device's reported per-workgroup maximum.
 constexpr int kLdsPerWorkgroup = 160 * 1024;
 constexpr int kStaticLdsReserve = 8192;
@@ -1153,7 +1154,8 @@ __device__ __forceinline__ bool uv_tabled(const Params& p) { return p.W <= kMaxU
 __device__ __forceinline__ void fill_uv(const Params& p, int m, float* su, float* sv, int tid,
                                         int nt) {
   if (!uv_tabled(p)) return;
-  for (int i = tid; i < p.W; i += nt) su[i] = u_of(p, m, i);
+  if (su)
+    for (int i = tid; i < p.W; i += nt) su[i] = u_of(p, m, i);
   for (int i = tid; i < p.H; i += nt) sv[i] = v_of(p, m, i);
 }
 
@@ -1242,6 +1244,8 @@ __device__ __forceinline__ void render_rgb8(const Params& p, int b, int uvm, int
                                             bool planar = false, uint32_t* tile = nullptr) {
   uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * p.W * p.H * 3;
   const bool tab = uv_tabled(p);
+  // No u table (su null): u from its formula.
+  const bool tab_u = tab && su != nullptr;
   if (!planar && tile && ((p.W & 7) == 0) && (((uintptr_t)out & 3u) == 0)) {
     const int lane = tid & (kWave - 1), nwv = nt >> 6;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1260,7 +1264,7 @@ __device__ __forceinline__ void render_rgb8(const Params& p, int b, int uvm, int
     for (int t = wv; t < ntiles; t += nwv) {
       const int py = tr * 8 + ty, px = tc * 8 + tx;
       uint32_t c = 0u;
-      if (py < p.H) c = color(tab ? su[px] : u_of(p, uvm, px), tab ? sv[py] : v_of(p, uvm, py));
+      if (py < p.H) c = color(tab_u ? su[px] : u_of(p, uvm, px), tab ? sv[py] : v_of(p, uvm, py));
       tile[lane] = c;
       __builtin_amdgcn_wave_barrier();
       if ((lane < 48) & (tr * 8 + r < p.H)) {
@@ -1278,7 +1282,7 @@ __device__ __forceinline__ void render_rgb8(const Params& p, int b, int uvm, int
     return;
   }
   for_tiles(p, tid, nt, [&](int px, int py) {
-    const uint32_t c = color(tab ? su[px] : u_of(p, uvm, px), tab ? sv[py] : v_of(p, uvm, py));
+    const uint32_t c = color(tab_u ? su[px] : u_of(p, uvm, px), tab ? sv[py] : v_of(p, uvm, py));
     if (planar) {
       typedef MC_GLOBAL uint8_t gu8;
       const int npix = p.W * p.H;
@@ -1645,7 +1649,8 @@ __device__ __forceinline__ void render_env(const Params& p, int b, const int8_t*
                                            const float2* colA = nullptr) {
   const int W = p.W, npix = W * p.H;
   const bool tab = uv_tabled(p);
-  auto ucol = [&](int px) { return tab ? su[px] : u_of(p, kUvShaded, px); };
+  const bool tab_u = tab && su != nullptr;  // no u table: u from its formula
+  auto ucol = [&](int px) { return tab_u ? su[px] : u_of(p, kUvShaded, px); };
   auto vrow = [&](int py) { return tab ? sv[py] : v_of(p, kUvShaded, py); };
   if (OBS == 0) {
     uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
@@ -2028,11 +2033,13 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
 
 // Both tables are read by every thread at clamped indices before either is
 // stored: one memory round trip.
+// flat: null in the kernels whose items keep the flat colours in their LDS
+// lead (run_item).
 __device__ __forceinline__ void load_palette(float* pal, uint32_t* flat, int tid) {
   const float pv = kPaletteTab[min(tid, kPaletteLen - 1)];
   const uint32_t fv = kFlatTab[min(tid, kFlatLen - 1)];
   if (tid < kPaletteLen) pal[tid] = pv;
-  if (tid < kFlatLen) flat[tid] = fv;
+  if (flat && tid < kFlatLen) flat[tid] = fv;
 }
 
 // Wave-0 logic of MineEnv.reset for env b (minecu/__init__.py:270-302):
@@ -2128,32 +2135,41 @@ enum ObsMode {
 __host__ __device__ __forceinline__ int world_stride_bytes(int X, int Y, int Z) {
   return (X * Y * Z + 15) & ~15;
 }
-// A workgroup's worlds sit between two margins of one y-layer each, so the
-// in-world DDA may read the voxel one step past its world's edge (which it
-// then discards) without an address select.
+// Item LDS: [lead][world 0] ... [world epw-1][trail].  The lead and the
+// trail are at least one y-layer each, so the in-world DDA may read the voxel
+// one step past its world's edge (which it then discards) without an address
+// select; the tables that live there are written before the render and only
+// ever read by such a walk as bytes it discards.
 __host__ __device__ __forceinline__ int world_margin_bytes(int X, int Z) {
   return (X * Z + 15) & ~15;
 }
-// The item's u and v tables (fill_uv: W then H floats) when the image fits
-// them.  They lie at the start of the item's LDS, in the leading margin (its
-// bytes are only ever read by a walk one step out of the world, and then
-// discarded), which grows to hold them and the column tables when it is
-// smaller.
-__host__ __device__ __forceinline__ int uv_table_bytes(int W, int H) {
-  return (W <= kMaxUV && H <= kMaxUV) ? ((W + H) * 4 + 15) & ~15 : 0;
+__host__ __device__ __forceinline__ int round16(int n) { return (n + 15) & ~15; }
+__host__ __device__ __forceinline__ bool uv_fits(int W, int H) {
+  return W <= kMaxUV && H <= kMaxUV;
 }
-// After them, per env of the item, the column terms of its camera rays
-// (render_in's colA: W float pairs).
+// The lead holds, when the image fits the u and v tables (fill_uv), either
+// - for the shaded images: the v table (H floats), then per env of the item
+//   the column terms of its camera rays (render_in's colA: W float pairs),
+//   with u taken from its formula where a render needs it; or
+// - for the other images: the u table (W floats), the v table (H floats),
+//   then the flat colours (kFlatLen words, the flat images' colour table).
 __host__ __device__ __forceinline__ int lead_bytes(int epw, int X, int Z, int W, int H) {
-  const int u = uv_table_bytes(W, H);
-  const int t = u ? u + epw * W * 8 : 0;
+  const int uv = uv_fits(W, H) ? round16(4 * (W + H)) : 0;
+  const int flat = uv + round16(4 * kFlatLen);
+  const int col = uv_fits(W, H) ? round16(4 * H) + epw * W * 8 : 0;
   const int m = world_margin_bytes(X, Z);
+  const int t = flat > col ? flat : col;
   return m > t ? m : t;
 }
-__host__ __device__ __forceinline__ int item_lds_bytes(int epw, int X, int Y, int Z, int W,
-                                                       int H) {
-  return lead_bytes(epw, X, Z, W, H) + epw * world_stride_bytes(X, Y, Z) +
-         world_margin_bytes(X, Z);
+// The trail holds the per-wave tile buffers of the uint8 render (64 words
+// per wave of the nt-thread workgroup).
+__host__ __device__ __forceinline__ int trail_bytes(int nt, int X, int Z) {
+  const int m = world_margin_bytes(X, Z), t = (nt / kWave) * kWave * 4;
+  return m > t ? m : t;
+}
+__host__ __device__ __forceinline__ int item_lds_bytes(int nt, int epw, int X, int Y, int Z,
+                                                       int W, int H) {
+  return lead_bytes(epw, X, Z, W, H) + epw * world_stride_bytes(X, Y, Z) + trail_bytes(nt, X, Z);
 }
 
 // upper: one word per wave of the workgroup (staged-height check).
@@ -2183,12 +2199,42 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   const bool staged = !GLB && (KIND == kItemStep) && p.ys < p.Y;
   const int nstage = staged ? p.X * p.ys * p.Z : nvox;
   int8_t* gworld = p.voxels + (int64_t)b * nvox;
-  int8_t* world = GLB ? gworld
-                      : lds + lead_bytes(epw, p.X, p.Z, p.W, p.H) +
-                            e * world_stride_bytes(p.X, staged ? p.ys : p.Y, p.Z);
-  if (!GLB) {
-    su = reinterpret_cast<float*>(lds);
-    sv = su + p.W;
+  const int lead = lead_bytes(epw, p.X, p.Z, p.W, p.H);
+  const int wstride = world_stride_bytes(p.X, staged ? p.ys : p.Y, p.Z);
+  int8_t* world = GLB ? gworld : lds + lead + e * wstride;
+  // Flat images for steps and camera renders of the flat modes; shaded
+  // uint8 for their reset and state renders.
+  const bool flat_kind = (KIND == kItemStep || KIND == kItemRenderCameras);
+  const int uvm = (KIND == kItemRenderBasis) ? kUvRaw
+                  : (flat_kind && (obs_mode == kObsU8Minimal || obs_mode == kObsU8Fp16))
+                      ? kUvMinimal
+                  : (flat_kind && obs_mode == kObsU8Prebasis) ? kUvBasis
+                                                              : kUvShaded;
+  const bool flat_img = (uvm != kUvShaded);
+  // Shaded renders with u and v tables take their camera rays' column terms
+  // from LDS (render_in's colA, filled below).
+  const bool coltab = !GLB && uvm == kUvShaded && uv_tabled(p);
+  // Tables in the lead (lead_bytes) and the per-wave tile buffers in the
+  // trail (trail_bytes); a world in global memory keeps them in static LDS.
+  uint32_t* tile;
+  if constexpr (GLB) {
+    __shared__ uint32_t s_tile[NT / kWave][kWave];
+    tile = s_tile[threadIdx.x >> 6];
+  } else {
+    tile = reinterpret_cast<uint32_t*>(lds + lead + epw * wstride) + (tid >> 6) * kWave;
+    if (coltab) {
+      su = nullptr;
+      sv = reinterpret_cast<float*>(lds);
+    } else {
+      su = reinterpret_cast<float*>(lds);
+      sv = su + p.W;
+    }
+    if (flat_img) {
+      uint32_t* fl = reinterpret_cast<uint32_t*>(lds + (uv_tabled(p) ? round16(4 * (p.W + p.H)) : 0));
+      const uint32_t fv = kFlatTab[min(tid, kFlatLen - 1)];
+      if (tid < kFlatLen) fl[tid] = fv;
+      flat = fl;
+    }
   }
   const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
   // The flag word is read here and tested after the world is staged, so its
@@ -2203,14 +2249,6 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   // divergent region whose end would wait for the loads.
   EnvIn in;
   if (KIND == kItemStep) in = load_env_in(p, active ? b : 0);
-  // Flat images for steps and camera renders of the flat modes; shaded
-  // uint8 for their reset and state renders.
-  const bool flat_kind = (KIND == kItemStep || KIND == kItemRenderCameras);
-  const int uvm = (KIND == kItemRenderBasis) ? kUvRaw
-                  : (flat_kind && (obs_mode == kObsU8Minimal || obs_mode == kObsU8Fp16))
-                      ? kUvMinimal
-                  : (flat_kind && obs_mode == kObsU8Prebasis) ? kUvBasis
-                                                              : kUvShaded;
   fill_uv(p, uvm, su, sv, tid, NT);
   if (active) {
     if (regen) store_fresh_world<TER>(p, b, gworld, GLB ? nullptr : world, nvox, et, ent);
@@ -2245,13 +2283,11 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   const bool go = active && !frames[e].deferred;
   // Column terms of the env's camera rays for render_in (shaded renders
   // with u and v tables), published by the barrier below.
-  const bool coltab = !GLB && uvm == kUvShaded && uv_tabled(p);
-  float2* colA = coltab ? reinterpret_cast<float2*>(lds + uv_table_bytes(p.W, p.H)) + e * p.W
-                        : nullptr;
+  float2* colA = coltab ? reinterpret_cast<float2*>(lds + round16(4 * p.H)) + e * p.W : nullptr;
   if (coltab && go) {
     const float fx = frames[e].fx, fz = frames[e].fz, rx = frames[e].rx, rz = frames[e].rz;
     for (int px = et; px < p.W; px += ent) {
-      const float u = su[px];
+      const float u = u_of(p, kUvShaded, px);
       colA[px] = make_float2(fx + u * rx, fz + u * rz);
     }
   }
@@ -2265,7 +2301,6 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   // An eye inside a solid block sees that block in every pixel: a constant
   // uint8 image, written without the camera frame or the octant words.
   const int npix = p.W * p.H;
-  const bool flat_img = (uvm != kUvShaded);
   int fill = -1;
   // CHW kernels: only the planar uint8 image has a block fill (whole planes).
   const bool can_fill = CHW ? (obs_mode == kObsU8Chw) & ((npix & 15) == 0)
@@ -2273,9 +2308,6 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   if (go && ((npix * 3) & 15) == 0 && can_fill)
     fill = __builtin_amdgcn_readfirstlane(flat_img ? frames[e].flat_fill : frames[e].fill_block);
   uint32_t acc[3] = {0u, 0u, 0u};
-  // Per-wave staging of one 8x8 tile of uint8 pixels (render_rgb8).
-  __shared__ uint32_t s_tile[NT / kWave][kWave];
-  uint32_t* tile = s_tile[threadIdx.x >> 6];
   if (fill >= 0) {
     if (CNT && et == 0) acc[0] += npix, acc[2] += npix;
     const uint32_t rgb = flat_img ? flat[fill < kFlatSky ? fill : kFlatSky]
@@ -2353,13 +2385,12 @@ __attribute__((amdgpu_waves_per_eu((NT == 256 && KIND == kItemStep) ? 7 : 1)))
 void k_env(Params p, int obs_mode, int wpe) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
-  __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
   __shared__ int s_upper[NT / kWave];
-  load_palette(s_pal, s_flat, threadIdx.x);
+  load_palette(s_pal, nullptr, threadIdx.x);
   const int epw = (NT / kWave) / wpe;
   run_item<NT, KIND, LIFT, CHW, false, false, CNT, false, F32>(
-      p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_flat, s_frames, nullptr, nullptr,
+      p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, nullptr, s_frames, nullptr, nullptr,
       s_upper);
 }
 
@@ -2372,18 +2403,17 @@ template <int NT, bool LIFT, bool CHW = false>
 __global__ __launch_bounds__(NT) void k_env_fb(Params p, int obs_mode) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
-  __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
   __shared__ int s_upper[NT / kWave];
   const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
   uint32_t* list = p.fb + (size_t)slot * (p.B + 2);
   const int n = (int)min(list[0], (uint32_t)p.B);
   if (n == 0) return;  // nothing deferred: the list is already empty
-  load_palette(s_pal, s_flat, threadIdx.x);
+  load_palette(s_pal, nullptr, threadIdx.x);
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     __syncthreads();  // the previous env is done with LDS
     run_item<NT, kItemStep, LIFT, CHW>(p, obs_mode, (int)list[2 + i], NT / kWave, s_lds, s_pal,
-                                       s_flat, s_frames, nullptr, nullptr, s_upper);
+                                       nullptr, s_frames, nullptr, nullptr, s_upper);
   }
   if (threadIdx.x == 0) {
     __threadfence();
@@ -2401,12 +2431,11 @@ template <int NT, int KIND, bool LIFT>
 __global__ __launch_bounds__(NT) void k_env_terrain(Params p, int obs_mode, int wpe) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
-  __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
   __shared__ int s_upper[NT / kWave];
-  load_palette(s_pal, s_flat, threadIdx.x);
+  load_palette(s_pal, nullptr, threadIdx.x);
   const int epw = (NT / kWave) / wpe;
-  run_item<NT, KIND, LIFT, false, true>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, s_flat,
+  run_item<NT, KIND, LIFT, false, true>(p, obs_mode, blockIdx.x * epw, wpe, s_lds, s_pal, nullptr,
                                         s_frames, nullptr, nullptr, s_upper);
 }
 
@@ -2414,18 +2443,17 @@ template <int NT, bool LIFT>
 __global__ __launch_bounds__(NT) void k_env_fb_terrain(Params p, int obs_mode) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
-  __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
   __shared__ int s_upper[NT / kWave];
   const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
   uint32_t* list = p.fb + (size_t)slot * (p.B + 2);
   const int n = (int)min(list[0], (uint32_t)p.B);
   if (n == 0) return;
-  load_palette(s_pal, s_flat, threadIdx.x);
+  load_palette(s_pal, nullptr, threadIdx.x);
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     __syncthreads();
     run_item<NT, kItemStep, LIFT, false, true>(p, obs_mode, (int)list[2 + i], NT / kWave, s_lds,
-                                               s_pal, s_flat, s_frames, nullptr, nullptr, s_upper);
+                                               s_pal, nullptr, s_frames, nullptr, nullptr, s_upper);
   }
   if (threadIdx.x == 0) {
     __threadfence();
@@ -2442,31 +2470,29 @@ template <int NT, bool LIFT, bool TER>
 __global__ __launch_bounds__(NT) void k_env_col(Params p, int obs_mode, int wpe) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
-  __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
   __shared__ int s_upper[NT / kWave];
-  load_palette(s_pal, s_flat, threadIdx.x);
+  load_palette(s_pal, nullptr, threadIdx.x);
   const int epw = (NT / kWave) / wpe;
   run_item<NT, kItemStep, LIFT, false, TER, true>(p, obs_mode, blockIdx.x * epw, wpe, s_lds,
-                                                  s_pal, s_flat, s_frames, nullptr, nullptr, s_upper);
+                                                  s_pal, nullptr, s_frames, nullptr, nullptr, s_upper);
 }
 
 template <int NT, bool LIFT, bool TER>
 __global__ __launch_bounds__(NT) void k_env_fb_col(Params p, int obs_mode) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
-  __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
   __shared__ int s_upper[NT / kWave];
   const int slot = p.slot_dev ? (int)*p.slot_dev : p.slot;
   uint32_t* list = p.fb + (size_t)slot * (p.B + 2);
   const int n = (int)min(list[0], (uint32_t)p.B);
   if (n == 0) return;
-  load_palette(s_pal, s_flat, threadIdx.x);
+  load_palette(s_pal, nullptr, threadIdx.x);
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     __syncthreads();
     run_item<NT, kItemStep, LIFT, false, TER, true>(p, obs_mode, (int)list[2 + i], NT / kWave,
-                                                    s_lds, s_pal, s_flat, s_frames, nullptr, nullptr,
+                                                    s_lds, s_pal, nullptr, s_frames, nullptr, nullptr,
                                                     s_upper);
   }
   if (threadIdx.x == 0) {
@@ -2560,7 +2586,6 @@ template <int NT, bool LIFT, bool PERSIST, bool CHW = false, bool TER = false, b
 __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
-  __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
   __shared__ int s_item;
   __shared__ int s_upper[NT / kWave];
@@ -2570,7 +2595,7 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
   typedef __attribute__((address_space(4))) const MultiEntry CEntry;
   typedef __attribute__((address_space(4))) const uint32_t CWord;
   CEntry* entries = (CEntry*)a.entries;
-  load_palette(s_pal, s_flat, threadIdx.x);
+  load_palette(s_pal, nullptr, threadIdx.x);
   // First item of group `lane` (past the last group: never <= an item).
   const int lane = threadIdx.x & (kWave - 1);
   const int my_begin = (lane < a.n_groups) ? entries[lane].item_begin : 0x7fffffff;
@@ -2604,7 +2629,7 @@ __global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
     const int wpe = me.wpe;
     const int epw = (NT / kWave) / wpe;
     run_item<NT, kItemStep, LIFT, CHW, TER, COL>(p, me.obs_mode, (item - me.item_begin) * epw, wpe,
-                                                 s_lds, s_pal, s_flat, s_frames, nullptr, nullptr, s_upper);
+                                                 s_lds, s_pal, nullptr, s_frames, nullptr, nullptr, s_upper);
     if (!PERSIST) break;
   }
   if (PERSIST && threadIdx.x == 0) {
@@ -2627,7 +2652,6 @@ template <int NT, bool LIFT, bool CHW = false, bool TER = false, bool COL = fals
 __global__ __launch_bounds__(NT) void k_multi_fb(MultiArgs a) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
-  __shared__ uint32_t s_flat[kFlatLen];
   __shared__ Frame s_frames[NT / kWave];
   __shared__ int s_upper[NT / kWave];
   typedef __attribute__((address_space(4))) const MultiEntry CEntry;
@@ -2648,7 +2672,7 @@ __global__ __launch_bounds__(NT) void k_multi_fb(MultiArgs a) {
     const int n = (int)min(list[0], (uint32_t)p.B);
     if (n == 0) continue;
     if (!palette) {
-      load_palette(s_pal, s_flat, threadIdx.x);
+      load_palette(s_pal, nullptr, threadIdx.x);
       palette = true;
     }
     p.actions = me.actions;
@@ -2660,7 +2684,7 @@ __global__ __launch_bounds__(NT) void k_multi_fb(MultiArgs a) {
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
       __syncthreads();  // the previous env is done with LDS
       run_item<NT, kItemStep, LIFT, CHW, TER, COL>(p, me.obs_mode, (int)list[2 + i], NT / kWave,
-                                                   s_lds, s_pal, s_flat, s_frames, nullptr, nullptr,
+                                                   s_lds, s_pal, nullptr, s_frames, nullptr, nullptr,
                                                    s_upper);
     }
     if (threadIdx.x == 0) {
