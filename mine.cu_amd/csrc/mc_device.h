@@ -2376,12 +2376,17 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
 
 // One group, one item per workgroup: grid = ceil(B / epw).  CHW: the planar
 // image modes, in kernels of their own.  CNT: the measurement build of the
-// step, counting ray steps into p.ray_steps.  The 256-thread step kernel is
-// held to 72 vector registers (7 waves per SIMD, 7 workgroups per CU); it
-// needs no scratch for that.
+// step, counting ray steps into p.ray_steps.
+// Register caps: gfx950 runs 7 waves per SIMD only when a wave holds at most
+// 96 scalar registers in total (measured: 94 gives 7, 102 gives 6, whatever
+// the compiler's occupancy estimate says), so the step kernels are held to 96
+// (the excess spills to vector-register lanes, as before); the 256-thread
+// step kernel is also held to 72 vector registers (7 waves per SIMD, 7
+// workgroups per CU at 32^3), with no scratch.
 template <int NT, int KIND, bool LIFT, bool CHW = false, bool CNT = false, bool F32 = false>
 __global__ __launch_bounds__(NT)
 __attribute__((amdgpu_waves_per_eu((NT == 256 && KIND == kItemStep) ? 7 : 1)))
+__attribute__((amdgpu_num_sgpr(96)))
 void k_env(Params p, int obs_mode, int wpe) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
@@ -2428,7 +2433,8 @@ __global__ __launch_bounds__(NT) void k_env_fb(Params p, int obs_mode) {
 // k_env and k_env_fb for the seeded terrain world (interleaved images only):
 // kernels of their own, so the flat-world kernels keep their code.
 template <int NT, int KIND, bool LIFT>
-__global__ __launch_bounds__(NT) void k_env_terrain(Params p, int obs_mode, int wpe) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_num_sgpr(96)))
+void k_env_terrain(Params p, int obs_mode, int wpe) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
   __shared__ Frame s_frames[NT / kWave];
@@ -2467,7 +2473,8 @@ __global__ __launch_bounds__(NT) void k_env_fb_terrain(Params p, int obs_mode) {
 
 // Step kernels with voxel collision (flat or seeded terrain world, TER).
 template <int NT, bool LIFT, bool TER>
-__global__ __launch_bounds__(NT) void k_env_col(Params p, int obs_mode, int wpe) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_num_sgpr(96)))
+void k_env_col(Params p, int obs_mode, int wpe) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
   __shared__ Frame s_frames[NT / kWave];
