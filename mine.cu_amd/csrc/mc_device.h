@@ -2589,8 +2589,12 @@ static_assert(sizeof(float) * (kPaletteLen + 2 * kMaxUV) + sizeof(uint32_t) * kF
 // the kernel needs fewer registers: 73 VGPRs against 118 on gfx950).
 // CHW / TER / COL: a launch over groups of one mode family (planar images,
 // seeded terrain, voxel collision), as the single-group kernels separate them.
+// Register caps as for k_env: 96 scalar registers; the one-item form also 72
+// vector registers (7 waves per SIMD).
 template <int NT, bool LIFT, bool PERSIST, bool CHW = false, bool TER = false, bool COL = false>
-__global__ __launch_bounds__(NT) void k_multi(MultiArgs a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_num_sgpr(96)))
+__attribute__((amdgpu_waves_per_eu(PERSIST ? 1 : 7)))
+void k_multi(MultiArgs a) {
   extern __shared__ __align__(16) int8_t s_lds[];
   __shared__ float s_pal[kPaletteLen];
   __shared__ Frame s_frames[NT / kWave];
