@@ -60,6 +60,24 @@ def test_flat_env_matches_oracle(mode, ws, W, H):
     assert st.exact, st.summary()
 
 
+@pytest.mark.parametrize("mode", ("uint8", "float32", "uint8_minimal", "uint8_prebasis"))
+def test_images_wider_than_the_lds_tables(mode):
+    """136 x 130 images: both sides above the 128-entry u and v tables, so u and
+    v come from their formulas, the flat colours sit at the start of the item's
+    LDS lead and the shaded renders keep no column terms; 17 tile columns
+    (interleaved uint8 tile stores)."""
+    B, T = 6, 14
+    env, orc = _envs(batch_size=B, world_size=16, render_width=136, render_height=130,
+                     obs_dtype=mode, episode_length=9)
+    st = Stats()
+    env.reset()
+    orc.reset()
+    compare_env(env, orc, st, f"{mode} reset")
+    run_pair(env, orc, _break_heavy(T, B, 23), st, f"{mode} 136x130")
+    _basis_equal(env, orc, mode)
+    assert st.exact, st.summary()
+
+
 @pytest.mark.parametrize("mode", ("uint8_minimal", "uint8_prebasis", "uint8_fp16"))
 def test_flat_eye_outside_world_and_odd_image(mode):
     """9^3 world: the spawn eye (y 11.6) is above the world, so rays start
