@@ -1333,9 +1333,10 @@ __device__ __forceinline__ void render_env_flat(const Params& p, int b, const in
 }
 
 // ---------------------------------------------------------------------------
-// The shaded uint8 render of an env whose eye voxel lies in the world
-// (trace()'s in-world branch for every pixel), with u and v from the tables
-// and the tile stores of render_rgb8: the same arithmetic as trace(), laid out
+// The shaded render of an env whose eye voxel lies in the world (trace()'s
+// in-world branch for every pixel), with the per-column ray terms and the v
+// table from LDS and the tile stores of render_rgb8 (in the float32 and
+// planar kernels, per-pixel stores): the same arithmetic as trace(), laid out
 // for instruction count.  The env's camera frame, face distances and address
 // steps are held in vector registers for the whole image (as scalars they
 // outnumber the scalar file and are reloaded per pixel); the ray set-up tests
