@@ -2587,7 +2587,7 @@ static_assert(sizeof(float) * (kPaletteLen + 2 * kMaxUV) + sizeof(uint32_t) * kF
 // PERSIST: a grid sized to the resident capacity whose workgroups loop over
 // items taken from an atomic counter; otherwise one workgroup per item (the
 // loop is compiled out; the hardware dispatcher then balances the load, and
-// the kernel needs fewer registers: 73 VGPRs against 118 on gfx950).
+// the kernel needs fewer registers: 72 VGPRs against 111 on gfx950).
 // CHW / TER / COL: a launch over groups of one mode family (planar images,
 // seeded terrain, voxel collision), as the single-group kernels separate them.
 // Register caps as for k_env: 96 scalar registers; the one-item form also 72
