@@ -565,8 +565,39 @@ __device__ __forceinline__ void store_basis(const Params& p, int b, const Frame&
   bs[11] = p.flat_su; bs[12] = p.flat_hf;
 }
 
+// Wave-wide minima of xmin and zmin and maxima of xmax, zmax and ymax, by DPP
+// lane moves (no LDS round trip): pairs and quads by quad permutes, half rows
+// and rows by mirrors (every lane has a source), then rows 0-1 into row 1 and
+// rows 0-3 into row 3 by row broadcasts; the results are read from lane 63.
+// The broadcasts leave rows 0 and 2 undefined, which lane 63 never reads.  The
+// five reductions advance together, so no move reads a register written by
+// the instruction just before it.  Every lane must be active.
+__device__ __forceinline__ void wave_extent(int& xmin, int& xmax, int& zmin, int& zmax,
+                                            int& ymax) {
+#define MC_EXT_STEP(CTRL, ROWS)                                                   \
+  xmin = min(xmin, __builtin_amdgcn_mov_dpp(xmin, CTRL, ROWS, 0xf, false));       \
+  zmin = min(zmin, __builtin_amdgcn_mov_dpp(zmin, CTRL, ROWS, 0xf, false));       \
+  xmax = max(xmax, __builtin_amdgcn_mov_dpp(xmax, CTRL, ROWS, 0xf, false));       \
+  zmax = max(zmax, __builtin_amdgcn_mov_dpp(zmax, CTRL, ROWS, 0xf, false));       \
+  ymax = max(ymax, __builtin_amdgcn_mov_dpp(ymax, CTRL, ROWS, 0xf, false));
+  MC_EXT_STEP(0xb1, 0xf)   // quad_perm 1,0,3,2
+  MC_EXT_STEP(0x4e, 0xf)   // quad_perm 2,3,0,1
+  MC_EXT_STEP(0x141, 0xf)  // row_half_mirror
+  MC_EXT_STEP(0x140, 0xf)  // row_mirror
+  MC_EXT_STEP(0x142, 0xa)  // row_bcast:15 into rows 1 and 3
+  MC_EXT_STEP(0x143, 0xc)  // row_bcast:31 into rows 2 and 3
+#undef MC_EXT_STEP
+  xmin = __builtin_amdgcn_readlane(xmin, 63);
+  zmin = __builtin_amdgcn_readlane(zmin, 63);
+  xmax = __builtin_amdgcn_readlane(xmax, 63);
+  zmax = __builtin_amdgcn_readlane(zmax, 63);
+  ymax = __builtin_amdgcn_readlane(ymax, 63);
+}
+
 // Extent of the solid voxels at heights >= y0 (16-byte LDS chunks; Z % 16 ==
-// 0), reduced over the env's threads into f (LDS) with LDS atomics.
+// 0), reduced over the env's threads into f (LDS) with LDS atomics.  Chunk c
+// = (y X + x) zc + zq; a lane's first chunk and its stride are split into
+// (y, x, zq) by shifts when X and zc are powers of two, else by division.
 __device__ __forceinline__ void solid_extent(const Params& p, const int8_t* world, Frame* f,
                                              int y0, int tid, int nt) {
   const int X = p.X, Y = p.ys, zc = p.Z >> 4;
@@ -575,11 +606,26 @@ __device__ __forceinline__ void solid_extent(const Params& p, const int8_t* worl
   if (y0 < Y) {
     const v4u* w4 = reinterpret_cast<const v4u*>(world);
     const int n16 = Y * X * zc;
-    // chunk c = (y * X + x) * zc + zq, walked with per-lane stride nt
     const int c_begin = y0 * X * zc + tid;
-    int zq = c_begin % zc, yx = c_begin / zc;
-    int x = yx % X, y = yx / X;
-    const int dzq = nt % zc, dyx = nt / zc, dx = dyx % X, dy = dyx / X;
+    int zq, x, y, dzq, dx, dy;
+    if (((X & (X - 1)) | (zc & (zc - 1))) == 0) {
+      const int lz = __builtin_ctz(zc), lx = __builtin_ctz(X);
+      zq = c_begin & (zc - 1);
+      x = (c_begin >> lz) & (X - 1);
+      y = c_begin >> (lz + lx);
+      dzq = nt & (zc - 1);
+      dx = (nt >> lz) & (X - 1);
+      dy = nt >> (lz + lx);
+    } else {
+      zq = c_begin % zc;
+      const int yx = c_begin / zc;
+      x = yx % X;
+      y = yx / X;
+      dzq = nt % zc;
+      const int dyx = nt / zc;
+      dx = dyx % X;
+      dy = dyx / X;
+    }
     for (int c = c_begin; c < n16; c += nt) {
       const v4u w = w4[c];
       const uint32_t m0 = ~w.x & 0x80808080u, m1 = ~w.y & 0x80808080u;
@@ -603,14 +649,7 @@ __device__ __forceinline__ void solid_extent(const Params& p, const int8_t* worl
       if (x >= X) { x -= X; ++y; }
     }
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    xmin = min(xmin, __shfl_xor(xmin, o));
-    xmax = max(xmax, __shfl_xor(xmax, o));
-    zmin = min(zmin, __shfl_xor(zmin, o));
-    zmax = max(zmax, __shfl_xor(zmax, o));
-    ymax = max(ymax, __shfl_xor(ymax, o));
-  }
+  wave_extent(xmin, xmax, zmin, zmax, ymax);
   if ((tid & (kWave - 1)) == 0) {
     atomicMin(&f->hx0, xmin);
     atomicMax(&f->hx1, xmax);
