@@ -1993,7 +1993,7 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
     const uint64_t inb_mask = __ballot(inb);
     if (hit) {
       const int k = __ffsll((unsigned long long)hit) - 1;
-      const int8_t hit_blk = (int8_t)__shfl((int)blk, k);
+      const int8_t hit_blk = (int8_t)__builtin_amdgcn_readlane((int)blk, k);  // k is uniform
       if (brk) {
         if (p.target_block < 0 || (int)hit_blk == p.target_block) reward = p.reward_value;
         if (lane == k) {
