@@ -1841,21 +1841,27 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
                                             int8_t* gworld, bool any_done, const EnvIn& in,
                                             bool upper_solid, float spawn_y) {
   const int lane = threadIdx.x & (kWave - 1);
+  // Uniform parameters read once, as values the register allocator cannot
+  // rematerialise: it would otherwise reload them from the kernel arguments
+  // (a scalar-cache round trip) at each use.
+  int episode_length = p.episode_length, action_kind = p.action_kind;
+  float look_delta = p.look_delta;
+  asm volatile("" : "+s"(episode_length), "+s"(action_kind), "+s"(look_delta));
   const int sc = (p.done_mode == 0) ? in.sc : 0;
-  const bool done = (p.done_mode == 0) ? ((p.episode_length >= 0) && (sc >= p.episode_length))
+  const bool done = (p.done_mode == 0) ? ((episode_length >= 0) && (sc >= episode_length))
                                        : ((in.dr & 0xffu) != 0u);
   // Buttons and look: from the discrete id (docs/environment.md:3-17; 1..6
   // press one button, 7..10 turn, other ids do nothing), by selects rather
   // than a branch chain, or as given.
   uint32_t bt_lo = in.a_lo, bt_hi = in.a_hi;
   float look_yaw = in.look_yaw, look_pitch = in.look_pitch;
-  if (p.action_kind == 1) {
+  if (action_kind == 1) {
     const int64_t a64 = (int64_t)(((uint64_t)in.a_hi << 32) | in.a_lo);
     const int a = (a64 >= 1 && a64 <= 10) ? (int)a64 : 0;
     bt_lo = (a >= 1 && a <= 4) ? (1u << ((8 * (a - 1)) & 31)) : 0u;
     bt_hi = (a == 5 || a == 6) ? (1u << ((8 * (a - 5)) & 31)) : 0u;
-    look_yaw = (a == 7) ? p.look_delta : (a == 8) ? -p.look_delta : 0.0f;
-    look_pitch = (a == 9) ? p.look_delta : (a == 10) ? -p.look_delta : 0.0f;
+    look_yaw = (a == 7) ? look_delta : (a == 8) ? -look_delta : 0.0f;
+    look_pitch = (a == 9) ? look_delta : (a == 10) ? -look_delta : 0.0f;
   } else if (!p.look) {
     look_yaw = look_pitch = 0.0f;
   }
@@ -2044,7 +2050,7 @@ __device__ __forceinline__ Frame env_update(const Params& p, int slot, int b, in
       // L2 and is written back when the launch ends; an agent-scope atomic
       // store writes through, and 32K envs setting the same word in one step
       // serialised for about 0.7 ms.
-      if (p.ring && p.episode_length >= 0 && nsc >= p.episode_length)
+      if (p.ring && episode_length >= 0 && nsc >= episode_length)
         p.ring[(slot + 1) % 3] = 1u;
     }
     if (p.forward_in) {
