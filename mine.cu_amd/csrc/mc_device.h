@@ -1229,6 +1229,31 @@ __device__ __forceinline__ void fill_planes8(uint8_t* out, int npix, uint32_t rg
   }
 }
 
+// Constant-colour fill of env b's float32 image (c0, c1, c2) with 16-byte
+// stores: interleaved, chunk c holds floats 4c .. 4c+3, whose channels start
+// at phase 4c mod 3 = c mod 3; planar, each plane is one value repeated.
+// Needs npix % 4 == 0 and a 16-byte aligned image.
+__device__ __forceinline__ void fill_f32(float* out, int npix, float c0, float c1, float c2,
+                                         bool planar, int tid, int nt) {
+  gfloat4* o4 = (gfloat4*)out;
+  const int n16 = (npix * 3) >> 2;
+  if (planar) {
+    const int per = npix >> 2;
+    for (int c = tid; c < n16; c += nt) {
+      const float v = (c < per) ? c0 : (c < 2 * per) ? c1 : c2;
+      o4[c] = v4f{v, v, v, v};
+    }
+  } else {
+    for (int c = tid; c < n16; c += nt) {
+      const int ph = c % 3;
+      const float a = (ph == 0) ? c0 : (ph == 1) ? c1 : c2;
+      const float bq = (ph == 0) ? c1 : (ph == 1) ? c2 : c0;
+      const float d = (ph == 0) ? c2 : (ph == 1) ? c0 : c1;
+      o4[c] = v4f{a, bq, d, a};
+    }
+  }
+}
+
 // Pixels are traced in 8x8 tiles, one pixel per lane of a wave per tile, and
 // a wave's loop over a tile ends when its longest ray ends: neighbouring
 // pixels have rays of similar length, so square tiles waste fewer lane-steps
@@ -2349,19 +2374,27 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   const int npix = p.W * p.H;
   int fill = -1;
   // CHW kernels: only the planar uint8 image has a block fill (whole planes).
-  const bool can_fill = CHW ? (obs_mode == kObsU8Chw) & ((npix & 15) == 0)
-                            : (flat_img || obs_mode != kObsF32);
-  if (go && ((npix * 3) & 15) == 0 && can_fill)
+  // Float32 images fill with float stores (16-byte aligned images only).
+  const bool f32_img = !flat_img && (obs_mode == kObsF32 || obs_mode == kObsF32Chw);
+  const bool can_fill = f32_img ? ((npix & 3) == 0) & (((uintptr_t)p.obs & 15u) == 0)
+                                : (((npix * 3) & 15) == 0) & (!CHW | ((npix & 15) == 0));
+  if (go && can_fill)
     fill = __builtin_amdgcn_readfirstlane(flat_img ? frames[e].flat_fill : frames[e].fill_block);
   uint32_t acc[3] = {0u, 0u, 0u};
   if (fill >= 0) {
     if (CNT && et == 0) acc[0] += npix, acc[2] += npix;
-    const uint32_t rgb = flat_img ? flat[fill < kFlatSky ? fill : kFlatSky]
-                                  : to_u8(pal[fill * 3]) | (to_u8(pal[fill * 3 + 1]) << 8) |
-                                        (to_u8(pal[fill * 3 + 2]) << 16);
-    uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
-    if (CHW) fill_planes8(out, npix, rgb, et, ent);
-    else fill_rgb8(out, npix, rgb, et, ent);
+    if (f32_img) {
+      // trace()'s colour for the eye's block at t = 0, face -1: col * 1 * 1 + sky * 0.
+      float* out = reinterpret_cast<float*>(p.obs) + (int64_t)b * npix * 3;
+      fill_f32(out, npix, pal[fill * 3], pal[fill * 3 + 1], pal[fill * 3 + 2], CHW, et, ent);
+    } else {
+      const uint32_t rgb = flat_img ? flat[fill < kFlatSky ? fill : kFlatSky]
+                                    : to_u8(pal[fill * 3]) | (to_u8(pal[fill * 3 + 1]) << 8) |
+                                          (to_u8(pal[fill * 3 + 2]) << 16);
+      uint8_t* out = reinterpret_cast<uint8_t*>(p.obs) + (int64_t)b * npix * 3;
+      if (CHW) fill_planes8(out, npix, rgb, et, ent);
+      else fill_rgb8(out, npix, rgb, et, ent);
+    }
     if (!GLB && KIND == kItemStep && any_done)
       store_fresh_world<TER>(p, b, gworld, nullptr, nvox, et, ent);
   } else if (go) {
