@@ -2374,8 +2374,12 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   const int npix = p.W * p.H;
   int fill = -1;
   // CHW kernels: only the planar uint8 image has a block fill (whole planes).
-  // Float32 images fill with float stores (16-byte aligned images only).
-  const bool f32_img = !flat_img && (obs_mode == kObsF32 || obs_mode == kObsF32Chw);
+  // Float32 images fill with float stores (16-byte aligned images only).  The
+  // interleaved uint8 step kernel never sees a float32 mode (those run the F32
+  // and CHW kernels), so it compiles without that path.
+  constexpr bool kMayF32 = F32 || CHW || TER || COL || GLB || KIND != kItemStep;
+  const bool f32_img =
+      kMayF32 && !flat_img && (obs_mode == kObsF32 || obs_mode == kObsF32Chw);
   const bool can_fill = f32_img ? ((npix & 3) == 0) & (((uintptr_t)p.obs & 15u) == 0)
                                 : (((npix * 3) & 15) == 0) & (!CHW | ((npix & 15) == 0));
   if (go && can_fill)
