@@ -2258,6 +2258,10 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   // A 64-thread workgroup is one wave: one env per item, one wave per env
   // (a compile-time constant there, so every per-env stride folds).
   if (NT == kWave) wpe = 1;
+  // The per-env phases before the render (loads, env logic, extent scan) are
+  // serial latency on the item's path; their waves issue ahead of rendering
+  // waves (wave priority, lowered again before the render).
+  __builtin_amdgcn_s_setprio(3);
   const int tid = threadIdx.x, wave = tid / kWave;
   const int epw = (NT / kWave) / wpe;
   const int e = wave / wpe;                                 // env slot within the item
@@ -2385,6 +2389,7 @@ __device__ __forceinline__ void run_item(const Params& p, int obs_mode, int b0, 
   if (go && can_fill)
     fill = __builtin_amdgcn_readfirstlane(flat_img ? frames[e].flat_fill : frames[e].fill_block);
   uint32_t acc[3] = {0u, 0u, 0u};
+  __builtin_amdgcn_s_setprio(0);
   if (fill >= 0) {
     if (CNT && et == 0) acc[0] += npix, acc[2] += npix;
     if (f32_img) {
