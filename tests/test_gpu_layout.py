@@ -32,7 +32,7 @@ def _check(env, orc, where):
 
 
 @pytest.mark.parametrize("dtype", ["uint8", "float32"])
-@pytest.mark.parametrize("ws,W,H", [(16, 32, 32), (32, 30, 20), (16, 24, 8)])
+@pytest.mark.parametrize("ws,W,H", [(16, 32, 32), (32, 30, 20), (16, 24, 8), (16, 17, 11)])
 def test_planar_observations_match_oracle(dtype, ws, W, H):
     import minecu
     B = 48

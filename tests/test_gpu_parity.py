@@ -138,6 +138,19 @@ def test_float32_observations():
     print(st.summary())
 
 
+def test_float32_constant_images_of_odd_size():
+    """17 x 11 float32 images (187 pixels, not a multiple of 4): constant images
+    of an eye inside a block take the per-pixel path instead of the 16-byte
+    fill; landed agents sit in dirt for most of the episode."""
+    env, orc = _envs(batch_size=32, world_size=16, render_width=17, render_height=11,
+                     obs_dtype="float32", episode_length=25)
+    st = Stats()
+    env.reset()
+    orc.reset()
+    run_pair(env, orc, _discrete(40, 32, 8), st, "f32 odd")
+    assert st.exact, st.summary()
+
+
 def test_buttons_and_look_continuous():
     """Multi-hot buttons (including non-0/1 int8 values) and continuous look
     with large yaw drift, non-square 48x40 image, 32^3 world, placement."""
