@@ -459,14 +459,22 @@ __device__ __forceinline__ void stage_world(int8_t* lds, const int8_t* gworld, i
   }
 }
 
-// Whether any of the n voxels at g (this thread's share) is not air.
+// Whether any of the n voxels at g (this thread's share) is not air.  Four
+// 16-byte loads per lane are in flight before the first test (out-of-range
+// slots reload the last chunk, in bounds, and are not tested), so at 32^3
+// the scan adds no round trip to the staging loads'.
 __device__ __forceinline__ bool any_not_air(const int8_t* g, int n, int tid, int nt) {
   bool found = false;
   if ((((uintptr_t)g | (uintptr_t)n) & 15) == 0) {
     const guint4* g4 = (const guint4*)g;
-    for (int c = tid; c < (n >> 4); c += nt) {
-      const v4u w = g4[c];
-      found |= (w.x & w.y & w.z & w.w) != 0xffffffffu;
+    const int n16 = n >> 4;
+    for (int c0 = tid; c0 < n16; c0 += 4 * nt) {
+      v4u w[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) w[k] = g4[min(c0 + k * nt, n16 - 1)];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        found |= (c0 + k * nt < n16) & ((w[k].x & w[k].y & w[k].z & w[k].w) != 0xffffffffu);
     }
   } else {
     for (int i = tid; i < n; i += nt) found |= g[i] != kAir;
