@@ -140,10 +140,12 @@ def preroll_steps(w, warmup):
 
 
 def run_gpu(w, obs, steps, warmup, layout="hwc", terrain="flat", collision=False,
-            count_rays=False):
+            count_rays=False, per_launch=False):
     """Reset, warmup + preroll untimed steps, then `steps` timed steps.
     count_rays: instead of timing, run the ray-step counting build of the step
-    over the same window and return its counters."""
+    over the same window and return its counters.  per_launch: also record a
+    HIP event pair around every step launch (their records lengthen each step,
+    so a throughput figure comes from a run without them)."""
     import torch
     import minecu
     dev = torch.device("cuda", 0)
@@ -169,21 +171,25 @@ def run_gpu(w, obs, steps, warmup, layout="hwc", terrain="flat", collision=False
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream(dev)
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(steps)]
+          for _ in range(steps if per_launch else 0)]
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t_wall = time.perf_counter()
     start.record(stream)
-    for t in range(steps):
-        ev[t][0].record(stream)
-        env.step(acts[warmup + t])
-        ev[t][1].record(stream)
+    if per_launch:
+        for t in range(steps):
+            ev[t][0].record(stream)
+            env.step(acts[warmup + t])
+            ev[t][1].record(stream)
+    else:
+        for t in range(steps):
+            env.step(acts[warmup + t])
     end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall
     total_ms = start.elapsed_time(end)
-    per_launch_ms = [a.elapsed_time(b) for a, b in ev]
+    per_launch_ms = [a.elapsed_time(b) for a, b in ev] if per_launch else None
     env.close()
     return total_ms, wall, per_launch_ms
 
@@ -455,8 +461,13 @@ def main():
         return
     w = WORKLOADS[args.workload]
     obs_bytes = 1 if args.obs == "uint8" else 4
-    total_ms, wall, per_launch = run_gpu(w, args.obs, args.steps, args.warmup,
-                                         terrain=args.terrain, collision=args.collision)
+    # The timed steps, then the same window again (a fresh env, the same
+    # actions) with an event pair around every step launch for the
+    # per-launch durations.
+    total_ms, wall, _ = run_gpu(w, args.obs, args.steps, args.warmup, terrain=args.terrain,
+                                collision=args.collision)
+    inst_ms, _, per_launch = run_gpu(w, args.obs, args.steps, args.warmup, terrain=args.terrain,
+                                     collision=args.collision, per_launch=True)
     K, B = args.steps, w["batch"]
     pre = preroll_steps(w, args.warmup)
     # The reference's configuration: the flat world and its ground plane.
@@ -493,7 +504,11 @@ def main():
                                    "a done step with its world regeneration is timed)"},
         "wall_s": wall,
         "step_launch_ms": {"avg": avg_launch_ms, "p50": per_launch[len(per_launch) // 2],
-                           "p99": per_launch[min(len(per_launch) - 1, int(0.99 * len(per_launch)))]},
+                           "p99": per_launch[min(len(per_launch) - 1, int(0.99 * len(per_launch)))],
+                           "pass": "a second pass over the same window (fresh env, same actions) "
+                                   "with a HIP event pair around each step launch on its stream; "
+                                   "value and ms_per_step come from the pass without them",
+                           "instrumented_ms_per_step": inst_ms / K},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, args.obs) if ref_world else None,
@@ -525,7 +540,7 @@ def main():
         out["config_length_run"] = {"steps": n_full, "value": B * n_full / (f_ms / 1e3),
                                     "ms_per_step": f_ms / n_full}
     if args.obs == "uint8" and not args.no_f32:
-        f_ms, _, f_launch = run_gpu(w, "float32", min(K, 200), args.warmup)
+        f_ms, _, _ = run_gpu(w, "float32", min(K, 200), args.warmup)
         out["float32_obs_value"] = B * min(K, 200) / (f_ms / 1e3)
         if args.workload == "c2":
             out["float32_obs_vs_baseline"] = out["float32_obs_value"] / REFERENCE_C2
