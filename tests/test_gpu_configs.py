@@ -2,7 +2,8 @@
 
 * the plain step at batch 32768 (16^3, 32x32) and 4096 (32^3, 64x64, the
   break-heavy action mix) through a done step with regeneration, 256 sampled
-  envs compared with the oracle fed those envs' actions;
+  envs compared with the oracle fed those envs' actions, and over each
+  workload's configured length (1000 and 500 steps) with 64 sampled envs;
 * the batched workload (8 uneven groups, seed 3) stepped in one launch per
   step after autotuning: every group equals stepping it alone, bit for bit,
   through a done step, and 64 sampled envs per group equal the oracle;
@@ -70,6 +71,43 @@ def test_c2_full_batch_sampled_against_oracle():
 
 def test_c3_full_batch_sampled_against_oracle():
     _plain("c3")
+
+
+@pytest.mark.parametrize("workload", ["c2", "c3"])
+def test_configured_length_sampled_against_oracle(workload):
+    """The plain step over the workload's configured length (C2: 1000 steps,
+    about 16 episodes; C3: 500 steps), 64 sampled envs against the oracle:
+    rewards and dones every step; state, decode buffers and images on every
+    step either side of a done step, on the done step itself and every 50th
+    step."""
+    import minecu
+    w = bench.WORKLOADS[workload]
+    B, n_steps, ep = w["batch"], w["config_steps"], w["episode_length"]
+    kw = dict(world_size=w["world"], render_width=w["W"], render_height=w["H"],
+              obs_dtype="uint8", episode_length=ep)
+    env = minecu.MineEnv(batch_size=B, device="cuda", **kw)
+    acts = bench.make_actions(w, n_steps)           # CPU generator, the bench's seed
+    idx = _sample(B, 64, 200 + w["seed"])
+    orc = oracle.OracleEnv(batch_size=len(idx), **kw)
+    env.reset()
+    orc.reset()
+    a_dev = acts.cuda()
+    ti = torch.as_tensor(idx, device="cuda")
+    st = Stats()
+    n_done = 0
+    for t in range(n_steps):
+        _, r, d = env.step(a_dev[t])
+        _, o_r, o_d = orc.step(acts[t].numpy()[idx])
+        where = f"{workload} step {t}"
+        assert np.array_equal(r.index_select(0, ti).cpu().numpy(), o_r), f"{where}: rewards"
+        g_d = d.index_select(0, ti).cpu().numpy().astype(bool)
+        assert np.array_equal(g_d, o_d), f"{where}: dones"
+        n_done += int(g_d.all())
+        if t % ep in (ep - 1, 0, 1) or t % 50 == 0:
+            compare_sampled(env, idx, orc, st, where)
+    assert n_done == n_steps // ep, "a done step every episode_length steps"
+    print(workload, n_steps, "steps", st.summary())
+    assert st.exact, st.summary()
 
 
 def _group_env(c, **extra):
