@@ -57,7 +57,7 @@ def _bind(L):
     L.orc_fmix32.restype = u
     L.orc_terrain_column.argtypes = [i, i, i, i, i, u]
     L.orc_terrain_heights.argtypes = [P, i, i, i, i, i, u]
-    L.orc_terrain_world.argtypes = [P, i, i, i, i, i, i, u, i, i, i, i]
+    L.orc_terrain_world.argtypes = [P, i, i, i, i, i, i, u, i, i, i, i, P]
     L.orc_reset.argtypes = [P, P, P, P, P, P, i, f, f, f]
     L.orc_discrete_to_buttons.argtypes = [P, i, f, P, P]
     L.orc_decode_actions.argtypes = [P, P, P, P, P, P, P, P, P, P, i, f]
@@ -131,15 +131,20 @@ def half_fov_tan(fov_degrees: float) -> float:
 
 
 def fresh_world(B, X, Y, Z, ground_height, tree_xz, house_xz, seed=42,
-                terrain_amplitude=None) -> np.ndarray:
+                terrain_amplitude=None, env_ids=None) -> np.ndarray:
     """generate_world + place_tree + place_house, the reference's regen order
     (minecu/__init__.py:258-268).  terrain_amplitude: the seeded noise terrain
-    of that amplitude instead (None: the reference's flat world)."""
+    of that amplitude instead (None: the reference's flat world), seeded by
+    each world's env index (env_ids: an int32 array of B indices, or None for
+    0 .. B-1)."""
     v = np.empty((B, Y, X, Z), dtype=np.int8)
     L = lib()
     if terrain_amplitude is not None:
+        ids = None if env_ids is None else np.ascontiguousarray(env_ids, dtype=np.int32)
+        assert ids is None or ids.shape == (B,)
         L.orc_terrain_world(_p(v), B, X, Y, Z, ground_height, terrain_amplitude,
-                            seed & 0xFFFFFFFF, tree_xz[0], tree_xz[1], house_xz[0], house_xz[1])
+                            seed & 0xFFFFFFFF, tree_xz[0], tree_xz[1], house_xz[0], house_xz[1],
+                            None if ids is None else _p(ids))
         return v
     L.orc_generate_world(_p(v), B, X, Y, Z, ground_height, seed)
     L.orc_place_tree(_p(v), B, X, Y, Z, tree_xz[0], tree_xz[1], ground_height)
@@ -248,9 +253,16 @@ class OracleEnv:
                  fov_degrees=70.0, dt=0.05, gravity=-20.0, walk_speed=4.0, sprint_mult=1.5,
                  jump_vel=8.0, ground_height=8, target_block=OAKLOG, reward_value=1.0,
                  episode_length=None, seed=42, obs_dtype="uint8", look_delta=0.1,
-                 nthreads=None, terrain="flat", terrain_amplitude=3, collision=False):
+                 nthreads=None, terrain="flat", terrain_amplitude=3, collision=False,
+                 env_ids=None):
+        """env_ids: the env index of each env (default 0 .. B-1).  The seeded
+        terrain and its spawn heights depend on it, so a few envs of a large
+        batch are stepped alone by passing their indices."""
         assert obs_dtype in _FLAT, obs_dtype
         self.batch_size = B = batch_size
+        self.env_ids = (np.arange(B, dtype=np.int32) if env_ids is None
+                        else np.ascontiguousarray(env_ids, dtype=np.int32))
+        assert self.env_ids.shape == (B,)
         self.world_x = world_x if world_x is not None else world_size
         self.world_y = world_y if world_y is not None else world_size
         self.world_z = world_z if world_z is not None else world_size
@@ -277,8 +289,8 @@ class OracleEnv:
         self.spawn_ys = np.full(B, self.spawn_y, np.float32)
         if self.terrain_amplitude is not None:
             for b in range(B):
-                h = terrain_column(b, int(self.spawn_x), int(self.spawn_z), ground_height,
-                                   self.terrain_amplitude, seed)
+                h = terrain_column(int(self.env_ids[b]), int(self.spawn_x), int(self.spawn_z),
+                                   ground_height, self.terrain_amplitude, seed)
                 self.spawn_ys[b] = np.float32(self.spawn_y) + np.float32(h - ground_height)
         # State (minecu/__init__.py:214-256).
         X, Y, Z = self.world_x, self.world_y, self.world_z
@@ -310,7 +322,8 @@ class OracleEnv:
     def _fresh(self):
         return fresh_world(self.batch_size, self.world_x, self.world_y, self.world_z,
                            self.ground_height, (self.tree_x, self.tree_z),
-                           (self.house_x, self.house_z), self.seed, self.terrain_amplitude)
+                           (self.house_x, self.house_z), self.seed, self.terrain_amplitude,
+                           self.env_ids)
 
     def _regen(self):
         self.voxels[...] = self._fresh()

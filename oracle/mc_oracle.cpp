@@ -822,12 +822,16 @@ void orc_terrain_heights(int* heights, int b, int X, int Z, int ground_height, i
     }
 }
 
+// ids: the env index of each of the B worlds (the terrain is seeded by it), or
+// null for 0 .. B-1.
 void orc_terrain_world(int8_t* voxels, int B, int X, int Y, int Z, int ground_height, int amp,
-                       unsigned int seed, int tree_x, int tree_z, int house_x, int house_z) {
+                       unsigned int seed, int tree_x, int tree_z, int house_x, int house_z,
+                       const int* ids) {
   const int64_t wxyz = (int64_t)X * Y * Z;
   std::vector<int> h((size_t)X * Z);
-  for (int b = 0; b < B; ++b) {
-    int8_t* v = voxels + b * wxyz;
+  for (int k = 0; k < B; ++k) {
+    int8_t* v = voxels + k * wxyz;
+    const int b = ids ? ids[k] : k;
     orc_terrain_heights(h.data(), b, X, Z, ground_height, amp, seed);
     for (int y = 0; y < Y; ++y)
       for (int x = 0; x < X; ++x)

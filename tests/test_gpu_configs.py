@@ -4,6 +4,8 @@
   break-heavy action mix) through a done step with regeneration, 256 sampled
   envs compared with the oracle fed those envs' actions, and over each
   workload's configured length (1000 and 500 steps) with 64 sampled envs;
+* the C2 sizes on the seeded noise terrain, with and without voxel
+  collision, over 200 steps with 64 sampled envs;
 * the batched workload (8 uneven groups, seed 3) stepped in one launch per
   step after autotuning: every group equals stepping it alone, bit for bit,
   through a done step, and 64 sampled envs per group equal the oracle;
@@ -73,22 +75,20 @@ def test_c3_full_batch_sampled_against_oracle():
     _plain("c3")
 
 
-@pytest.mark.parametrize("workload", ["c2", "c3"])
-def test_configured_length_sampled_against_oracle(workload):
-    """The plain step over the workload's configured length (C2: 1000 steps,
-    about 16 episodes; C3: 500 steps), 64 sampled envs against the oracle:
-    rewards and dones every step; state, decode buffers and images on every
-    step either side of a done step, on the done step itself and every 50th
-    step."""
+def _long_run(workload, n_steps, n_sample=64, **extra):
+    """The plain step for n_steps, n_sample sampled envs against the oracle
+    stepping those env indices: rewards and dones every step; state, decode
+    buffers and images on every step either side of a done step, on the done
+    step itself and every 50th step."""
     import minecu
     w = bench.WORKLOADS[workload]
-    B, n_steps, ep = w["batch"], w["config_steps"], w["episode_length"]
+    B, ep = w["batch"], w["episode_length"]
     kw = dict(world_size=w["world"], render_width=w["W"], render_height=w["H"],
-              obs_dtype="uint8", episode_length=ep)
+              obs_dtype="uint8", episode_length=ep, **extra)
     env = minecu.MineEnv(batch_size=B, device="cuda", **kw)
     acts = bench.make_actions(w, n_steps)           # CPU generator, the bench's seed
-    idx = _sample(B, 64, 200 + w["seed"])
-    orc = oracle.OracleEnv(batch_size=len(idx), **kw)
+    idx = _sample(B, n_sample, 200 + w["seed"])
+    orc = oracle.OracleEnv(batch_size=len(idx), env_ids=idx, **kw)
     env.reset()
     orc.reset()
     a_dev = acts.cuda()
@@ -98,7 +98,7 @@ def test_configured_length_sampled_against_oracle(workload):
     for t in range(n_steps):
         _, r, d = env.step(a_dev[t])
         _, o_r, o_d = orc.step(acts[t].numpy()[idx])
-        where = f"{workload} step {t}"
+        where = f"{workload} {extra} step {t}"
         assert np.array_equal(r.index_select(0, ti).cpu().numpy(), o_r), f"{where}: rewards"
         g_d = d.index_select(0, ti).cpu().numpy().astype(bool)
         assert np.array_equal(g_d, o_d), f"{where}: dones"
@@ -106,8 +106,22 @@ def test_configured_length_sampled_against_oracle(workload):
         if t % ep in (ep - 1, 0, 1) or t % 50 == 0:
             compare_sampled(env, idx, orc, st, where)
     assert n_done == n_steps // ep, "a done step every episode_length steps"
-    print(workload, n_steps, "steps", st.summary())
+    print(workload, extra, n_steps, "steps", st.summary())
     assert st.exact, st.summary()
+
+
+@pytest.mark.parametrize("workload", ["c2", "c3"])
+def test_configured_length_sampled_against_oracle(workload):
+    """C2 over its configured 1000 steps (about 16 episodes), C3 over 500."""
+    _long_run(workload, bench.WORKLOADS[workload]["config_steps"])
+
+
+@pytest.mark.parametrize("collision", [False, True])
+def test_c2_noise_terrain_sampled_against_oracle(collision):
+    """The C2 sizes on the seeded noise terrain (amplitude 3), with and
+    without voxel collision, over 200 steps: three done steps, each
+    regenerating every world as a fresh height map."""
+    _long_run("c2", 200, terrain="noise", terrain_amplitude=3, seed=42, collision=collision)
 
 
 def _group_env(c, **extra):

@@ -125,3 +125,30 @@ def test_flat_world_spawn_is_spawn_y():
     env = oracle.OracleEnv(batch_size=3, world_size=16, render_width=4, render_height=4)
     env.reset()
     assert (env.positions[:, 1] == 10.0).all()
+
+
+@pytest.mark.parametrize("collision", [False, True])
+def test_env_ids_step_a_sample_as_the_full_batch(collision):
+    """OracleEnv(env_ids=idx) steps envs idx of a larger batch alone: the
+    seeded worlds and spawn heights follow the env index, so state and images
+    equal those rows of the full batch through done steps and their
+    regeneration."""
+    B, T = 40, 26
+    idx = np.array([0, 3, 17, 39], dtype=np.int32)
+    kw = dict(world_size=16, render_width=16, render_height=12, obs_dtype="uint8",
+              episode_length=10, terrain="noise", terrain_amplitude=4, collision=collision, seed=7)
+    full = oracle.OracleEnv(batch_size=B, **kw)
+    part = oracle.OracleEnv(batch_size=len(idx), env_ids=idx, **kw)
+    plain = oracle.OracleEnv(batch_size=len(idx), **kw)
+    for e in (full, part, plain):
+        e.reset()
+    rng = np.random.default_rng(5)
+    for t in range(T):
+        a = rng.integers(0, 11, B)
+        full.step(a)
+        part.step(a[idx])
+        plain.step(a[idx])
+        for k in ("voxels", "positions", "velocities", "obs_buffer", "rewards", "step_count"):
+            assert np.array_equal(getattr(full, k)[idx], getattr(part, k)), (t, k)
+    # without the indices, envs 1..3 of the sample get envs 1..3's worlds
+    assert not np.array_equal(plain.voxels[1:], part.voxels[1:])
