@@ -127,15 +127,19 @@ SAMPLED_KEYS = ("voxels", "positions", "velocities", "yaws", "pitches", "on_grou
                 "cameras", "rewards", "obs_buffer") + DECODE_BUFFERS
 
 
-def compare_sampled(env, idx, orc, stats: Stats, where: str):
+def compare_sampled(env, idx, orc, stats: Stats, where: str, chw: bool = False):
     """Envs idx of a (large) MineEnv against an OracleEnv that stepped exactly
     those envs (envs are independent, and episodes run in lockstep from one
-    reset, so the group-wide regeneration happens on the same steps)."""
+    reset, so the group-wide regeneration happens on the same steps).  chw:
+    the MineEnv writes planar [B, 3, H, W] images, compared transposed back to
+    the oracle's [B, H, W, 3]."""
     import torch
     ti = torch.as_tensor(idx, device=env.device)
     for k in SAMPLED_KEYS:
         g = getattr(env, k).index_select(0, ti).cpu().numpy()
         o = getattr(orc, k)
+        if k == "obs_buffer" and chw:      # compared as [B, H, W, 3] pixels
+            g = np.ascontiguousarray(g.transpose(0, 2, 3, 1))
         if k == "obs_buffer":
             compare_obs(g, o, stats, where)
         elif g.dtype in (np.float32,) and k in ("positions", "velocities", "yaws", "pitches",

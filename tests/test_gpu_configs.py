@@ -5,7 +5,8 @@
   envs compared with the oracle fed those envs' actions, and over each
   workload's configured length (1000 and 500 steps) with 64 sampled envs;
 * the C2 sizes on the seeded noise terrain, with and without voxel
-  collision, over 200 steps with 64 sampled envs;
+  collision, and with float32 and planar observations, over 200 steps with
+  64 sampled envs;
 * the batched workload (8 uneven groups, seed 3) stepped in one launch per
   step after autotuning: every group equals stepping it alone, bit for bit,
   through a done step, and 64 sampled envs per group equal the oracle;
@@ -84,8 +85,10 @@ def _long_run(workload, n_steps, n_sample=64, **extra):
     w = bench.WORKLOADS[workload]
     B, ep = w["batch"], w["episode_length"]
     kw = dict(world_size=w["world"], render_width=w["W"], render_height=w["H"],
-              obs_dtype="uint8", episode_length=ep, **extra)
-    env = minecu.MineEnv(batch_size=B, device="cuda", **kw)
+              obs_dtype="uint8", episode_length=ep)
+    kw.update(extra)
+    layout = kw.pop("obs_layout", "hwc")
+    env = minecu.MineEnv(batch_size=B, device="cuda", obs_layout=layout, **kw)
     acts = bench.make_actions(w, n_steps)           # CPU generator, the bench's seed
     idx = _sample(B, n_sample, 200 + w["seed"])
     orc = oracle.OracleEnv(batch_size=len(idx), env_ids=idx, **kw)
@@ -104,7 +107,7 @@ def _long_run(workload, n_steps, n_sample=64, **extra):
         assert np.array_equal(g_d, o_d), f"{where}: dones"
         n_done += int(g_d.all())
         if t % ep in (ep - 1, 0, 1) or t % 50 == 0:
-            compare_sampled(env, idx, orc, st, where)
+            compare_sampled(env, idx, orc, st, where, chw=layout == "chw")
     assert n_done == n_steps // ep, "a done step every episode_length steps"
     print(workload, extra, n_steps, "steps", st.summary())
     assert st.exact, st.summary()
@@ -122,6 +125,13 @@ def test_c2_noise_terrain_sampled_against_oracle(collision):
     without voxel collision, over 200 steps: three done steps, each
     regenerating every world as a fresh height map."""
     _long_run("c2", 200, terrain="noise", terrain_amplitude=3, seed=42, collision=collision)
+
+
+@pytest.mark.parametrize("obs_dtype,layout", [("float32", "hwc"), ("uint8", "chw"),
+                                              ("float32", "chw")])
+def test_c2_obs_modes_sampled_against_oracle(obs_dtype, layout):
+    """The C2 sizes with float32 and planar observations over 200 steps."""
+    _long_run("c2", 200, obs_dtype=obs_dtype, obs_layout=layout)
 
 
 def _group_env(c, **extra):
